@@ -1,0 +1,384 @@
+"""ctypes binding of libgsp_hip.so (include/gsp.h).
+
+The shared library is the product: every computation of SEQSIM and LUSIM runs
+in it on the GPU. There is no CPU fallback; loading fails loudly when the
+library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libgsp_hip.so"))
+
+GSP_OK, GSP_ERR_ARGUMENT, GSP_ERR_ASSERTION, GSP_ERR_POSDEF = 0, 1, 2, 3
+GSP_ERR_DEVICE, GSP_ERR_MEMORY, GSP_ERR_UNSUPPORTED, GSP_ERR_INTERNAL = 4, 5, 6, 7
+SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
+VARIOGRAM, COVARIANCE = 0, 1
+KNN_THEN_MASK, MASKED_KNN = 0, 1
+FP64, FP32_REFINE = 0, 1
+LU_FP64, LU_MIXED = 0, 1
+FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK = 1, 2, 4
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError (lusim.jl:46, field.jl:99)."""
+
+
+class PosDefException(np.linalg.LinAlgError):
+    """LinearAlgebra.PosDefException raised by `cholesky` (lusim.jl:92,98,103)."""
+
+
+class GspError(RuntimeError):
+    """Device, memory or internal failure of the backend."""
+
+
+class UnsupportedError(GspError):
+    """A configuration this backend does not implement."""
+
+
+class GspModel(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("sill", ctypes.c_double), ("nugget", ctypes.c_double),
+                ("ranges", ctypes.c_double * 3), ("rotation", ctypes.c_double * 9),
+                ("has_rotation", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class GspGrid(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("dims", ctypes.c_int64 * 3), ("origin", ctypes.c_double * 3),
+                ("spacing", ctypes.c_double * 3)]
+
+
+class GspSeqsimProblem(ctypes.Structure):
+    _fields_ = [("grid", GspGrid), ("model", GspModel), ("mean", ctypes.c_double),
+                ("n_data", ctypes.c_int64), ("data_coords", ctypes.c_void_p),
+                ("n_path", ctypes.c_int64), ("path", ctypes.c_void_p),
+                ("minneighbors", ctypes.c_int32), ("maxneighbors", ctypes.c_int32),
+                ("radius", ctypes.c_double), ("search_mode", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("device_pointers", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+class GspLusimProblem(ctypes.Structure):
+    _fields_ = [("grid", GspGrid), ("model", GspModel), ("mean", ctypes.c_double),
+                ("n_data", ctypes.c_int64), ("dinds", ctypes.c_void_p),
+                ("z1", ctypes.c_void_p), ("precision", ctypes.c_int32),
+                ("device_pointers", ctypes.c_int32)]
+
+
+class GspSeqsimInfo(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int64), ("n_data", ctypes.c_int64),
+                ("minneighbors", ctypes.c_int32), ("maxneighbors", ctypes.c_int32),
+                ("depth", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("alpha", ctypes.c_double), ("n_prior", ctypes.c_int64),
+                ("n_fallback", ctypes.c_int64)]
+
+
+K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_COUNT = 0, 1, 2, 3, 4, 5, 8
+
+
+class GspTimings(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * K_COUNT), ("launches", ctypes.c_int64 * K_COUNT)]
+
+
+# every symbol include/gsp.h declares, with (restype, argtypes)
+_VP, _I32, _I64, _D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+SIGNATURES = {
+    "gsp_abi_version": (_I32, []),
+    "gsp_ctx_create": (_I32, [_I32, ctypes.POINTER(_VP)]),
+    "gsp_ctx_destroy": (_I32, [_VP]),
+    "gsp_ctx_stream": (_I32, [_VP, ctypes.POINTER(_VP)]),
+    "gsp_ctx_synchronize": (_I32, [_VP]),
+    "gsp_last_error": (ctypes.c_char_p, []),
+    "gsp_ctx_enable_timing": (_I32, [_VP, _I32]),
+    "gsp_ctx_timings": (_I32, [_VP, ctypes.POINTER(GspTimings), _I32]),
+    "gsp_device_alloc": (_I32, [_VP, _I64, ctypes.POINTER(_VP)]),
+    "gsp_device_free": (_I32, [_VP, _VP]),
+    "gsp_memcpy_to_device": (_I32, [_VP, _VP, _VP, _I64]),
+    "gsp_memcpy_to_host": (_I32, [_VP, _VP, _VP, _I64]),
+    "gsp_seqsim_plan_create": (_I32, [_VP, ctypes.POINTER(GspSeqsimProblem), ctypes.POINTER(_VP)]),
+    "gsp_seqsim_plan_info": (_I32, [_VP, ctypes.POINTER(GspSeqsimInfo)]),
+    "gsp_seqsim_plan_export": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "gsp_seqsim_sweep": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP, _I32]),
+    "gsp_seqsim_plan_free": (_I32, [_VP]),
+    "gsp_lusim_factor": (_I32, [_VP, ctypes.POINTER(GspLusimProblem), ctypes.POINTER(_VP)]),
+    "gsp_lusim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
+    "gsp_lusim_dims": (_I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "gsp_lusim_export": (_I32, [_VP, _VP, _VP, _VP]),
+    "gsp_lusim_plan_free": (_I32, [_VP]),
+}
+
+# test hooks (include/gsp_testing.h)
+TESTING_SIGNATURES = {
+    "gsp_selftest_mfma": (_I32, [_VP, _I32, _VP]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libgsp_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(
+                    f"libgsp_hip.so not found at {LIB_PATH}; build it with "
+                    "`make -C geostatsprocesses.jl_amd/csrc` (hipcc, gfx950)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in {**SIGNATURES, **TESTING_SIGNATURES}.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc == GSP_OK:
+        return
+    msg = load().gsp_last_error().decode(errors="replace")
+    if rc == GSP_ERR_ARGUMENT:
+        raise ArgumentError(msg)
+    if rc == GSP_ERR_ASSERTION:
+        raise AssertionError(msg)
+    if rc == GSP_ERR_POSDEF:
+        raise PosDefException(msg)
+    if rc == GSP_ERR_UNSUPPORTED:
+        raise UnsupportedError(msg)
+    raise GspError(f"gsp status {rc}: {msg}")
+
+
+def ptr(a) -> int:
+    """Address of a numpy array, a DeviceBuffer, or any object with data_ptr()."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    return int(a.data_ptr())
+
+
+class DeviceBuffer:
+    """A device allocation made by the library (gsp_device_alloc)."""
+
+    def __init__(self, ctx, shape, dtype=np.float64):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = ctypes.c_void_p()
+        check(load().gsp_device_alloc(ctx.handle, self.nbytes, ctypes.byref(h)))
+        self._p = h
+
+    @classmethod
+    def from_array(cls, ctx, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = cls(ctx, arr.shape, arr.dtype)
+        buf.upload(arr)
+        return buf
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        check(load().gsp_memcpy_to_device(self.ctx.handle, self._p, arr.ctypes.data, self.nbytes))
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(load().gsp_memcpy_to_host(self.ctx.handle, out.ctypes.data, self._p, self.nbytes))
+        return out
+
+    def free(self):
+        if self._p and self._p.value:
+            load().gsp_device_free(self.ctx.handle, self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def selftest_mfma(ctx, fp32: bool):
+    out = np.zeros((2, 64, 4))
+    check(load().gsp_selftest_mfma(ctx.handle, int(bool(fp32)), out.ctypes.data))
+    return out
+
+
+class Context:
+    """One HIP stream + workspace on one device (gsp_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(load().gsp_ctx_create(int(device), ctypes.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        check(load().gsp_ctx_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def synchronize(self):
+        check(load().gsp_ctx_synchronize(self._h))
+
+    def enable_timing(self, on=True):
+        check(load().gsp_ctx_enable_timing(self._h, int(bool(on))))
+
+    def timings(self, reset=False):
+        """{kernel slot: (milliseconds, launches)} accumulated with HIP events."""
+        t = GspTimings()
+        check(load().gsp_ctx_timings(self._h, ctypes.byref(t), int(bool(reset))))
+        return {k: (t.ms[k], t.launches[k]) for k in range(K_COUNT)}
+
+    def close(self):
+        if self._h and self._h.value:
+            load().gsp_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = {}
+
+
+def default_context(device: int | None = None) -> Context:
+    if device is None:
+        device = int(os.environ.get("GSP_DEVICE", "0"))
+    with _lock:
+        ctx = _default.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _default[device] = ctx
+    return ctx
+
+
+class SeqsimPlan:
+    """gsp_seqsim_plan: realization-invariant neighbours, weights and sigmas."""
+
+    def __init__(self, ctx: Context, problem: GspSeqsimProblem, keep=()):
+        self.ctx = ctx
+        self._keep = keep
+        self._h = ctypes.c_void_p()
+        check(load().gsp_seqsim_plan_create(ctx.handle, ctypes.byref(problem), ctypes.byref(self._h)))
+        info = GspSeqsimInfo()
+        check(load().gsp_seqsim_plan_info(self._h, ctypes.byref(info)))
+        self.info = info
+
+    @property
+    def n_nodes(self):
+        return self.info.n_nodes
+
+    @property
+    def kmax(self):
+        return self.info.maxneighbors
+
+    def export(self):
+        N, K = self.info.n_nodes, self.info.maxneighbors
+        out = dict(nfound=np.zeros(N, np.int32), nbr=np.zeros((N, K), np.int32),
+                   lam=np.zeros((N, K)), sigma=np.zeros(N), flags=np.zeros(N, np.uint8),
+                   level=np.zeros(N, np.int32))
+        check(load().gsp_seqsim_plan_export(self.ctx.handle, self._h, *(ptr(out[k]) for k in
+                                            ("nfound", "nbr", "lam", "sigma", "flags", "level"))))
+        return out
+
+    def sweep(self, z, data_vals=None, out=None, device_pointers=False):
+        """z: (R, N) standard normals in path order. Returns/out: (R, N)."""
+        if device_pointers:
+            R = int(z.shape[0])
+            check(load().gsp_seqsim_sweep(self.ctx.handle, self._h, R, ptr(z), ptr(data_vals),
+                                          ptr(out), 1))
+            return out
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        if z.ndim == 1:
+            z = z[None, :]
+        R = z.shape[0]
+        if z.shape[1] != self.info.n_nodes:
+            raise ArgumentError("z must hold one standard normal per path node")
+        dv = None
+        if self.info.n_data:
+            dv = np.ascontiguousarray(data_vals, dtype=np.float64)
+            if dv.shape != (self.info.n_data,):
+                raise ArgumentError("data_vals must hold one value per datum")
+        res = np.empty((R, self.info.n_nodes)) if out is None else out
+        check(load().gsp_seqsim_sweep(self.ctx.handle, self._h, R, ptr(z), ptr(dv), ptr(res), 0))
+        return res
+
+    def close(self):
+        if self._h and self._h.value:
+            load().gsp_seqsim_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LusimPlan:
+    """gsp_lusim_plan: d2 and L22 of one variable."""
+
+    def __init__(self, ctx: Context, problem: GspLusimProblem, keep=()):
+        self.ctx = ctx
+        self._keep = keep
+        self._h = ctypes.c_void_p()
+        check(load().gsp_lusim_factor(ctx.handle, ctypes.byref(problem), ctypes.byref(self._h)))
+        ns, nd = ctypes.c_int64(), ctypes.c_int64()
+        check(load().gsp_lusim_dims(self._h, ctypes.byref(ns), ctypes.byref(nd)))
+        self.n_sim, self.n_data = ns.value, nd.value
+        self.n_nodes = self.n_sim + self.n_data
+
+    def export(self):
+        L = np.zeros((self.n_sim, self.n_sim))
+        d2 = np.zeros(self.n_sim)
+        check(load().gsp_lusim_export(self.ctx.handle, self._h, ptr(L), ptr(d2)))
+        return L, d2
+
+    def sample(self, w, out=None, device_pointers=False):
+        """w: (R, n_sim) standard normals. Returns (R, N)."""
+        if device_pointers:
+            R = int(w.shape[0])
+            check(load().gsp_lusim_sample(self.ctx.handle, self._h, R, ptr(w), ptr(out), 1))
+            return out
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if w.ndim == 1:
+            w = w[None, :]
+        if w.shape[1] != self.n_sim:
+            raise ArgumentError("w must hold one standard normal per simulated cell")
+        R = w.shape[0]
+        res = np.empty((R, self.n_nodes)) if out is None else out
+        check(load().gsp_lusim_sample(self.ctx.handle, self._h, R, ptr(w), ptr(res), 0))
+        return res
+
+    def close(self):
+        if self._h and self._h.value:
+            load().gsp_lusim_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

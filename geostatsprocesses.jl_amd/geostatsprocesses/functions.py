@@ -1,0 +1,131 @@
+"""Geostatistical functions (GeoStatsFunctions 0.15 restated, un-vendored):
+spherical, exponential and gaussian variograms and covariances with nugget and
+axis ranges (optionally rotated). C(h) = sill - gamma(h), gamma(0) = 0 and
+gamma(u) = (sill - nugget) * shape(u) + nugget for u > 0 (practical ranges)."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+
+class GeoStatsFunction:
+    family = None
+    kind = _lib.VARIOGRAM
+
+    def __init__(self, sill=1.0, range=1.0, nugget=0.0, ranges=None, rotation=None):
+        self._sill = float(sill)
+        self._nugget = float(nugget)
+        self._ranges = tuple(float(r) for r in ranges) if ranges is not None else (float(range),)
+        self.rotation = None if rotation is None else np.asarray(rotation, dtype=np.float64).reshape(3, 3)
+
+    # trait functions (GeoStatsFunctions: isbanded, isstationary, issymmetric)
+    def isbanded(self):
+        return self.kind == _lib.COVARIANCE
+
+    def isstationary(self):
+        return True
+
+    def issymmetric(self):
+        return True
+
+    def nvariables(self):
+        return 1
+
+    def sill(self):
+        return self._sill
+
+    def nugget(self):
+        return self._nugget
+
+    def ranges(self, ndim=None):
+        r = self._ranges
+        if ndim is None:
+            return r
+        if len(r) == 1:
+            return r * ndim
+        if len(r) != ndim:
+            raise ValueError(f"function has {len(r)} ranges, domain has {ndim} dimensions")
+        return r
+
+    def range(self):
+        """range(f): the largest range."""
+        return max(self._ranges)
+
+    def shape(self, u):
+        raise NotImplementedError
+
+    def gamma(self, h):
+        """Variogram value at normalised lag(s) u = h / range."""
+        u = np.asarray(h, dtype=np.float64) / self._ranges[0]
+        g = (self._sill - self._nugget) * self.shape(u) + self._nugget
+        return np.where(u > 0, g, 0.0)
+
+    def cov(self, h):
+        return self._sill - self.gamma(h)
+
+    def struct(self, ndim) -> _lib.GspModel:
+        m = _lib.GspModel()
+        m.family = self.family
+        m.kind = self.kind
+        m.sill = self._sill
+        m.nugget = self._nugget
+        r = self.ranges(ndim)
+        for d in range(3):
+            m.ranges[d] = r[d] if d < ndim else 1.0
+        if self.rotation is not None:
+            for q in range(9):
+                m.rotation[q] = self.rotation.flat[q]
+            m.has_rotation = 1
+        return m
+
+    def __repr__(self):
+        return (f"{type(self).__name__}(sill={self._sill}, nugget={self._nugget}, "
+                f"ranges={self._ranges})")
+
+
+class _Spherical:
+    family = _lib.SPHERICAL
+
+    def shape(self, u):
+        u = np.asarray(u, dtype=np.float64)
+        return np.where(u < 1.0, 1.5 * u - 0.5 * u ** 3, 1.0)
+
+
+class _Exponential:
+    family = _lib.EXPONENTIAL
+
+    def shape(self, u):
+        return 1.0 - np.exp(-3.0 * np.asarray(u, dtype=np.float64))
+
+
+class _Gaussian:
+    family = _lib.GAUSSIAN
+
+    def shape(self, u):
+        u = np.asarray(u, dtype=np.float64)
+        return 1.0 - np.exp(-3.0 * u * u)
+
+
+class SphericalVariogram(_Spherical, GeoStatsFunction):
+    kind = _lib.VARIOGRAM
+
+
+class ExponentialVariogram(_Exponential, GeoStatsFunction):
+    kind = _lib.VARIOGRAM
+
+
+class GaussianVariogram(_Gaussian, GeoStatsFunction):
+    kind = _lib.VARIOGRAM
+
+
+class SphericalCovariance(_Spherical, GeoStatsFunction):
+    kind = _lib.COVARIANCE
+
+
+class ExponentialCovariance(_Exponential, GeoStatsFunction):
+    kind = _lib.COVARIANCE
+
+
+class GaussianCovariance(_Gaussian, GeoStatsFunction):
+    kind = _lib.COVARIANCE

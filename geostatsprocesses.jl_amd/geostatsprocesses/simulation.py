@@ -1,0 +1,228 @@
+"""Field simulation API mirroring the reference's plugin contract:
+
+  rand(process, domain, nreals; data, method, init, ...)   field.jl:47-124
+  preprocess(rng, process, method, init, domain, data)    called once
+  randsingle(rng, process, method, domain, data, preproc) called per realization
+
+for (GaussianProcess, SEQSIM) and (GaussianProcess, LUSIM). All arithmetic
+runs in libgsp_hip.so on the GPU; this module only shapes arguments. The
+ensemble form of `rand` draws every realization's normals in the reference's
+order (realization after realization, one per path node / simulated cell) and
+runs them in one device call.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .ensembles import Ensemble
+from .geometry import CartesianGrid, GeoTable, PointSet
+from .initialization import ExplicitInit, NearestInit, initialize
+from .processes import GaussianProcess, LinearPath
+
+
+class FieldSimulationMethod:
+    pass
+
+
+class MetricBall:
+    """Isotropic search ball of a given radius (Meshes MetricBall)."""
+
+    def __init__(self, radius):
+        self.radius = float(radius)
+
+
+class SEQSIM(FieldSimulationMethod):
+    """SEQSIM(path, minneighbors=1, maxneighbors=26, neighborhood=':range',
+    distance='euclidean') — seqsim.jl:42-48, plus the backend options
+    search_mode (k nearest then mask | masked k nearest) and precision
+    (FP64 | FP32 with one FP64-residual refinement step)."""
+
+    def __init__(self, path=None, minneighbors=1, maxneighbors=26, neighborhood=":range",
+                 distance="euclidean", search_mode=_lib.KNN_THEN_MASK, precision=_lib.FP64):
+        self.path = LinearPath() if path is None else path
+        self.minneighbors = int(minneighbors)
+        self.maxneighbors = int(maxneighbors)
+        self.neighborhood = neighborhood
+        if distance not in ("euclidean", None):
+            raise _lib.UnsupportedError("only the Euclidean distance is supported")
+        self.distance = distance
+        self.search_mode = search_mode
+        self.precision = precision
+
+
+class LUSIM(FieldSimulationMethod):
+    """LUSIM() — lusim.jl:36; precision LU_FP64 or LU_MIXED (FP32 MFMA trailing
+    updates, FP64 panels and conditioning)."""
+
+    def __init__(self, precision=_lib.LU_FP64):
+        self.precision = precision
+
+
+class FFTSIM(FieldSimulationMethod):
+    """Marker of the reference's FFT method (not provided by this backend)."""
+
+    def __init__(self, **kwargs):
+        self.options = kwargs
+
+
+def _grid_struct(domain) -> _lib.GspGrid:
+    if not isinstance(domain, CartesianGrid):
+        raise _lib.UnsupportedError("this backend simulates on CartesianGrid domains")
+    g = _lib.GspGrid()
+    g.ndim = domain.ndim
+    for d in range(3):
+        g.dims[d] = domain.dims[d] if d < domain.ndim else 1
+        g.origin[d] = float(domain.origin[d]) if d < domain.ndim else 0.0
+        g.spacing[d] = float(domain.spacing[d]) if d < domain.ndim else 1.0
+    return g
+
+
+def _sides(domain):
+    lo, hi = domain.boundingbox() if isinstance(domain, CartesianGrid) else (
+        domain.centroids().min(axis=0), domain.centroids().max(axis=0))
+    return hi - lo
+
+
+def defaultsimulation(process, domain, data=None):
+    """field.jl:152-166."""
+    if not isinstance(process, GaussianProcess):
+        return None
+    f = process.func
+    if (isinstance(domain, CartesianGrid) and f.isstationary() and f.nvariables() == 1
+            and f.range() <= np.min(_sides(domain)) / 3 and data is None):
+        return FFTSIM()
+    if domain.nelements() < 100 * 100 and f.isstationary() and f.issymmetric() and f.isbanded():
+        return LUSIM()
+    return SEQSIM()
+
+
+# ---------------------------------------------------------------- SEQSIM ----
+class SeqsimPreproc:
+    def __init__(self, plan, names, data_vals, path):
+        self.plan = plan
+        self.names = names
+        self.data_vals = data_vals
+        self.path = path
+
+
+def _seqsim_problem(process, method, domain, data, ctx):
+    f = process.func
+    if f.nvariables() != 1:
+        raise _lib.UnsupportedError("multivariate SEQSIM is not supported by this backend")
+    grid = _grid_struct(domain)
+    nd = domain.ndim
+    p = _lib.GspSeqsimProblem()
+    p.grid = grid
+    p.model = f.struct(nd)
+    p.mean = float(process.mean)
+    if data is not None:
+        coords = np.ascontiguousarray(data.domain.centroids(), dtype=np.float64).reshape(-1, nd)
+        names = data.names()
+        if len(names) != 1:
+            raise _lib.UnsupportedError("SEQSIM data must have exactly one variable")
+        vals = np.ascontiguousarray(data[names[0]], dtype=np.float64)
+        if np.isnan(vals).any():
+            raise _lib.UnsupportedError("missing data values are not supported by this backend")
+    else:
+        coords = np.zeros((0, nd))
+        names = process.defaultnames()
+        vals = np.zeros(0)
+    path = np.ascontiguousarray(method.path.traverse(domain), dtype=np.int64)
+    p.n_data = len(coords)
+    p.data_coords = coords.ctypes.data if len(coords) else None
+    p.n_path = len(path)
+    p.path = path.ctypes.data
+    p.minneighbors = method.minneighbors
+    p.maxneighbors = method.maxneighbors
+    neigh = method.neighborhood
+    if isinstance(neigh, str) and neigh == ":range":
+        p.radius = f.range()
+    elif isinstance(neigh, MetricBall):
+        p.radius = neigh.radius
+    elif neigh is None:
+        p.radius = 0.0
+    else:
+        p.radius = float(neigh)
+    p.search_mode = method.search_mode
+    p.precision = method.precision
+    p.device_pointers = 0
+    plan = _lib.SeqsimPlan(ctx, p, keep=(coords, path))
+    return SeqsimPreproc(plan, names, vals, path)
+
+
+# ----------------------------------------------------------------- LUSIM ----
+class LusimPreproc:
+    def __init__(self, plan, names, mean):
+        self.plan = plan
+        self.names = names
+        self.mean = mean
+
+
+def _lusim_problem(process, method, init, domain, data, ctx):
+    f = process.func
+    grid = _grid_struct(domain)
+    real, mask = initialize(process, domain, data, init)
+    names = tuple(real)
+    assert len(names) == f.nvariables(), "incompatible number of variables for geostatistical function"
+    assert len(names) in (1, 2), "LUSIM only supports univariate and bivariate simulation"
+    if len(names) != 1:
+        raise _lib.UnsupportedError("bivariate LUSIM is not supported by this backend")
+    var = names[0]
+    dinds = np.ascontiguousarray(np.flatnonzero(mask[var]), dtype=np.int64)
+    z1 = np.ascontiguousarray(real[var][dinds], dtype=np.float64)
+    p = _lib.GspLusimProblem()
+    p.grid = grid
+    p.model = f.struct(domain.ndim)
+    p.mean = float(process.mean)
+    p.n_data = len(dinds)
+    p.dinds = dinds.ctypes.data if len(dinds) else None
+    p.z1 = z1.ctypes.data if len(dinds) else None
+    p.precision = method.precision
+    p.device_pointers = 0
+    plan = _lib.LusimPlan(ctx, p, keep=(dinds, z1))
+    return LusimPreproc(plan, names, float(process.mean))
+
+
+# -------------------------------------------------------- plugin contract ----
+def preprocess(rng, process, method, init, domain, data, ctx=None):
+    ctx = _lib.default_context() if ctx is None else ctx
+    if not isinstance(process, GaussianProcess):
+        raise _lib.UnsupportedError("only GaussianProcess is accelerated by this backend")
+    if isinstance(method, SEQSIM):
+        return _seqsim_problem(process, method, domain, data, ctx)
+    if isinstance(method, LUSIM):
+        return _lusim_problem(process, method, init, domain, data, ctx)
+    raise _lib.UnsupportedError(f"{type(method).__name__} is not provided by this backend")
+
+
+def _draw(rng, preproc, R):
+    if isinstance(preproc, SeqsimPreproc):
+        z = rng.standard_normal((R, preproc.plan.n_nodes))
+        return preproc.plan.sweep(z, preproc.data_vals)
+    w = rng.standard_normal((R, preproc.plan.n_sim))
+    return preproc.plan.sample(w)
+
+
+def randsingle(rng, process, method, domain, data, preproc):
+    out = _draw(rng, preproc, 1)
+    return {preproc.names[0]: out[0]}
+
+
+def rand(process, domain, nreals=None, *, rng=None, data=None, method=None, init=None,
+         workers=None, async_=False, showprogress=False, ctx=None):
+    """rand([rng], process, domain, [nreals]; data, method, init, ...)."""
+    if rng is None:
+        rng = np.random.default_rng()
+    init = NearestInit() if init is None else init
+    if async_ and workers is not None and 0 in list(workers):
+        raise _lib.ArgumentError(
+            "the `async` option is not allowed when the master process is in the `workers`")
+    smethod = defaultsimulation(process, domain, data) if method is None else method
+    preproc = preprocess(rng, process, smethod, init, domain, data, ctx=ctx)
+    if nreals is None:
+        return GeoTable(randsingle(rng, process, smethod, domain, data, preproc), domain)
+    out = _draw(rng, preproc, int(nreals))
+    return Ensemble(domain, {preproc.names[0]: out})

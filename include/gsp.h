@@ -1,0 +1,245 @@
+/*
+ * gsp.h — C ABI of libgsp_hip.so, the MI355X (gfx950) backend for
+ * GeoStatsProcesses' GaussianProcess simulation methods SEQSIM and LUSIM.
+ *
+ * The Julia host keeps `rand(process, domain, nreals; method=...)` unchanged
+ * (src/simulation/field.jl:73-124). A package extension (the pattern of
+ * ext/GeoStatsProcessesTuringPatternsExt.jl:15, `import GeoStatsProcesses:
+ * preprocess, randsingle`) overrides the two plugin functions for
+ * (GaussianProcess, SEQSIM) and (GaussianProcess, LUSIM) and `ccall`s the
+ * entry points below. Plain C types only; every function returns a gsp_status.
+ *
+ * Conventions
+ *  - Indices are 0-based; grids are column-major (x fastest), like Meshes'
+ *    LinearIndices(size(grid)).
+ *  - Coordinates, ranges and radii are in physical units with units stripped
+ *    (the shim applies `ustrip` and restores units, seqsim.jl:86,141-143).
+ *  - Buffers are host pointers unless a call says otherwise or the problem's
+ *    `device_pointers` field is 1. The library copies host buffers in and out
+ *    and never keeps a host pointer after returning.
+ *  - Calls on one context are serialised by the context; distinct contexts
+ *    may be used from different threads.
+ */
+#ifndef GSP_H
+#define GSP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSP_ABI_VERSION 1
+
+typedef enum {
+  GSP_OK = 0,
+  GSP_ERR_ARGUMENT = 1,    /* Julia ArgumentError (lusim.jl:46, field.jl:99) */
+  GSP_ERR_ASSERTION = 2,   /* Julia AssertionError (lusim.jl:59-60, explicit.jl:29-30) */
+  GSP_ERR_POSDEF = 3,      /* LinearAlgebra.PosDefException (lusim.jl:92,98,103) */
+  GSP_ERR_DEVICE = 4,      /* HIP runtime failure */
+  GSP_ERR_MEMORY = 5,      /* device allocation failure */
+  GSP_ERR_UNSUPPORTED = 6, /* a configuration this backend does not implement */
+  GSP_ERR_INTERNAL = 7     /* a device-side consistency check failed */
+} gsp_status;
+
+typedef enum { GSP_SPHERICAL = 0, GSP_EXPONENTIAL = 1, GSP_GAUSSIAN = 2 } gsp_family;
+typedef enum { GSP_VARIOGRAM = 0, GSP_COVARIANCE = 1 } gsp_kind;
+
+/* Neighbour-search semantics of Meshes KBallSearch/KNearestSearch with a mask
+ * (called at seqsim.jl:107): the k nearest of all augmented points, then drop
+ * those outside the ball or not yet simulated (GSP_KNN_THEN_MASK, default), or
+ * the k nearest among the simulated points in the ball (GSP_MASKED_KNN). Ties
+ * in distance go to the lower augmented index. */
+typedef enum { GSP_KNN_THEN_MASK = 0, GSP_MASKED_KNN = 1 } gsp_search_mode;
+
+/* Precision of the local kriging solves: FP64 Cholesky, or FP32 Cholesky with
+ * one FP64-residual refinement step (FP64 retry if the FP32 factor fails). */
+typedef enum { GSP_FP64 = 0, GSP_FP32_REFINE = 1 } gsp_precision;
+
+/* LUSIM factorisation precision: FP64 throughout, or FP32 MFMA trailing
+ * updates with FP64 diagonal panels and FP64 conditioning solves. */
+typedef enum { GSP_LU_FP64 = 0, GSP_LU_MIXED = 1 } gsp_lu_precision;
+
+/* Plan flags per node (SEQSIM). */
+enum {
+  GSP_FLAG_FEW = 1,          /* n < minneighbors: prior draw (seqsim.jl:109-111) */
+  GSP_FLAG_FAIL = 2,         /* kriging Cholesky failed: prior (seqsim.jl:127-131) */
+  GSP_FLAG_FP64_FALLBACK = 4 /* FP32 factor failed, FP64 solve used */
+};
+
+/* A GeoStatsFunctions variogram or covariance (un-vendored, 0.15):
+ * C(h) = sill - gamma(h), gamma(0) = 0, gamma = (sill - nugget) * shape(u) +
+ * nugget for u > 0, u = |diag(1/ranges) R^T (x - y)|. */
+typedef struct {
+  int32_t family;       /* gsp_family */
+  int32_t kind;         /* gsp_kind */
+  double sill;
+  double nugget;
+  double ranges[3];     /* one per axis (only ndim used) */
+  double rotation[9];   /* row-major R, used when has_rotation != 0 */
+  int32_t has_rotation;
+  int32_t pad_;
+} gsp_model;
+
+/* A Meshes CartesianGrid: cell centroid of index (i,j,k) is
+ * origin + ((i,j,k) + 1/2) .* spacing. */
+typedef struct {
+  int32_t ndim;         /* 1..3 */
+  int32_t pad_;
+  int64_t dims[3];
+  double origin[3];     /* minimum corner */
+  double spacing[3];
+} gsp_grid;
+
+/* Everything `preprocess(rng, process, ::SEQSIM, init, domain, data)` needs
+ * (seqsim.jl:50-70): the grid, the process function and mean, the data
+ * locations (values come at sweep time), the traversal path and the SEQSIM
+ * options (seqsim.jl:42-48; maxneighbors/minneighbors are clamped as in
+ * seqsim.jl:202-208; neighborhood :range is passed as radius = range(f)). */
+typedef struct {
+  gsp_grid grid;
+  gsp_model model;
+  double mean;                  /* process.mean */
+  int64_t n_data;
+  const double* data_coords;    /* n_data x ndim, point-major */
+  int64_t n_path;
+  const int64_t* path;          /* traverse(domain, path): a permutation of 0..N-1 */
+  int32_t minneighbors;
+  int32_t maxneighbors;         /* <= 64 after clamping */
+  double radius;                /* MetricBall radius; <= 0 means KNearestSearch */
+  int32_t search_mode;          /* gsp_search_mode */
+  int32_t precision;            /* gsp_precision */
+  int32_t device_pointers;      /* 1: data_coords and path are device pointers */
+  int32_t pad_;
+} gsp_seqsim_problem;
+
+/* Everything `preprocess(rng, process, ::LUSIM, init, domain, data)` needs
+ * after `initialize` has placed the data (lusim.jl:53-72). */
+typedef struct {
+  gsp_grid grid;
+  gsp_model model;              /* must be a covariance (lusim.jl:44-50) */
+  double mean;
+  int64_t n_data;               /* number of data cells */
+  const int64_t* dinds;         /* findall(mask): ascending grid ids (lusim.jl:71) */
+  const double* z1;             /* data values at dinds (lusim.jl:75) */
+  int32_t precision;            /* gsp_lu_precision */
+  int32_t device_pointers;      /* 1: dinds and z1 are device pointers */
+} gsp_lusim_problem;
+
+typedef struct {
+  int64_t n_nodes;              /* N */
+  int64_t n_data;
+  int32_t minneighbors;         /* after clamping */
+  int32_t maxneighbors;         /* after clamping */
+  int32_t depth;                /* length of the longest dependency chain */
+  int32_t pad_;
+  double alpha;                 /* scaling factor of seqsim.jl:162-166 */
+  int64_t n_prior;              /* nodes drawn from the prior */
+  int64_t n_fallback;           /* nodes whose FP32 factor fell back to FP64 */
+} gsp_seqsim_info;
+
+/* Per-kernel device time, measured with HIP events recorded on the context
+ * stream around each launch while timing is enabled. */
+enum {
+  GSP_K_PLAN = 0,        /* neighbour search + kriging solves */
+  GSP_K_LEVELS = 1,      /* dependency depth */
+  GSP_K_SWEEP = 2,       /* dataflow draw sweep */
+  GSP_K_TRANSPOSE = 3,   /* node-major -> realization-major output */
+  GSP_K_LU_FACTOR = 4,   /* LUSIM assembly + factorisations (whole call) */
+  GSP_K_LU_SAMPLE = 5,   /* LUSIM d2 + L22 W and scatter (whole call) */
+  GSP_K_COUNT = 8
+};
+typedef struct {
+  double ms[GSP_K_COUNT];       /* accumulated milliseconds */
+  int64_t launches[GSP_K_COUNT];
+} gsp_timings;
+
+typedef struct gsp_ctx gsp_ctx;
+typedef struct gsp_seqsim_plan gsp_seqsim_plan;
+typedef struct gsp_lusim_plan gsp_lusim_plan;
+
+/* ---- context -------------------------------------------------------------
+ * One per (process, device): owns a HIP stream and reusable device workspace.
+ * Replaces nothing in the reference (Julia has no device state); the shim
+ * creates it lazily and frees it in a finalizer. */
+int32_t gsp_abi_version(void);
+int32_t gsp_ctx_create(int32_t device, gsp_ctx** out);
+int32_t gsp_ctx_destroy(gsp_ctx* ctx);
+/* The hipStream_t the context launches on (for events / interop). */
+int32_t gsp_ctx_stream(gsp_ctx* ctx, void** stream);
+int32_t gsp_ctx_synchronize(gsp_ctx* ctx);
+/* Thread-local message of the last failed call on this thread. */
+const char* gsp_last_error(void);
+/* Kernel timing (no counterpart in the reference, which has no tracing): when
+ * enabled, every launch of the kernels listed above is bracketed by HIP events
+ * on the context stream; gsp_ctx_timings returns the sums (and clears them
+ * when reset != 0). */
+int32_t gsp_ctx_enable_timing(gsp_ctx* ctx, int32_t on);
+int32_t gsp_ctx_timings(gsp_ctx* ctx, gsp_timings* out, int32_t reset);
+
+/* ---- device memory -------------------------------------------------------
+ * For callers that pass device pointers (device_pointers = 1) and have no HIP
+ * allocator of their own in the process. No counterpart in the reference.
+ * Copies run on the context stream and return when complete. */
+int32_t gsp_device_alloc(gsp_ctx* ctx, int64_t bytes, void** ptr);
+int32_t gsp_device_free(gsp_ctx* ctx, void* ptr);
+int32_t gsp_memcpy_to_device(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int32_t gsp_memcpy_to_host(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
+/* ---- SEQSIM --------------------------------------------------------------
+ * gsp_seqsim_plan_create replaces the realization-invariant part of
+ * preprocess + randsingle for SEQSIM (seqsim.jl:50-70 and the search / fit /
+ * status / predictprob steps of seqsim.jl:101-131): for every grid node the
+ * neighbour set, simple-kriging weights and standard deviation, computed once
+ * on the device (kernels: neighbour search + batched Cholesky solves). */
+int32_t gsp_seqsim_plan_create(gsp_ctx* ctx, const gsp_seqsim_problem* problem,
+                               gsp_seqsim_plan** out);
+int32_t gsp_seqsim_plan_info(const gsp_seqsim_plan* plan, gsp_seqsim_info* info);
+/* Copy the plan to host arrays indexed by grid id (domain order):
+ * nfound[N], nbr[N*kmax] (augmented ids, -1 padded), lambda[N*kmax],
+ * sigma[N], flags[N], level[N] (dependency depth). Any pointer may be NULL. */
+int32_t gsp_seqsim_plan_export(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t* nfound,
+                               int32_t* nbr, double* lambda, double* sigma, uint8_t* flags,
+                               int32_t* level);
+/* gsp_seqsim_sweep replaces R calls of randsingle(rng, process, ::SEQSIM, ...)
+ * (seqsim.jl:72-156, driven by the pmap at field.jl:113-119): the sequential
+ * draw x_i = mean + sum_j lambda_ij (x_nbr(j) - mean) + sigma_i z over the path.
+ *   z:          R x n_path standard normals, realization-major in path order,
+ *               i.e. exactly the order of the reference's `randn` calls
+ *               (one per path node, realization after realization).
+ *   data_vals:  n_data values of the conditioning data (ExplicitInit onto the
+ *               augmented domain, seqsim.jl:267-271).
+ *   out:        R x N, realization-major, domain order.
+ * device_pointers = 1: z, data_vals and out are device pointers and the call
+ * returns once the work is queued on the context stream. */
+int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                         const double* z, const double* data_vals, double* out,
+                         int32_t device_pointers);
+int32_t gsp_seqsim_plan_free(gsp_seqsim_plan* plan);
+
+/* ---- LUSIM ---------------------------------------------------------------
+ * gsp_lusim_factor replaces preprocess(rng, process, ::LUSIM, ...) for one
+ * variable (lusim.jl:66-107): C22, C11, C12 from the covariance (utils.jl:50-62),
+ * L11 = chol(C11), B12 = L11 \ C12, d2 = B12' (L11 \ z1),
+ * L22 = chol(C22 - B12' B12), or L22 = chol(C22) without data.
+ * Returns GSP_ERR_ARGUMENT for a variogram (lusim.jl:44-50) and
+ * GSP_ERR_POSDEF when a factorisation meets a non-positive pivot. */
+int32_t gsp_lusim_factor(gsp_ctx* ctx, const gsp_lusim_problem* problem,
+                         gsp_lusim_plan** out);
+/* gsp_lusim_sample replaces R calls of randsingle(..., ::LUSIM, ...) / _lusim
+ * (lusim.jl:112-126,145-175): z2 = d2 + L22 w, scatter z[dinds] = z1,
+ * z[sinds] = z2, + mean only without data (lusim.jl:172).
+ *   w:   R x n_s standard normals, realization-major (lusim.jl:160)
+ *   out: R x N, realization-major, domain order. */
+int32_t gsp_lusim_sample(gsp_ctx* ctx, const gsp_lusim_plan* plan, int32_t R,
+                         const double* w, double* out, int32_t device_pointers);
+/* n_s (number of simulated cells) and n_data of a factor. */
+int32_t gsp_lusim_dims(const gsp_lusim_plan* plan, int64_t* n_sim, int64_t* n_data);
+/* Copy L22 (n_s x n_s, row-major lower, upper zeroed) and d2 (n_s); NULL skips. */
+int32_t gsp_lusim_export(gsp_ctx* ctx, const gsp_lusim_plan* plan, double* L22, double* d2);
+int32_t gsp_lusim_plan_free(gsp_lusim_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSP_H */

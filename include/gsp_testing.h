@@ -1,0 +1,25 @@
+/*
+ * gsp_testing.h — test hooks of libgsp_hip.so (not part of the drop-in surface;
+ * no counterpart in the reference).
+ */
+#ifndef GSP_TESTING_H
+#define GSP_TESTING_H
+
+#include "gsp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Runs one 16x16x4 MFMA (FP64 when fp32 == 0, else FP32) twice and copies the
+ * 4 accumulator values of each of the 64 lanes to out[2][64][4]:
+ *   setup 0: A operand = lane + 1, B operand = 1 (outputs are row sums of A)
+ *   setup 1: A operand = 1, B operand = lane + 1 (outputs are column sums of B)
+ * so a test can confirm the operand and accumulator lane layouts the LUSIM
+ * GEMM relies on. */
+int32_t gsp_selftest_mfma(gsp_ctx* ctx, int32_t fp32, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSP_TESTING_H */

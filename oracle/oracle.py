@@ -1,0 +1,378 @@
+"""CPU oracle bindings — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module, and only as the checker (or as the timed CPU restatement). The product
+library never loads it.
+
+* SEQSIM: ctypes bindings to oracle/libgsp_oracle.so (gsp_oracle.c restates
+  src/simulation/field/seqsim.jl:50-156 and the dependency semantics it calls).
+* LUSIM: a numpy/scipy restatement of src/simulation/field/lusim.jl:38-175 and
+  src/utils.jl:50-62 (LAPACK potrf/trsm through scipy).
+* NearestInit / ExplicitInit: src/initialization/nearest.jl:12-34 and
+  src/initialization/explicit.jl:19-46.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
+KNN_THEN_MASK, MASKED_KNN = 0, 1
+FP64, FP32_REFINE = 0, 1
+FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK = 1, 2, 4
+
+
+class OrModel(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("sill", ctypes.c_double), ("nugget", ctypes.c_double),
+                ("ranges", ctypes.c_double * 3), ("rotation", ctypes.c_double * 9),
+                ("has_rotation", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class OrGrid(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int32), ("pad_", ctypes.c_int32),
+                ("dims", ctypes.c_int64 * 3), ("origin", ctypes.c_double * 3),
+                ("spacing", ctypes.c_double * 3)]
+
+
+class OrProblem(ctypes.Structure):
+    _fields_ = [("grid", OrGrid), ("model", OrModel), ("mean", ctypes.c_double),
+                ("n_data", ctypes.c_int64), ("data_coords", ctypes.POINTER(ctypes.c_double)),
+                ("n_path", ctypes.c_int64), ("path", ctypes.POINTER(ctypes.c_int64)),
+                ("minneighbors", ctypes.c_int32), ("maxneighbors", ctypes.c_int32),
+                ("radius", ctypes.c_double), ("search_mode", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("device_pointers", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
+
+
+@dataclass
+class SeqsimCase:
+    """A SEQSIM problem in physical units (0-based grid ids, column-major grid)."""
+    dims: tuple
+    origin: tuple = None
+    spacing: tuple = None
+    family: int = SPHERICAL
+    sill: float = 1.0
+    nugget: float = 0.0
+    ranges: tuple = (1.0,)
+    rotation: np.ndarray | None = None
+    mean: float = 0.0
+    data_coords: np.ndarray | None = None
+    data_vals: np.ndarray | None = None
+    path: np.ndarray | None = None
+    kmin: int = 1
+    kmax: int = 26
+    radius: float = 0.0
+    mode: int = KNN_THEN_MASK
+    precision: int = FP64
+    _keep: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        nd = len(self.dims)
+        if self.origin is None:
+            self.origin = (0.0,) * nd
+        if self.spacing is None:
+            self.spacing = (1.0,) * nd
+        if len(self.ranges) == 1 and nd > 1:
+            self.ranges = tuple(self.ranges) * nd
+        if self.data_coords is None:
+            self.data_coords = np.zeros((0, nd))
+        self.data_coords = np.ascontiguousarray(self.data_coords, dtype=np.float64).reshape(-1, nd)
+        if self.data_vals is None:
+            self.data_vals = np.zeros(len(self.data_coords))
+        self.data_vals = np.ascontiguousarray(self.data_vals, dtype=np.float64)
+        if self.path is None:
+            self.path = np.arange(self.n_nodes, dtype=np.int64)
+        self.path = np.ascontiguousarray(self.path, dtype=np.int64)
+
+    @property
+    def ndim(self):
+        return len(self.dims)
+
+    @property
+    def n_nodes(self):
+        return int(np.prod(self.dims))
+
+    @property
+    def n_data(self):
+        return len(self.data_coords)
+
+    def fill(self, st):
+        """Fill a ctypes problem struct (shared field layout) from this case."""
+        nd = self.ndim
+        st.grid.ndim = nd
+        for d in range(3):
+            st.grid.dims[d] = int(self.dims[d]) if d < nd else 1
+            st.grid.origin[d] = float(self.origin[d]) if d < nd else 0.0
+            st.grid.spacing[d] = float(self.spacing[d]) if d < nd else 1.0
+            st.model.ranges[d] = float(self.ranges[d]) if d < nd else 1.0
+        st.model.family = self.family
+        st.model.kind = 0
+        st.model.sill = self.sill
+        st.model.nugget = self.nugget
+        if self.rotation is not None:
+            rot = np.asarray(self.rotation, dtype=np.float64).reshape(3, 3)
+            for q in range(9):
+                st.model.rotation[q] = rot.flat[q]
+            st.model.has_rotation = 1
+        st.mean = self.mean
+        st.n_data = self.n_data
+        st.data_coords = self.data_coords.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        st.n_path = len(self.path)
+        st.path = self.path.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        st.minneighbors = self.kmin
+        st.maxneighbors = self.kmax
+        st.radius = self.radius
+        st.search_mode = self.mode
+        st.precision = self.precision
+        return st
+
+    def struct(self):
+        return self.fill(OrProblem())
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libgsp_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        P = ctypes.POINTER
+        L.or_exp.restype = ctypes.c_double
+        L.or_exp.argtypes = [ctypes.c_double]
+        L.or_cov_u.restype = ctypes.c_double
+        L.or_cov_u.argtypes = [P(OrModel), ctypes.c_double]
+        L.or_scale_factor.restype = ctypes.c_double
+        L.or_scale_factor.argtypes = [P(OrProblem)]
+        L.or_search_brute.restype = ctypes.c_int32
+        L.or_search_brute.argtypes = [P(OrProblem), ctypes.c_int32, P(ctypes.c_int64),
+                                      ctypes.c_int64, P(ctypes.c_int32)]
+        L.or_bucket_create.restype = ctypes.c_void_p
+        L.or_bucket_create.argtypes = [P(OrProblem)]
+        L.or_bucket_free.argtypes = [ctypes.c_void_p]
+        L.or_search_bucket.restype = ctypes.c_int32
+        L.or_search_bucket.argtypes = [P(OrProblem), ctypes.c_void_p, ctypes.c_int32,
+                                       P(ctypes.c_int64), ctypes.c_int64, P(ctypes.c_int32)]
+        L.or_krige.restype = ctypes.c_int32
+        L.or_krige.argtypes = [P(OrProblem), ctypes.c_double, ctypes.c_int64, P(ctypes.c_int32),
+                               ctypes.c_int32, ctypes.c_int32, P(ctypes.c_double),
+                               P(ctypes.c_double)]
+        L.or_seqsim_plan.restype = ctypes.c_int32
+        L.or_seqsim_plan.argtypes = [P(OrProblem), ctypes.c_int32, P(ctypes.c_int32),
+                                     P(ctypes.c_int32), P(ctypes.c_double), P(ctypes.c_double),
+                                     P(ctypes.c_uint8), P(ctypes.c_int32), P(ctypes.c_int32)]
+        L.or_seqsim_sweep.restype = None
+        L.or_seqsim_sweep.argtypes = [P(OrProblem), ctypes.c_int32, P(ctypes.c_int32),
+                                      P(ctypes.c_int32), P(ctypes.c_double), P(ctypes.c_double),
+                                      P(ctypes.c_uint8), ctypes.c_int32, P(ctypes.c_double),
+                                      P(ctypes.c_double), P(ctypes.c_double)]
+        L.or_seqsim_reference_cost.restype = ctypes.c_int64
+        L.or_seqsim_reference_cost.argtypes = [P(OrProblem), ctypes.c_int32, P(ctypes.c_double),
+                                               P(ctypes.c_double), P(ctypes.c_double),
+                                               ctypes.c_int32, ctypes.c_int64]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+# ------------------------------------------------------------------ primitives
+def exp(x: float) -> float:
+    return lib().or_exp(float(x))
+
+
+def cov_u(family, sill, nugget, u) -> float:
+    m = OrModel(family=family, kind=0, sill=sill, nugget=nugget)
+    return lib().or_cov_u(ctypes.byref(m), float(u))
+
+
+def scale_factor(case: SeqsimCase) -> float:
+    return lib().or_scale_factor(ctypes.byref(case.struct()))
+
+
+def clamp_neighbors(nelm, nmin, nmax):
+    """seqsim.jl:202-208."""
+    if nmax > nelm or nmax < 1:
+        nmax = nelm
+    if nmin > nmax or nmin < 1:
+        nmin = 1
+    return nmin, nmax
+
+
+def positions(case: SeqsimCase) -> np.ndarray:
+    pos = np.empty(case.n_nodes, dtype=np.int64)
+    pos[case.path] = np.arange(len(case.path))
+    return pos
+
+
+def search(case: SeqsimCase, target: int, brute=True, pos=None, bucket=None):
+    st = case.struct()
+    _, nmax = clamp_neighbors(case.n_nodes, case.kmin, case.kmax)
+    pos = positions(case) if pos is None else pos
+    out = np.full(nmax, -1, dtype=np.int32)
+    if brute:
+        n = lib().or_search_brute(ctypes.byref(st), nmax, _p(pos, ctypes.c_int64), int(target),
+                                  _p(out, ctypes.c_int32))
+    else:
+        own = bucket is None
+        b = lib().or_bucket_create(ctypes.byref(st)) if own else bucket
+        n = lib().or_search_bucket(ctypes.byref(st), b, nmax, _p(pos, ctypes.c_int64),
+                                   int(target), _p(out, ctypes.c_int32))
+        if own:
+            lib().or_bucket_free(b)
+    return out[:n].copy()
+
+
+def krige(case: SeqsimCase, target: int, nbr, precision=None):
+    st = case.struct()
+    alpha = lib().or_scale_factor(ctypes.byref(st))
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    lam = np.zeros(max(len(nbr), 1))
+    sig = np.zeros(1)
+    prec = case.precision if precision is None else precision
+    f = lib().or_krige(ctypes.byref(st), alpha, int(target), _p(nbr, ctypes.c_int32), len(nbr),
+                       prec, _p(lam, ctypes.c_double), _p(sig, ctypes.c_double))
+    return lam[:len(nbr)], float(sig[0]), int(f)
+
+
+@dataclass
+class Plan:
+    kmax: int
+    nfound: np.ndarray
+    nbr: np.ndarray
+    lam: np.ndarray
+    sigma: np.ndarray
+    flags: np.ndarray
+    level: np.ndarray
+
+
+def seqsim_plan(case: SeqsimCase, brute=False) -> Plan:
+    st = case.struct()
+    N = case.n_nodes
+    _, nmax = clamp_neighbors(N, case.kmin, case.kmax)
+    nfound = np.zeros(N, np.int32)
+    nbr = np.zeros((N, nmax), np.int32)
+    lam = np.zeros((N, nmax))
+    sigma = np.zeros(N)
+    flags = np.zeros(N, np.uint8)
+    level = np.zeros(N, np.int32)
+    kout = ctypes.c_int32(0)
+    rc = lib().or_seqsim_plan(ctypes.byref(st), int(brute), _p(nfound, ctypes.c_int32),
+                              _p(nbr, ctypes.c_int32), _p(lam, ctypes.c_double),
+                              _p(sigma, ctypes.c_double), _p(flags, ctypes.c_uint8),
+                              _p(level, ctypes.c_int32), ctypes.byref(kout))
+    if rc != 0:
+        raise ValueError("oracle rejected the problem")
+    return Plan(kout.value, nfound, nbr, lam, sigma, flags, level)
+
+
+def seqsim_sweep(case: SeqsimCase, plan: Plan, z: np.ndarray) -> np.ndarray:
+    """z: (R, n_path) standard normals in path order. Returns (R, N)."""
+    st = case.struct()
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    R = z.shape[0]
+    out = np.zeros((R, case.n_nodes))
+    lib().or_seqsim_sweep(ctypes.byref(st), plan.kmax, _p(plan.nfound, ctypes.c_int32),
+                          _p(plan.nbr, ctypes.c_int32), _p(plan.lam, ctypes.c_double),
+                          _p(plan.sigma, ctypes.c_double), _p(plan.flags, ctypes.c_uint8), R,
+                          _p(z, ctypes.c_double), _p(case.data_vals, ctypes.c_double),
+                          _p(out, ctypes.c_double))
+    return out
+
+
+def seqsim_reference_cost(case: SeqsimCase, z: np.ndarray, threads=1, n_nodes_limit=0):
+    """Search + solve per node per realization (seqsim.jl:98-138 cost structure)."""
+    st = case.struct()
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    R = z.shape[0]
+    out = np.zeros((R, case.n_nodes))
+    n = lib().or_seqsim_reference_cost(ctypes.byref(st), R, _p(z, ctypes.c_double),
+                                       _p(case.data_vals, ctypes.c_double),
+                                       _p(out, ctypes.c_double), int(threads), int(n_nodes_limit))
+    return out, int(n)
+
+
+# ---------------------------------------------------------------------- LUSIM
+def covariance_np(family, sill, nugget, ranges, xa, xb, rotation=None):
+    """Pairwise C(x_a, x_b) = sill - gamma(h) (utils.jl:50-62), numpy exp."""
+    xa = np.asarray(xa, dtype=np.float64)
+    xb = np.asarray(xb, dtype=np.float64)
+    delta = xa[:, None, :] - xb[None, :, :]
+    if rotation is not None:
+        delta = delta @ np.asarray(rotation, dtype=np.float64).reshape(3, 3)[:xa.shape[1], :xa.shape[1]]
+    r = np.asarray(ranges, dtype=np.float64)[: xa.shape[1]]
+    u = np.sqrt(np.sum((delta / r) ** 2, axis=-1))
+    if family == SPHERICAL:
+        shape = np.where(u < 1.0, 1.5 * u - 0.5 * u ** 3, 1.0)
+    elif family == EXPONENTIAL:
+        shape = 1.0 - np.exp(-3.0 * u)
+    else:
+        shape = 1.0 - np.exp(-3.0 * u * u)
+    gamma = (sill - nugget) * shape + nugget
+    C = sill - gamma
+    return np.where(u == 0.0, sill, C)
+
+
+def grid_centroids(dims, origin=None, spacing=None):
+    """Cell centroids of a CartesianGrid, column-major (x fastest), 0-based."""
+    nd = len(dims)
+    origin = np.zeros(nd) if origin is None else np.asarray(origin, dtype=np.float64)
+    spacing = np.ones(nd) if spacing is None else np.asarray(spacing, dtype=np.float64)
+    idx = np.indices(dims[::-1]).reshape(nd, -1)[::-1].T  # column-major order
+    return origin + (idx + 0.5) * spacing
+
+
+def nearest_init(centroids, data_coords):
+    """NearestInit (nearest.jl:12-34): each datum goes to its nearest element;
+    a later datum overwrites an earlier one. Ties go to the lower index."""
+    centroids = np.asarray(centroids, dtype=np.float64)
+    dest = np.empty(len(data_coords), dtype=np.int64)
+    for i, x in enumerate(np.asarray(data_coords, dtype=np.float64)):
+        d2 = np.sum((centroids - x) ** 2, axis=1)
+        dest[i] = int(np.argmin(d2))  # argmin returns the first (lowest) index on ties
+    return dest
+
+
+def lusim_preprocess(centroids, dinds, z1, family, sill, nugget, ranges):
+    """lusim.jl:66-107 for one variable: returns (d2, L22, sinds)."""
+    import scipy.linalg as sla
+    N = len(centroids)
+    dinds = np.asarray(dinds, dtype=np.int64)
+    sinds = np.setdiff1d(np.arange(N), dinds)
+    sdom = centroids[sinds]
+    C22 = covariance_np(family, sill, nugget, ranges, sdom, sdom)
+    if len(dinds) == 0:
+        L22 = sla.cholesky(C22, lower=True)
+        return np.zeros(len(sinds)), L22, sinds
+    ddom = centroids[dinds]
+    C11 = covariance_np(family, sill, nugget, ranges, ddom, ddom)
+    C12 = covariance_np(family, sill, nugget, ranges, ddom, sdom)
+    L11 = sla.cholesky(C11, lower=True)
+    B12 = sla.solve_triangular(L11, C12, lower=True)
+    d2 = B12.T @ sla.solve_triangular(L11, np.asarray(z1, dtype=np.float64), lower=True)
+    L22 = sla.cholesky(C22 - B12.T @ B12, lower=True)
+    return d2, L22, sinds
+
+
+def lusim_sample(d2, L22, dinds, sinds, z1, mean, w):
+    """lusim.jl:145-175: w is (R, n_s); returns (R, N)."""
+    w = np.asarray(w, dtype=np.float64)
+    R = w.shape[0]
+    N = len(dinds) + len(sinds)
+    out = np.empty((R, N))
+    z2 = d2[None, :] + w @ L22.T
+    out[:, sinds] = z2
+    if len(dinds):
+        out[:, dinds] = np.asarray(z1)[None, :]
+    else:
+        out += mean
+    return out

@@ -1,0 +1,166 @@
+"""CPU tests: the C-ABI library loads and exports every symbol include/gsp.h
+declares; the host-side mirror of the reference interface (grids, initialization,
+method defaults, ensembles) behaves like the reference's tests expect."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import geostatsprocesses as gsp
+from geostatsprocesses import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols(header="gsp.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^\s*(?:int32_t|const char\*)\s+(gsp_\w+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 16
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES)
+    testing = declared_symbols("gsp_testing.h")
+    assert testing and set(testing) == set(_lib.TESTING_SIGNATURES)
+    for s in testing:
+        assert hasattr(lib, s), s
+    assert lib.gsp_abi_version() == 1
+
+
+def test_no_device_is_a_clean_error():
+    """Without a GPU the context call fails with a status code, never a crash."""
+    import ctypes
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.gsp_ctx_create(0, ctypes.byref(h))
+    if rc == 0:  # a GPU is visible
+        lib.gsp_ctx_destroy(h)
+        pytest.skip("GPU present")
+    assert rc == _lib.GSP_ERR_DEVICE
+    assert b"device" in lib.gsp_last_error()
+
+
+def test_null_arguments_are_rejected():
+    import ctypes
+    lib = _lib.load()
+    assert lib.gsp_ctx_create(0, None) == _lib.GSP_ERR_ARGUMENT
+    assert lib.gsp_seqsim_plan_info(None, None) == _lib.GSP_ERR_ARGUMENT
+    assert lib.gsp_lusim_dims(None, None, None) == _lib.GSP_ERR_ARGUMENT
+    assert lib.gsp_seqsim_sweep(None, None, 1, None, None, None, 0) == _lib.GSP_ERR_ARGUMENT
+    out = ctypes.c_void_p()
+    assert lib.gsp_seqsim_plan_create(None, None, ctypes.byref(out)) == _lib.GSP_ERR_ARGUMENT
+
+
+# ------------------------------------------------------------- host logic ----
+def load_tsv(name, ncoord):
+    arr = np.loadtxt(os.path.join(GOLDEN, name), skiprows=1)
+    return gsp.georef({"value": arr[:, ncoord]}, [tuple(r) for r in arr[:, :ncoord]])
+
+
+def test_grid_linear_indices_column_major():
+    g = gsp.CartesianGrid(100, 100)
+    assert g.linear_index(24, 24) == 2424
+    assert g.linear_index(1, 0) == 1 and g.linear_index(0, 1) == 100
+    np.testing.assert_allclose(g.centroid(2424), [24.5, 24.5])
+    g = gsp.CartesianGrid((0.5, 0.5), (100.5, 100.5), dims=(100, 100))
+    np.testing.assert_allclose(g.centroid(g.linear_index(24, 24)), [25.0, 25.0])
+
+
+def test_nearest_init_reference_mappings():
+    """test/initialization.jl:5-29 (1-based indices there)."""
+    proc = gsp.GaussianProcess(gsp.GaussianVariogram())
+    init = gsp.NearestInit()
+    data1D = load_tsv("data1D.tsv", 1)
+    grid = gsp.CartesianGrid((-0.5,), (99.5,), dims=(100,))
+    real, mask = gsp.initialize(proc, grid, data1D, init)
+    for i, j in {100: 11, 81: 9, 11: 2, 21: 3, 91: 10, 51: 6, 61: 7, 71: 8, 31: 4, 41: 5, 1: 1}.items():
+        assert real["value"][i - 1] == data1D.value[j - 1]
+        assert mask["value"][i - 1]
+    data2D = load_tsv("data2D.tsv", 2)
+    grid = gsp.CartesianGrid((-0.5, -0.5), (99.5, 99.5), dims=(100, 100))
+    real, mask = gsp.initialize(proc, grid, data2D, init)
+    for i, j in {5076: 3, 2526: 1, 7551: 2}.items():
+        assert real["value"][i - 1] == data2D.value[j - 1]
+        assert mask["value"][i - 1]
+    pset = gsp.PointSet([(25, 25), (50, 75), (75, 50)])
+    real, mask = gsp.initialize(proc, pset, data2D, init)
+    for i, j in {2: 2, 3: 3, 1: 1}.items():
+        assert real["value"][i - 1] == data2D.value[j - 1]
+        assert mask["value"][i - 1]
+
+
+def test_explicit_init_reference_placements():
+    """test/initialization.jl:31-59."""
+    proc = gsp.GaussianProcess(gsp.GaussianVariogram())
+    grid = gsp.CartesianGrid(10, 10, 10)
+    rng = np.random.default_rng(0)
+    data = gsp.georef({"z": rng.random(10)}, [tuple(p) for p in rng.random((10, 3))])
+    real, mask = gsp.initialize(proc, grid, data, gsp.ExplicitInit(np.arange(10)))
+    assert np.array_equal(real["z"][:10], data.z) and mask["z"][:10].all()
+    real, mask = gsp.initialize(proc, grid, data, gsp.ExplicitInit(np.arange(990, 1000)))
+    assert np.array_equal(real["z"][990:], data.z) and mask["z"][990:].all()
+    real, mask = gsp.initialize(proc, grid, data, gsp.ExplicitInit(np.arange(3), np.arange(997, 1000)))
+    assert np.array_equal(real["z"][997:], data.z[:3]) and mask["z"].sum() == 3
+    with pytest.raises(AssertionError):
+        gsp.initialize(proc, grid, data, gsp.ExplicitInit(np.arange(3), np.arange(5)))
+
+
+def test_defaultsimulation_reference_choices():
+    """test/field.jl:3-15."""
+    proc1 = gsp.GaussianProcess(gsp.GaussianVariogram())
+    proc2 = gsp.GaussianProcess(gsp.GaussianCovariance())
+    grid = gsp.CartesianGrid(100, 100)
+    rng = np.random.default_rng(1)
+    pset1 = gsp.PointSet(rng.random((1000, 2)))
+    pset2 = gsp.PointSet(rng.random((10000, 2)))
+    assert isinstance(gsp.defaultsimulation(proc1, grid), gsp.FFTSIM)
+    assert isinstance(gsp.defaultsimulation(proc1, pset1), gsp.SEQSIM)
+    assert isinstance(gsp.defaultsimulation(proc1, pset2), gsp.SEQSIM)
+    assert isinstance(gsp.defaultsimulation(proc2, pset1), gsp.LUSIM)
+
+
+def test_process_and_method_defaults():
+    proc = gsp.GaussianProcess(gsp.SphericalVariogram(range=35.0))
+    assert proc.mean == 0.0 and proc.defaultnames() == ("field",)
+    m = gsp.SEQSIM()
+    assert (m.minneighbors, m.maxneighbors, m.neighborhood) == (1, 26, ":range")
+    assert isinstance(m.path, gsp.LinearPath)
+    with pytest.raises(AssertionError):
+        gsp.GaussianProcess(gsp.SphericalVariogram(), [0.0, 1.0])
+    f = gsp.ExponentialVariogram(ranges=(40.0, 40.0, 8.0), nugget=0.05)
+    assert f.range() == 40.0 and not f.isbanded()
+    assert gsp.ExponentialCovariance(range=20.0).isbanded()
+
+
+def test_paths_are_permutations():
+    g = gsp.CartesianGrid(7, 5)
+    for p in (gsp.LinearPath(), gsp.RandomPath(seed=3), gsp.SourcePath([4, 2, 4])):
+        order = p.traverse(g)
+        assert sorted(order) == list(range(35))
+    assert list(gsp.SourcePath([4, 2, 4]).traverse(g)[:3]) == [4, 2, 0]
+
+
+def test_async_on_master_is_argument_error():
+    """test/field.jl:321: async with the master among the workers."""
+    proc = gsp.GaussianProcess(gsp.GaussianVariogram())
+    with pytest.raises(gsp.ArgumentError):
+        gsp.rand(proc, gsp.CartesianGrid(10, 10), 3, workers=[0], async_=True)
+
+
+def test_ensemble_reductions():
+    g = gsp.CartesianGrid(3)
+    vals = np.array([[0.0, 1.0, 2.0], [2.0, 3.0, 4.0], [1.0, 2.0, 9.0]])
+    e = gsp.Ensemble(g, {"field": vals})
+    assert len(e) == 3
+    np.testing.assert_allclose(e.mean().field, vals.mean(axis=0))
+    np.testing.assert_allclose(e.var().field, vals.var(axis=0, ddof=1))
+    np.testing.assert_allclose(e.cdf(1.5).field, [2 / 3, 1 / 3, 0.0])
+    np.testing.assert_allclose(e.ccdf(1.5).field, [1 / 3, 2 / 3, 1.0])
+    np.testing.assert_allclose(e.quantile(0.5).field, np.median(vals, axis=0))
+    assert np.array_equal(e[1].field, vals[1])

@@ -1,0 +1,335 @@
+"""Pins the CPU oracle (oracle/gsp_oracle.c, oracle/oracle.py) before it is
+trusted as the checker: against closed forms, independent numpy/scipy
+computations, brute force on small inputs and the reference's own test
+expectations. CPU only."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle as O
+
+
+# ------------------------------------------------------------------- exp ----
+def test_exp_matches_numpy_within_two_ulp():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-745, 709, 20000), rng.uniform(-1, 1, 5000),
+                         [0.0, -0.0, 1e-300, -1e-300, 0.5 * np.log(2), -0.5 * np.log(2),
+                          -3.0, -30.0, -700.0, -744.0, 709.0]])
+    ours = np.array([O.exp(x) for x in xs])
+    ref = np.exp(xs)
+    fin = ref > 0
+    ulp = np.abs(ours[fin] - ref[fin]) / np.spacing(ref[fin])
+    assert ulp.max() <= 2.0
+    assert O.exp(0.0) == 1.0
+    assert O.exp(-800.0) == 0.0 and np.isinf(O.exp(800.0))
+
+
+# ------------------------------------------------------------ variograms ----
+@pytest.mark.parametrize("fam", [O.SPHERICAL, O.EXPONENTIAL, O.GAUSSIAN])
+def test_covariance_invariants(fam):
+    sill, nug = 2.0, 0.3
+    assert O.cov_u(fam, sill, nug, 0.0) == sill           # gamma(0) = 0
+    us = np.linspace(1e-6, 3, 400)
+    c = np.array([O.cov_u(fam, sill, nug, u) for u in us])
+    assert np.all(np.diff(c) <= 1e-15)                     # monotone non-increasing
+    assert abs(c[0] - (sill - nug)) < 1e-5                 # nugget jump at 0+
+    assert c.min() >= -1e-15
+
+
+def test_covariance_closed_forms():
+    # spherical: C(u) = s (1 - 3u/2 + u^3/2) for u < 1 (nugget 0), 0 beyond
+    for u in (0.1, 0.25, 0.5, 0.9):
+        assert np.isclose(O.cov_u(O.SPHERICAL, 1.5, 0.0, u), 1.5 * (1 - 1.5 * u + 0.5 * u ** 3),
+                          rtol=1e-14)
+    assert O.cov_u(O.SPHERICAL, 1.0, 0.0, 1.0) == 0.0
+    assert O.cov_u(O.SPHERICAL, 1.0, 0.0, 7.0) == 0.0
+    # practical range: 95% of the structured sill is reached at u = 1
+    assert np.isclose(O.cov_u(O.EXPONENTIAL, 1.0, 0.0, 1.0), np.exp(-3.0), rtol=1e-14)
+    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 1.0), np.exp(-3.0), rtol=1e-14)
+    assert np.isclose(O.cov_u(O.EXPONENTIAL, 1.0, 0.01, 0.5), 0.99 * np.exp(-1.5), rtol=1e-14)
+    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 0.5), np.exp(-0.75), rtol=1e-14)
+
+
+# --------------------------------------------------------------- scaling ----
+def test_scale_factor_closed_form():
+    # seqsim.jl:162-189: alpha = 1 / max(|grid bbox|, |data bbox|, range)
+    c = O.SeqsimCase(dims=(100, 100), ranges=(10.0,))
+    assert O.scale_factor(c) == 1.0 / 100.0
+    c = O.SeqsimCase(dims=(100, 100), ranges=(250.0,))
+    assert O.scale_factor(c) == 1.0 / 250.0
+    c = O.SeqsimCase(dims=(10, 10), origin=(-40.0, 0.0), ranges=(5.0,),
+                     data_coords=np.array([[3.0, -70.5]]))
+    assert O.scale_factor(c) == 1.0 / 70.5
+
+
+def test_clamp_neighbors():
+    # seqsim.jl:202-208
+    assert O.clamp_neighbors(100, 1, 26) == (1, 26)
+    assert O.clamp_neighbors(10, 1, 26) == (1, 10)
+    assert O.clamp_neighbors(10, 5, 0) == (5, 10)
+    assert O.clamp_neighbors(10, 12, 8) == (1, 8)
+
+
+# ---------------------------------------------------------------- search ----
+def numpy_search(case, target, pos):
+    """Independent restatement: sort every augmented point by (d2, index)."""
+    N = case.n_nodes
+    cents = O.grid_centroids(case.dims, case.origin, case.spacing)
+    tgt_ijk = np.array(np.unravel_index(target, case.dims, order="F"))
+    all_ijk = np.array(np.unravel_index(np.arange(N), case.dims, order="F")).T
+    d2g = np.sum(((all_ijk - tgt_ijk) * np.asarray(case.spacing)) ** 2, axis=1)
+    d2d = np.sum((case.data_coords - cents[target]) ** 2, axis=1)
+    d2 = np.concatenate([d2g, d2d])
+    idx = np.arange(N + case.n_data)
+    elig = np.concatenate([pos < pos[target], np.ones(case.n_data, bool)])
+    inball = d2 <= case.radius ** 2 if case.radius > 0 else np.ones_like(d2, bool)
+    _, k = O.clamp_neighbors(N, case.kmin, case.kmax)
+    order = np.lexsort((idx, d2))
+    order = order[inball[order]]
+    if case.mode == O.KNN_THEN_MASK:
+        top = order[:k]
+        return top[elig[top]].astype(np.int32)
+    order = order[elig[order]]
+    return order[:k].astype(np.int32)
+
+
+CASES = [
+    dict(dims=(40,), ranges=(6.0,), kmax=5, radius=0.0, nd=4),
+    dict(dims=(25, 18), ranges=(8.0,), kmax=12, radius=6.0, nd=12),
+    dict(dims=(25, 18), ranges=(8.0,), kmax=16, radius=0.0, nd=0),
+    dict(dims=(9, 8, 7), ranges=(4.0, 4.0, 2.0), kmax=24, radius=4.0, nd=10),
+    dict(dims=(12, 10), spacing=(1.0, 2.5), ranges=(8.0,), kmax=8, radius=7.5, nd=30),
+]
+
+
+@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_search_brute_and_bucket_match_numpy(ci, mode):
+    spec = dict(CASES[ci])
+    rng = np.random.default_rng(ci)
+    nd = spec.pop("nd")
+    dims = spec["dims"]
+    N = int(np.prod(dims))
+    sp = np.asarray(spec.get("spacing", (1.0,) * len(dims)))
+    # some data outside the grid box, some exactly on node centres (ties)
+    data = rng.uniform(-0.1, 1.1, (nd, len(dims))) * np.asarray(dims) * sp
+    if nd >= 2:
+        data[0] = (np.array([3] + [2] * (len(dims) - 1)) + 0.5) * sp
+    if mode == O.MASKED_KNN and spec["radius"] == 0.0:
+        spec["radius"] = 6.0
+    case = O.SeqsimCase(**spec, data_coords=data, path=rng.permutation(N), mode=mode)
+    pos = O.positions(case)
+    for t in range(N):
+        ref = numpy_search(case, t, pos)
+        assert np.array_equal(O.search(case, t, brute=True, pos=pos), ref), t
+        assert np.array_equal(O.search(case, t, brute=False, pos=pos), ref), t
+
+
+def test_search_ties_lower_index_wins():
+    # 5x5 grid, target in the centre, all 4 axial neighbours at distance 1
+    first = [17, 13, 11, 7, 12]            # all four axial neighbours precede 12
+    path = np.array(first + [i for i in range(25) if i not in first])
+    case = O.SeqsimCase(dims=(5, 5), kmax=3, radius=0.0, mode=O.MASKED_KNN, path=path)
+    got = O.search(case, 12)
+    assert list(got) == [7, 11, 13]  # the four at d=1 are 7, 11, 13, 17: lowest three
+
+
+def test_search_knn_then_mask_counts_target():
+    # k nearest of ALL points include the target itself, which is then masked
+    first = [17, 13, 11, 7, 12]
+    path = np.array(first + [i for i in range(25) if i not in first])
+    case = O.SeqsimCase(dims=(5, 5), kmax=3, radius=0.0, mode=O.KNN_THEN_MASK, path=path)
+    got = O.search(case, 12)
+    assert list(got) == [7, 11]
+
+
+# --------------------------------------------------------------- kriging ----
+def numpy_krige(case, target, nbr):
+    alpha = O.scale_factor(case)
+    N = case.n_nodes
+    cents = O.grid_centroids(case.dims, case.origin, case.spacing)
+    pts = np.array([cents[a] if a < N else case.data_coords[a - N] for a in nbr])
+    rng_ = case.ranges[: case.ndim]
+    LHS = O.covariance_np(case.family, case.sill, case.nugget, rng_, pts, pts, case.rotation)
+    c0 = O.covariance_np(case.family, case.sill, case.nugget, rng_, pts, cents[target:target + 1],
+                         case.rotation)[:, 0]
+    lam = np.linalg.solve(LHS, c0)
+    var = max(case.sill - lam @ c0, 0.0)
+    return lam, np.sqrt(var), alpha
+
+
+@pytest.mark.parametrize("fam", [O.SPHERICAL, O.EXPONENTIAL, O.GAUSSIAN])
+def test_krige_matches_numpy_solve(fam):
+    rng = np.random.default_rng(7)
+    case = O.SeqsimCase(dims=(30, 20, 6), family=fam, ranges=(9.0, 7.0, 3.0), nugget=0.1,
+                        kmax=20, radius=9.0, data_coords=rng.uniform(0, 1, (25, 3)) * [30, 20, 6],
+                        path=rng.permutation(3600))
+    pos = O.positions(case)
+    checked = 0
+    for t in rng.choice(3600, 60, replace=False):
+        nbr = O.search(case, t, pos=pos)
+        if len(nbr) < 2:
+            continue
+        lam, sig, f = O.krige(case, t, nbr)
+        lam_np, sig_np, _ = numpy_krige(case, t, nbr)
+        assert f == 0
+        assert np.allclose(lam, lam_np, rtol=1e-9, atol=1e-12)
+        assert np.isclose(sig, sig_np, rtol=1e-9, atol=1e-12)
+        checked += 1
+    assert checked > 30
+
+
+def test_krige_rotation_matches_numpy():
+    th = 0.4
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1.0]])
+    case = O.SeqsimCase(dims=(20, 20), family=O.EXPONENTIAL, ranges=(10.0, 3.0), nugget=0.05,
+                        rotation=R, kmax=16, radius=10.0)
+    for t in (45, 210, 399):
+        nbr = O.search(case, t)
+        if len(nbr) < 2:
+            continue
+        lam, sig, f = O.krige(case, t, nbr)
+        lam_np, sig_np, _ = numpy_krige(case, t, nbr)
+        assert np.allclose(lam, lam_np, rtol=1e-9, atol=1e-12)
+        assert np.isclose(sig, sig_np, rtol=1e-9)
+
+
+def test_krige_single_neighbour_closed_form():
+    # k = 1: lambda = C(h)/C(0), var = C(0) - C(h)^2/C(0)
+    case = O.SeqsimCase(dims=(10,), family=O.EXPONENTIAL, ranges=(4.0,), sill=2.0, nugget=0.2)
+    lam, sig, f = O.krige(case, 5, np.array([3]))
+    ch = O.cov_u(O.EXPONENTIAL, 2.0, 0.2, 2.0 / 4.0)
+    assert np.isclose(lam[0], ch / 2.0, rtol=1e-14)
+    assert np.isclose(sig ** 2, 2.0 - ch * ch / 2.0, rtol=1e-13)
+
+
+def test_krige_colocated_datum_honours_data():
+    # a datum on the target's centre with nugget 0: lambda = e_j, sigma = 0
+    case = O.SeqsimCase(dims=(10, 10), family=O.SPHERICAL, ranges=(5.0,),
+                        data_coords=np.array([[4.5, 4.5]]))
+    N = 100
+    lam, sig, f = O.krige(case, 44, np.array([N, 43, 34]))
+    assert f == 0
+    assert np.allclose(lam, [1.0, 0.0, 0.0], atol=1e-12)
+    assert sig < 1e-6
+
+
+def test_krige_singular_system_fails_to_prior():
+    # two data on the same spot, nugget 0: LHS is singular -> status false
+    case = O.SeqsimCase(dims=(10, 10), family=O.GAUSSIAN, ranges=(50.0,),
+                        data_coords=np.array([[2.2, 2.2], [2.2, 2.2]]))
+    lam, sig, f = O.krige(case, 0, np.array([100, 101]))
+    assert f & O.FLAG_FAIL
+    assert np.all(lam == 0) and sig == 1.0
+
+
+def test_fp32_refine_close_to_fp64():
+    rng = np.random.default_rng(3)
+    case = O.SeqsimCase(dims=(24, 24, 8), family=O.EXPONENTIAL, ranges=(10.0, 10.0, 4.0),
+                        nugget=0.05, kmax=24, radius=10.0, path=rng.permutation(24 * 24 * 8))
+    pos = O.positions(case)
+    errs = []
+    for t in rng.choice(case.n_nodes, 80, replace=False):
+        nbr = O.search(case, t, pos=pos)
+        if len(nbr) < 2:
+            continue
+        l64, s64, _ = O.krige(case, t, nbr, precision=O.FP64)
+        l32, s32, f = O.krige(case, t, nbr, precision=O.FP32_REFINE)
+        errs.append(np.max(np.abs(l64 - l32)))
+        assert abs(s64 - s32) <= 1e-5
+    assert max(errs) <= 1e-5
+
+
+# ----------------------------------------------------------------- sweep ----
+def test_plan_sweep_equals_reference_cost_structure():
+    """Hoisting search + solve out of the realization loop changes nothing."""
+    rng = np.random.default_rng(11)
+    case = O.SeqsimCase(dims=(30, 25), family=O.SPHERICAL, ranges=(8.0,), kmax=12, radius=15.0,
+                        data_coords=rng.uniform(0, 25, (20, 2)), data_vals=rng.standard_normal(20),
+                        path=rng.permutation(750))
+    z = rng.standard_normal((3, 750))
+    plan = O.seqsim_plan(case)
+    a = O.seqsim_sweep(case, plan, z)
+    b, n = O.seqsim_reference_cost(case, z, threads=2)
+    assert n == 3 * 750
+    assert np.array_equal(a, b)
+
+
+def test_plan_levels_are_longest_chains():
+    rng = np.random.default_rng(5)
+    case = O.SeqsimCase(dims=(20, 20), ranges=(6.0,), kmax=8, radius=6.0, path=rng.permutation(400))
+    p = O.seqsim_plan(case)
+    for i in range(400):
+        k = 0 if p.flags[i] & 3 else p.nfound[i]
+        deps = [a for a in p.nbr[i, :k] if a < 400]
+        assert p.level[i] == (1 + max(p.level[deps]) if deps else 0)
+
+
+def test_seqsim_data_honouring_reference_case():
+    """test/field.jl:73-86: SEQSIM(maxneighbors=3), SphericalVariogram(range=35),
+    grid (0.5,0.5)-(100.5,100.5) 100x100, data at (25,25),(50,75),(75,50);
+    the node at (25,25) reproduces its datum (atol 1e-4)."""
+    rng = np.random.default_rng(2017)
+    case = O.SeqsimCase(dims=(100, 100), origin=(0.5, 0.5), family=O.SPHERICAL, ranges=(35.0,),
+                        kmax=3, radius=35.0, data_coords=np.array([[25.0, 25.0], [50.0, 75.0], [75.0, 50.0]]),
+                        data_vals=np.array([1.0, 0.0, 1.0]), path=rng.permutation(10000))
+    plan = O.seqsim_plan(case)
+    out = O.seqsim_sweep(case, plan, rng.standard_normal((2, 10000)))
+    i = 24 + 100 * 24  # inds[25, 25] (1-based) -> 0-based linear index
+    assert np.allclose(out[:, i], 1.0, atol=1e-4)
+    assert np.allclose(out[:, 74 + 100 * 49], 1.0, atol=1e-4)   # (75, 50)
+    assert np.allclose(out[:, 49 + 100 * 74], 0.0, atol=1e-4)        # (50, 75)
+
+
+def test_seqsim_unconditional_statistics():
+    """Ensemble mean ~ mu and variance ~ sill for an unconditional simulation."""
+    rng = np.random.default_rng(8)
+    case = O.SeqsimCase(dims=(40, 40), family=O.EXPONENTIAL, ranges=(6.0,), kmax=16, radius=12.0,
+                        mean=0.5, sill=2.0, path=rng.permutation(1600), mode=O.MASKED_KNN)
+    plan = O.seqsim_plan(case)
+    out = O.seqsim_sweep(case, plan, rng.standard_normal((200, 1600)))
+    assert abs(out.mean() - 0.5) < 0.1
+    assert abs(out.var(axis=0).mean() - 2.0) < 0.2
+
+
+# ----------------------------------------------------------------- LUSIM ----
+def test_lusim_oracle_matches_conditional_gaussian():
+    """d2 = C21 C11^-1 z1 and L22 L22^T = C22 - C21 C11^-1 C12 (inverse-based)."""
+    rng = np.random.default_rng(2)
+    dims = (12, 10)
+    cents = O.grid_centroids(dims)
+    dinds = np.sort(rng.choice(120, 9, replace=False))
+    z1 = rng.standard_normal(9)
+    d2, L22, sinds = O.lusim_preprocess(cents, dinds, z1, O.EXPONENTIAL, 1.0, 0.01, (6.0, 6.0))
+    C = O.covariance_np(O.EXPONENTIAL, 1.0, 0.01, (6.0, 6.0), cents, cents)
+    C11, C12, C22 = C[np.ix_(dinds, dinds)], C[np.ix_(dinds, sinds)], C[np.ix_(sinds, sinds)]
+    inv = np.linalg.inv(C11)
+    assert np.allclose(d2, C12.T @ inv @ z1, rtol=1e-9, atol=1e-12)
+    assert np.allclose(L22 @ L22.T, C22 - C12.T @ inv @ C12, atol=1e-11)
+    out = O.lusim_sample(d2, L22, dinds, sinds, z1, 0.3, rng.standard_normal((4, len(sinds))))
+    assert np.array_equal(out[:, dinds], np.broadcast_to(z1, (4, 9)))   # data reproduced exactly
+
+
+def test_lusim_oracle_unconditional_adds_mean():
+    cents = O.grid_centroids((8, 8))
+    d2, L22, sinds = O.lusim_preprocess(cents, [], [], O.SPHERICAL, 1.0, 0.0, (4.0, 4.0))
+    w = np.random.default_rng(1).standard_normal((2, 64))
+    out = O.lusim_sample(d2, L22, np.zeros(0, np.int64), sinds, [], 0.7, w)
+    assert np.allclose(out, 0.7 + w @ L22.T)
+
+
+def test_nearest_init_reference_fixtures():
+    """test/initialization.jl:9-28 expected cell mappings (1-based there)."""
+    d1 = np.loadtxt(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                               "data1D.tsv"), skiprows=1)
+    cents = O.grid_centroids((100,), origin=(-0.5,))
+    dest = O.nearest_init(cents, d1[:, :1])
+    expect = {100: 11, 81: 9, 11: 2, 21: 3, 91: 10, 51: 6, 61: 7, 71: 8, 31: 4, 41: 5, 1: 1}
+    for i, j in expect.items():
+        assert dest[j - 1] == i - 1
+    d2 = np.loadtxt(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                               "data2D.tsv"), skiprows=1)
+    cents = O.grid_centroids((100, 100), origin=(-0.5, -0.5))
+    dest = O.nearest_init(cents, d2[:, :2])
+    for i, j in {5076: 3, 2526: 1, 7551: 2}.items():
+        assert dest[j - 1] == i - 1
