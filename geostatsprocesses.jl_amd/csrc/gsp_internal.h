@@ -5,8 +5,10 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "gsp.h"
@@ -38,24 +40,106 @@ struct Fail {
     }                                \
   } while (0)
 
-// A device allocation owned by host code (freed on destruction).
+// Caching device allocator of one context. Freed blocks are kept by rounded
+// size and handed out again; all work of a context runs in order on its one
+// stream, so a block returned while a kernel still reads it is only reused by
+// work queued after that kernel.
+struct Pool {
+  int device = 0;
+  std::mutex mu;
+  std::unordered_map<size_t, std::vector<void*>> cache;
+  size_t cached = 0;
+  static constexpr size_t kMaxCached = (size_t)32 << 30;
+  static size_t round(size_t bytes) {
+    if (bytes <= 256) return 256;
+    if (bytes < ((size_t)1 << 20)) {
+      size_t r = 512;
+      while (r < bytes) r <<= 1;
+      return r;
+    }
+    const size_t mb = (size_t)1 << 20;
+    return (bytes + mb - 1) / mb * mb;
+  }
+  void* get(size_t rounded) {
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      auto it = cache.find(rounded);
+      if (it != cache.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        cached -= rounded;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, rounded);
+    if (e == hipErrorOutOfMemory) {  // give the cache back and retry once
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, rounded);
+    }
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu) failed: %s", rounded, hipGetErrorString(e));
+      throw Fail{e == hipErrorOutOfMemory ? GSP_ERR_MEMORY : GSP_ERR_DEVICE};
+    }
+    return p;
+  }
+  void put(void* p, size_t rounded) {
+    std::lock_guard<std::mutex> lock(mu);
+    if (cached + rounded > kMaxCached) {
+      (void)hipFree(p);
+      return;
+    }
+    cache[rounded].push_back(p);
+    cached += rounded;
+  }
+  void trim() {
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& kv : cache)
+      for (void* p : kv.second) (void)hipFree(p);
+    cache.clear();
+    cached = 0;
+  }
+  ~Pool() { trim(); }
+};
+
+// The pool of the context whose call is running on this thread (set by
+// gsp_guard); allocations made outside any call go straight to hipMalloc.
+extern thread_local Pool* current_pool;
+
+// A device allocation owned by host code (returned on destruction).
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  Pool* pool = nullptr;
+  size_t bytes = 0;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool)
+        pool->put(p, bytes);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    pool = nullptr;
+    bytes = 0;
   }
   void alloc(size_t count) {
     release();
     if (count == 0) count = 1;
-    GSP_HIP(hipMalloc(&p, count * sizeof(T)));
+    if (current_pool) {
+      bytes = Pool::round(count * sizeof(T));
+      p = (T*)current_pool->get(bytes);
+      pool = current_pool;
+    } else {
+      GSP_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
     n = count;
   }
   // grow-only reuse for workspaces
@@ -70,11 +154,13 @@ struct gsp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
+  std::shared_ptr<gsp::Pool> pool;  // first to be built, last to go
   // reusable workspace
   gsp::DevBuf<double> X;        // sweep state, (N + n_data) x realizations
   gsp::DevBuf<double> stage_in;  // host-pointer staging
   gsp::DevBuf<double> stage_in2;
   gsp::DevBuf<double> stage_out;
+  gsp::DevBuf<double> Zt;       // normals of one realization chunk, node-major
   gsp::DevBuf<int32_t> counters;  // [0] ticket, [1] error flag, [2..] scratch counters
   // kernel timing
   bool timing = false;
@@ -110,6 +196,11 @@ int32_t gsp_guard(gsp_ctx* ctx, F&& body) {
     return GSP_ERR_ARGUMENT;
   }
   std::lock_guard<std::mutex> lock(ctx->mu);
+  struct PoolScope {
+    gsp::Pool* prev;
+    explicit PoolScope(gsp::Pool* p) : prev(gsp::current_pool) { gsp::current_pool = p; }
+    ~PoolScope() { gsp::current_pool = prev; }
+  } scope(ctx->pool.get());
   try {
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) {
@@ -244,6 +335,7 @@ struct Offset {
 }  // namespace gsp
 
 struct gsp_seqsim_plan {
+  std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
   int64_t N = 0, n_data = 0;
   int32_t ndim = 0, kmin = 0, kmax = 0, depth = 0;
   double alpha = 0;
@@ -253,11 +345,13 @@ struct gsp_seqsim_plan {
   gsp::DevBuf<int32_t> nfound, nbr, level;
   gsp::DevBuf<double> lam, sigma;
   gsp::DevBuf<uint8_t> flags;
-  gsp::DevBuf<uint32_t> ready;  // per-node epoch flags of the dataflow sweep
-  uint32_t epoch = 0;
+  gsp::DevBuf<int32_t> pos;     // path position of every node
+  gsp::DevBuf<int32_t> order;   // nodes grouped by level
+  std::vector<int64_t> level_off;  // level l holds order[level_off[l] .. level_off[l+1])
 };
 
 struct gsp_lusim_plan {
+  std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
   int64_t N = 0, n_s = 0, n_d = 0;
   double mean = 0;
   gsp::DevBuf<double> L22;  // n_s x n_s, row-major lower

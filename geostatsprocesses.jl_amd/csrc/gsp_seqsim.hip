@@ -8,15 +8,23 @@
 //   k_plan      one wavefront per node: neighbour search (offset table over the
 //               implicit grid + a cell hash for the data) fused with covariance
 //               assembly and a register-resident Cholesky solve;
-//   k_levels    dependency depth of every node (diagnostic);
-//   k_sweep     the only sequential part: a dataflow sweep over the path in
-//               ticket order, every wave waiting on its own neighbours' ready
-//               flags, R realizations across the lanes;
-//   k_transpose X[node][r] -> out[r][node].
+//   k_dep_* / k_frontier_*  dependency level of every node (Kahn's algorithm
+//               on the reverse edges), giving the nodes grouped by level;
+//   k_sweep_level  the only sequential part: one launch per level, every node
+//               of a level independent, R realizations across the lanes; the
+//               kernel boundaries order the levels (on MI355X every XCD has its
+//               own L2, so per-node acquire/release inside one kernel would
+//               write back and invalidate L2 for every node);
+//   k_z_transpose / k_transpose  realization-major <-> node-major layouts.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
+
+#include <hipcub/hipcub.hpp>
 
 #include "gsp_internal.h"
 
@@ -24,9 +32,8 @@ namespace gsp {
 namespace {
 
 constexpr int kWaves = 4;  // wavefronts per 256-thread block
-constexpr int kSpinLimit = 1 << 22;
 
-enum { CTR_TICKET = 0, CTR_ERROR = 1, CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_DEPTH = 4, CTR_POS_ERR = 5 };
+enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5 };
 
 // ------------------------------------------------------------ wave helpers --
 __device__ inline double rl(double x, int lane) {
@@ -82,12 +89,16 @@ struct PlanArgs {
   double* sigma;
   uint8_t* flags;
   int32_t* counters;
+  int32_t* outdeg;  // dependents per node (atomic counts)
+  int32_t* indeg;   // grid neighbours per node
 };
 
 // Squared physical distance between datum `di` and the centre of node t.
 __device__ inline double data_d2(const PlanArgs& A, int di, const int* t) {
   double acc = 0.0;
-  for (int d = 0; d < A.ndim; ++d) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d >= A.ndim) break;
     double c = A.org[d] + ((double)t[d] + 0.5) * A.sp[d];
     double u = A.data[(int64_t)di * A.ndim + d] - c;
     acc = acc + u * u;
@@ -95,24 +106,62 @@ __device__ inline double data_d2(const PlanArgs& A, int di, const int* t) {
   return acc;
 }
 
-// Simple-kriging solve of the n-neighbour system held one row per lane
-// (lane j: neighbour j at scaled coordinates x). Same operation order as the
+// Per-wavefront LDS of the kriging solve: neighbour coordinates, the FP64
+// covariance matrix (both triangles; later the FP64 factor) and, for the FP32
+// path, the FP32 factor.
+template <int KB, int PREC>
+struct KrigeLds {
+  double x[KB][3];
+  double A[KB][KB + 1];
+  float F[PREC == GSP_FP32_REFINE ? KB : 1][KB + 1];
+};
+
+// (j, m) with j >= m of the e-th entry of a packed lower triangle.
+__device__ inline void tri_unpack(int e, int& j, int& m) {
+  int jj = (int)((__builtin_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  while (jj * (jj + 1) / 2 > e) --jj;
+  while ((jj + 1) * (jj + 2) / 2 <= e) ++jj;
+  j = jj;
+  m = e - jj * (jj + 1) / 2;
+}
+
+// Covariance assembly spread over the entries of the lower triangle (lane e
+// computes entries e, e + 64, ...), stored to both triangles; lane j < n also
+// returns its right-hand side c_j = C(x_j, x_target).
+template <int KB, int PREC>
+__device__ inline double assemble(KrigeLds<KB, PREC>& L, const DevModel& M, int n, int lane,
+                                  double x0, double x1, double x2, double t0, double t1,
+                                  double t2) {
+  if (lane < n) {
+    L.x[lane][0] = x0;
+    L.x[lane][1] = x1;
+    L.x[lane][2] = x2;
+  }
+  wave_sync();
+  const int T = n * (n + 1) / 2;
+  for (int e = lane; e < T; e += 64) {
+    int j, m;
+    tri_unpack(e, j, m);
+    const double v = cov_points(M, L.x[j][0], L.x[j][1], L.x[j][2], L.x[m][0], L.x[m][1],
+                                L.x[m][2]);
+    L.A[j][m] = v;
+    L.A[m][j] = v;
+  }
+  const double c = lane < n ? cov_points(M, x0, x1, x2, t0, t1, t2) : 0.0;
+  wave_sync();
+  return c;
+}
+
+// Simple-kriging solve, one matrix row per lane. Same operation order as the
 // oracle: right-looking Cholesky whose per-element subtraction order is
 // ascending, forward substitution ascending, back substitution descending.
-template <int KB>
-__device__ inline int krige_fp64(const DevModel& M, int n, int lane, double x0, double x1,
-                                 double x2, double t0, double t1, double t2, double C0,
+// Expects L.A assembled; leaves the factor in L.A.
+template <int KB, int PREC>
+__device__ inline int krige_fp64(KrigeLds<KB, PREC>& L, int n, int lane, double c, double C0,
                                  double& lam_out, double& sigma_out) {
   double a[KB];
 #pragma unroll
-  for (int m = 0; m < KB; ++m) {
-    if (m < n) {
-      a[m] = cov_points(M, x0, x1, x2, rl(x0, m), rl(x1, m), rl(x2, m));
-    } else {
-      a[m] = 0.0;
-    }
-  }
-  const double c = cov_points(M, x0, x1, x2, t0, t1, t2);
+  for (int m = 0; m < KB; ++m) a[m] = (m < n && lane < n) ? L.A[lane][m] : 0.0;
 #pragma unroll
   for (int col = 0; col < KB; ++col) {
     if (col < n) {
@@ -138,23 +187,21 @@ __device__ inline int krige_fp64(const DevModel& M, int n, int lane, double x0, 
       if (lane > m) s = s - a[m] * ym;
     }
   }
+  wave_sync();
+  if (lane < n) {
+#pragma unroll
+    for (int m = 0; m < KB; ++m)
+      if (m <= lane) L.A[lane][m] = a[m];
+  }
+  wave_sync();
   double s2 = y, lam = 0.0;
-#pragma unroll
-  for (int m = KB - 1; m >= 0; --m) {
-    if (m < n) {
-      const double lm = rl(s2, m) / rl(a[m], m);
-      if (lane == m) lam = lm;
-#pragma unroll
-      for (int jj = 0; jj < m; ++jj) {
-        const double v = rl(a[jj], m);
-        if (lane == jj) s2 = s2 - v * lm;
-      }
-    }
+  for (int m = n - 1; m >= 0; --m) {
+    const double lm = rl(s2, m) / L.A[m][m];
+    if (lane == m) lam = lm;
+    if (lane < m) s2 = s2 - L.A[m][lane] * lm;
   }
   double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < KB; ++j)
-    if (j < n) acc = acc + rl(lam, j) * rl(c, j);
+  for (int j = 0; j < n; ++j) acc = acc + rl(lam, j) * rl(c, j);
   double var = C0 - acc;
   if (var < 0.0) var = 0.0;
   lam_out = lam;
@@ -167,22 +214,11 @@ __device__ inline int krige_fp64(const DevModel& M, int n, int lane, double x0, 
 // refinement step to reach FP64-level weights (or singular), so the node is
 // solved in FP64 instead, which also keeps the prior fallback identical.
 template <int KB>
-__device__ inline int krige_fp32r(const DevModel& M, int n, int lane, double x0, double x1,
-                                  double x2, double t0, double t1, double t2, double C0,
-                                  double& lam_out, double& sigma_out) {
-  const float tau = (float)(M.sill * 0x1p-12);
-  double a[KB];
+__device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int lane, double c,
+                                  double C0, float tau, double& lam_out, double& sigma_out) {
   float f[KB];
 #pragma unroll
-  for (int m = 0; m < KB; ++m) {
-    if (m < n) {
-      a[m] = cov_points(M, x0, x1, x2, rl(x0, m), rl(x1, m), rl(x2, m));
-    } else {
-      a[m] = 0.0;
-    }
-    f[m] = (float)a[m];
-  }
-  const double c = cov_points(M, x0, x1, x2, t0, t1, t2);
+  for (int m = 0; m < KB; ++m) f[m] = (m < n && lane < n) ? (float)L.A[lane][m] : 0.0f;
   bool ok = true;
 #pragma unroll
   for (int col = 0; col < KB; ++col) {
@@ -203,10 +239,14 @@ __device__ inline int krige_fp32r(const DevModel& M, int n, int lane, double x0,
       }
     }
   }
-  if (!ok) {
-    const int r = krige_fp64<KB>(M, n, lane, x0, x1, x2, t0, t1, t2, C0, lam_out, sigma_out);
-    return r | GSP_FLAG_FP64_FALLBACK;
+  if (!ok) return krige_fp64<KB, GSP_FP32_REFINE>(L, n, lane, c, C0, lam_out, sigma_out) |
+                  GSP_FLAG_FP64_FALLBACK;
+  if (lane < n) {
+#pragma unroll
+    for (int m = 0; m < KB; ++m)
+      if (m <= lane) L.F[lane][m] = f[m];
   }
+  wave_sync();
   // two FP32 solves with the same factor: lambda0 = A^-1 c, delta = A^-1 r
   float rhs = (float)c, sol = 0.0f;
 #pragma unroll
@@ -221,24 +261,15 @@ __device__ inline int krige_fp32r(const DevModel& M, int n, int lane, double x0,
       }
     }
     float s2 = y, l = 0.0f;
-#pragma unroll
-    for (int m = KB - 1; m >= 0; --m) {
-      if (m < n) {
-        const float lm = rlf(s2, m) / rlf(f[m], m);
-        if (lane == m) l = lm;
-#pragma unroll
-        for (int jj = 0; jj < m; ++jj) {
-          const float v = rlf(f[jj], m);
-          if (lane == jj) s2 = s2 - v * lm;
-        }
-      }
+    for (int m = n - 1; m >= 0; --m) {
+      const float lm = rlf(s2, m) / L.F[m][m];
+      if (lane == m) l = lm;
+      if (lane < m) s2 = s2 - L.F[m][lane] * lm;
     }
     if (pass == 0) {
       sol = l;
       double r = c;
-#pragma unroll
-      for (int m = 0; m < KB; ++m)
-        if (m < n) r = r - a[m] * (double)rlf(l, m);
+      for (int m = 0; m < n; ++m) r = r - (lane < n ? L.A[lane][m] : 0.0) * (double)rlf(l, m);
       rhs = (float)r;
     } else {
       lam_out = (double)sol + (double)l;
@@ -246,22 +277,25 @@ __device__ inline int krige_fp32r(const DevModel& M, int n, int lane, double x0,
   }
   const double lam = lam_out;
   double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < KB; ++j)
-    if (j < n) acc = acc + rl(lam, j) * rl(c, j);
+  for (int j = 0; j < n; ++j) acc = acc + rl(lam, j) * rl(c, j);
   double var = C0 - acc;
   if (var < 0.0) var = 0.0;
   sigma_out = __builtin_sqrt(var);
   return 0;
 }
 
+// Wavefronts per block of the plan kernel (LDS per wave grows with KB^2).
+constexpr int plan_waves(int KB) { return KB <= 32 ? 4 : (KB <= 48 ? 2 : 1); }
+
 template <int KB, int PREC>
-__global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
+__global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) {
+  constexpr int kWaves = plan_waves(KB);
   __shared__ int s_gidx[kWaves][64];
   __shared__ double s_gd2[kWaves][64];
   __shared__ int s_didx[kWaves][128];
   __shared__ double s_dd2[kWaves][128];
   __shared__ int s_fidx[kWaves][64];
+  __shared__ KrigeLds<KB, PREC> s_krige[kWaves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int nx = A.dims[0], ny = A.dims[1], nz = A.dims[2];
@@ -310,7 +344,9 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
       int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
       const bool all = !(Dk < __builtin_inf());
       const double rho = all ? 0.0 : __builtin_sqrt(Dk) * (1.0 + 0x1p-20) + 0x1p-40;
-      for (int d = 0; d < A.ndim; ++d) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d >= A.ndim) break;
         if (all) {
           lo[d] = 0;
           hi[d] = A.hd[d] - 1;
@@ -347,6 +383,7 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
                 int keep_rank[2] = {128, 128};
                 int kid[2] = {0, 0};
                 double kd[2] = {0.0, 0.0};
+#pragma unroll
                 for (int h = 0; h < 2; ++h) {
                   const int m = lane + 64 * h;
                   if (m < nd) {
@@ -359,6 +396,7 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
                   }
                 }
                 wave_sync();
+#pragma unroll
                 for (int h = 0; h < 2; ++h)
                   if (keep_rank[h] < A.kmax) {
                     s_dd2[w][keep_rank[h]] = kd[h];
@@ -390,6 +428,7 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
       my_rank[0] = r;
       my_id[0] = id;
     }
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int m = lane + 64 * h;
       if (m < nd) {
@@ -403,6 +442,7 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
       }
     }
     wave_sync();
+#pragma unroll
     for (int h = 0; h < 3; ++h)
       if (my_rank[h] < A.kmax) s_fidx[w][my_rank[h]] = my_id[h];
     wave_sync();
@@ -442,10 +482,13 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
       const double t0 = A.ao[0] + ((double)ti[0] + 0.5) * A.as[0];
       const double t1 = A.ao[1] + ((double)ti[1] + 0.5) * A.as[1];
       const double t2 = A.ao[2] + ((double)ti[2] + 0.5) * A.as[2];
-      if (PREC == GSP_FP32_REFINE)
-        flag = krige_fp32r<KB>(A.M, n, lane, x0, x1, x2, t0, t1, t2, A.M.sill, lam, sig);
+      KrigeLds<KB, PREC>& KL = s_krige[w];
+      const double c = assemble<KB, PREC>(KL, A.M, n, lane, x0, x1, x2, t0, t1, t2);
+      if constexpr (PREC == GSP_FP32_REFINE)
+        flag = krige_fp32r<KB>(KL, n, lane, c, A.M.sill, (float)(A.M.sill * 0x1p-12), lam, sig);
       else
-        flag = krige_fp64<KB>(A.M, n, lane, x0, x1, x2, t0, t1, t2, A.M.sill, lam, sig);
+        flag = krige_fp64<KB, PREC>(KL, n, lane, c, A.M.sill, lam, sig);
+      wave_sync();
       if (flag & GSP_FLAG_FAIL) {
         lam = 0.0;
         sig = A.prior_sigma;
@@ -455,10 +498,15 @@ __global__ __launch_bounds__(256) void k_plan(const PlanArgs A) {
       A.nbr[(int64_t)i * A.kmax + lane] = nb;
       A.lam[(int64_t)i * A.kmax + lane] = lane < n ? lam : 0.0;
     }
+    // dependency degrees for the level computation (grid neighbours only)
+    const bool dep = !(flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) && lane < n && nb < A.N;
+    if (dep) atomicAdd(&A.outdeg[nb], 1);
+    const int ndep = __popcll(__ballot(dep));
     if (lane == 0) {
       A.sigma[i] = sig;
       A.flags[i] = (uint8_t)flag;
       A.nfound[i] = n;
+      A.indeg[i] = ndep;
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
@@ -492,132 +540,131 @@ __global__ void k_positions(const int64_t* path, int64_t N, int32_t* pos, int32_
   }
 }
 
-__device__ inline bool wave_all(bool v) { return __ballot(v) == __ballot(true); }
+// Dependency levels by Kahn's algorithm on the device. level(i) = 0 without
+// grid neighbours, else 1 + the largest level of its grid neighbours; the
+// nodes of level l are exactly the frontier after l rounds of removing
+// finished nodes. Reverse edges (neighbour -> dependent) live in CSR form.
+__device__ inline int deps_of(const uint8_t* flags, const int32_t* nfound, int i) {
+  return (flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : nfound[i];
+}
 
-// Dependency depth: level(i) = 0 without grid neighbours, else
-// 1 + max level of its grid neighbours. Waves take 64 consecutive path
-// positions in ticket order; a neighbour always has a smaller position, so the
-// wave holding the earliest unfinished position can always complete.
-__global__ __launch_bounds__(256) void k_levels(const int64_t* path, int32_t N, int32_t kmax,
-                                                const int32_t* nfound, const uint8_t* flags,
-                                                const int32_t* nbr, int32_t* level,
-                                                int32_t* counters) {
-  const int lane = threadIdx.x & 63;
-  while (true) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&counters[CTR_TICKET], 64);
-    base = rli(base, 0);
-    if (base >= N) break;
-    const int q = base + lane;
-    const bool act = q < N;
-    const int i = act ? (int)path[q] : 0;
-    const int k = act && !(flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? nfound[i] : 0;
-    const int32_t* nb = nbr + (int64_t)i * kmax;
-    int j = 0, lv = 0, spins = 0;
-    bool done = !act;
-    while (true) {
-      if (!done) {
-        while (j < k) {
-          const int a = nb[j];
-          if (a < N) {
-            const int la = __hip_atomic_load(&level[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (la < 0) break;
-            lv = la + 1 > lv ? la + 1 : lv;
-          }
-          ++j;
-        }
-        if (j == k || ++spins > kSpinLimit) {
-          if (j != k) atomicOr(&counters[CTR_ERROR], 1);
-          __hip_atomic_store(&level[i], lv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          atomicMax(&counters[CTR_DEPTH], lv + 1);
-          done = true;
-        }
-      }
-      if (wave_all(done)) break;
-      __builtin_amdgcn_s_sleep(2);
+__global__ void k_dep_fill(int32_t N, int32_t kmax, const int32_t* nfound, const uint8_t* flags,
+                           const int32_t* nbr, const int32_t* rowptr, int32_t* cursor,
+                           int32_t* deps) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = deps_of(flags, nfound, (int)i);
+    const int32_t* nb = nbr + i * kmax;
+    for (int j = 0; j < k; ++j) {
+      const int a = nb[j];
+      if (a < N) deps[rowptr[a] + atomicAdd(&cursor[a], 1)] = (int)i;
     }
   }
 }
 
+__global__ void k_frontier0(int32_t N, const int32_t* indeg, int32_t* order, int32_t* cnt,
+                            int32_t* level) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (indeg[i] == 0) {
+      order[atomicAdd(&cnt[0], 1)] = (int)i;
+      level[i] = 0;
+    }
+  }
+}
+
+// Frontier L occupies order[off[L] .. off[L] + cnt[L]); both are final when
+// this launch starts. Dependents whose last unfinished neighbour is in the
+// frontier join frontier L + 1, appended after it: each block gathers its
+// appends in an LDS queue and reserves space with one global atomic.
+constexpr int kQueue = 2048;
+
+__global__ __launch_bounds__(256) void k_frontier_step(int32_t L, const int32_t* rowptr,
+                                                       const int32_t* deps, int32_t* indeg,
+                                                       int32_t* order, int32_t* off, int32_t* cnt,
+                                                       int32_t* level) {
+  __shared__ int32_t q[kQueue];
+  __shared__ int32_t qn, base;
+  const int32_t start = off[L], size = cnt[L], next = start + size;
+  if (blockIdx.x == 0 && threadIdx.x == 0) off[L + 1] = next;
+  for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x; t0 < size;
+       t0 += (int64_t)gridDim.x * blockDim.x) {
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    const int64_t t = t0 + threadIdx.x;
+    if (t < size) {
+      const int i = order[start + t];
+      for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+        const int d = deps[e];
+        if (atomicSub(&indeg[d], 1) == 1) {
+          level[d] = L + 1;
+          const int slot = atomicAdd(&qn, 1);
+          if (slot < kQueue)
+            q[slot] = d;
+          else
+            order[next + atomicAdd(&cnt[L + 1], 1)] = d;
+        }
+      }
+    }
+    __syncthreads();
+    const int n = qn < kQueue ? qn : kQueue;
+    if (threadIdx.x == 0) base = n ? atomicAdd(&cnt[L + 1], n) : 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) order[next + base + j] = q[j];
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------- the sweep --
-struct SweepArgs {
-  const int64_t* path;
-  int32_t n_path, N, kmax;
-  int32_t RC;     // realizations in this chunk (<= 64)
-  int32_t shift;  // log2 of lanes per node
-  int32_t r0;     // first realization of the chunk
+// Zt[q * RC + r] = z[(r0 + r) * n_path + q]: normals of one realization chunk
+// made node-major (path order) through a 64-position LDS tile.
+__global__ __launch_bounds__(256) void k_z_transpose(const double* z, int32_t n_path, int32_t RC,
+                                                     int32_t r0, double* Zt) {
+  __shared__ double tile[64][65];
+  const int64_t q0 = (int64_t)blockIdx.x * 64;
+  const int cnt = (int)((n_path - q0) < 64 ? (n_path - q0) : 64);
+  for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
+    const int rr = t / cnt, qq = t - rr * cnt;
+    tile[rr][qq] = z[(int64_t)(r0 + rr) * n_path + q0 + qq];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
+    const int qq = t / RC, rr = t - qq * RC;
+    Zt[(q0 + qq) * RC + rr] = tile[rr][qq];
+  }
+}
+
+struct LevelArgs {
+  const int32_t* order;  // the nodes of this level
+  int32_t count;
+  int32_t N, kmax, RC, shift;
+  const int32_t* pos;
   const int32_t* nfound;
   const uint8_t* flags;
   const int32_t* nbr;
   const double* lam;
   const double* sigma;
   double mean;
-  const double* z;  // R x n_path
-  double* X;        // (N + n_data) x RC
-  uint32_t* ready;
-  uint32_t epoch;
-  int32_t* counters;
+  const double* Zt;  // n_path x RC
+  double* X;         // (N + n_data) x RC
 };
 
-// One wave serves 64 >> shift consecutive path positions, 1 << shift lanes per
-// node (lane = realization). A node is computed once all its grid neighbours
-// carry this sweep's epoch; then its values are stored and its flag released.
-// z reads are strided per lane, but consecutive tickets touch consecutive
-// addresses, so each z line is fetched once and reused from L2.
-__global__ __launch_bounds__(256) void k_sweep(const SweepArgs A) {
-  const int lane = threadIdx.x & 63;
-  const int per = 1 << A.shift;
-  const int G = 64 >> A.shift;
-  const int slot = lane >> A.shift;
-  const int r = lane & (per - 1);
-  while (true) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&A.counters[CTR_TICKET], G);
-    base = rli(base, 0);
-    if (base >= A.n_path) break;
-    const int q = base + slot;
-    const bool act = q < A.n_path;
-    const bool lact = act && r < A.RC;
-    const int i = act ? (int)A.path[q] : 0;
-    const int k = act && !(A.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? A.nfound[i] : 0;
-    const int32_t* nb = A.nbr + (int64_t)i * A.kmax;
-    const double* lm = A.lam + (int64_t)i * A.kmax;
-    int j = 0, spins = 0;
-    bool done = !act;
-    while (true) {
-      if (!done) {
-        while (j < k) {
-          const int a = nb[j];
-          if (a < A.N &&
-              __hip_atomic_load(&A.ready[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != A.epoch)
-            break;
-          ++j;
-        }
-        const bool ready = j == k;
-        if (ready || ++spins > kSpinLimit) {
-          if (!ready) atomicOr(&A.counters[CTR_ERROR], 2);
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          double acc = 0.0;
-          for (int jj = 0; jj < k; ++jj) {
-            const int a = nb[jj];
-            const double xa = lact ? A.X[(int64_t)a * A.RC + r] : 0.0;
-            acc = acc + lm[jj] * (xa - A.mean);
-          }
-          const double mu = A.mean + acc;
-          if (lact) {
-            const double zz = A.z[(int64_t)(A.r0 + r) * A.n_path + q];
-            A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * zz;
-          }
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          if (r == 0)
-            __hip_atomic_store(&A.ready[i], A.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          done = true;
-        }
-      }
-      if (wave_all(done)) break;
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
+// All nodes of one level are independent: x_i = (mean + sum_j lambda_ij
+// (x_nbr(j) - mean)) + sigma_i z_i for every realization, 1 << shift lanes per
+// node (lane = realization), neighbours from earlier levels.
+__global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t slot = t >> A.shift;
+  const int r = (int)(t & ((1 << A.shift) - 1));
+  if (slot >= A.count || r >= A.RC) return;
+  const int i = A.order[slot];
+  const int k = (A.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : A.nfound[i];
+  const int32_t* nb = A.nbr + (int64_t)i * A.kmax;
+  const double* lm = A.lam + (int64_t)i * A.kmax;
+  double acc = 0.0;
+  for (int j = 0; j < k; ++j) acc = acc + lm[j] * (A.X[(int64_t)nb[j] * A.RC + r] - A.mean);
+  const double mu = A.mean + acc;
+  A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * A.Zt[(int64_t)A.pos[i] * A.RC + r];
 }
 
 // X rows N..N+n_data-1 hold the conditioning data for every realization.
@@ -647,6 +694,25 @@ __global__ __launch_bounds__(256) void k_transpose(const double* X, int32_t N, i
 }
 
 // ------------------------------------------------------------- host side --
+// Optional phase trace (environment GSP_TRACE=1): synchronises the stream at
+// each mark and prints the host-side elapsed time of the phase to stderr.
+struct PhaseTrace {
+  bool on;
+  hipStream_t s;
+  std::chrono::steady_clock::time_point t;
+  explicit PhaseTrace(hipStream_t st) : on(std::getenv("GSP_TRACE") != nullptr), s(st) {
+    if (on) t = std::chrono::steady_clock::now();
+  }
+  void mark(const char* what) {
+    if (!on) return;
+    (void)hipStreamSynchronize(s);
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[gsp] %-24s %9.3f ms\n", what,
+                 std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
+  }
+};
+
 int grid_for(int64_t work, int per_block, int cap) {
   int64_t g = (work + per_block - 1) / per_block;
   if (g < 1) g = 1;
@@ -664,12 +730,10 @@ std::vector<Offset> build_offsets(const gsp_grid& g, double r2, int kmax, int mo
   int64_t lim[3] = {0, 0, 0};
   double bound = r2;
   const bool ball = r2 < std::numeric_limits<double>::infinity();
-  if (ball) {
-    for (int d = 0; d < nd; ++d) {
-      const double f = std::floor(std::sqrt(r2) / g.spacing[d]) + 1.0;
-      lim[d] = std::min<int64_t>(ext[d], (int64_t)std::min(f, 1e9));
-    }
-  } else if (mode == GSP_KNN_THEN_MASK) {
+  if (mode == GSP_KNN_THEN_MASK) {
+    // the kmax nearest in-grid nodes of any node lie within the Chebyshev
+    // radius M at which the smallest in-grid window holds kmax + 1 nodes;
+    // farther offsets can never be among the k nearest, ball or not
     int64_t N = 1;
     for (int d = 0; d < nd; ++d) N *= g.dims[d];
     const int64_t need = std::min<int64_t>(kmax + 1, N);
@@ -686,10 +750,15 @@ std::vector<Offset> build_offsets(const gsp_grid& g, double r2, int kmax, int mo
       const double t = (double)M * g.spacing[d];
       acc = acc + t * t;
     }
-    bound = acc;
+    bound = std::min(acc, r2);
     for (int d = 0; d < nd; ++d) {
       const double f = std::floor(std::sqrt(bound) / g.spacing[d]) + 1.0;
       lim[d] = std::min<int64_t>(ext[d], (int64_t)f);
+    }
+  } else if (ball) {
+    for (int d = 0; d < nd; ++d) {
+      const double f = std::floor(std::sqrt(r2) / g.spacing[d]) + 1.0;
+      lim[d] = std::min<int64_t>(ext[d], (int64_t)std::min(f, 1e9));
     }
   } else {
     for (int d = 0; d < nd; ++d) lim[d] = ext[d];
@@ -788,11 +857,13 @@ int64_t validate_grid(const gsp_grid& g) {
 }
 
 template <int KB>
-void launch_plan(const PlanArgs& A, int precision, int blocks, hipStream_t s) {
+void launch_plan(const PlanArgs& A, int precision, int64_t N, hipStream_t s) {
+  constexpr int W = plan_waves(KB);
+  const int blocks = grid_for(N, W, 65536 / W);
   if (precision == GSP_FP32_REFINE)
-    hipLaunchKernelGGL((k_plan<KB, GSP_FP32_REFINE>), dim3(blocks), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_plan<KB, GSP_FP32_REFINE>), dim3(blocks), dim3(64 * W), 0, s, A);
   else
-    hipLaunchKernelGGL((k_plan<KB, GSP_FP64>), dim3(blocks), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((k_plan<KB, GSP_FP64>), dim3(blocks), dim3(64 * W), 0, s, A);
 }
 
 void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_plan* plan) {
@@ -818,6 +889,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   GSP_REQUIRE(nmax <= 64, GSP_ERR_UNSUPPORTED, "maxneighbors above 64 is not supported");
 
   hipStream_t s = ctx->stream;
+  PhaseTrace tr(s);
   const int64_t nD = p->n_data;
   std::vector<double> data((size_t)(nD * nd));
   if (nD > 0) {
@@ -866,6 +938,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
       hitems[(size_t)(hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] = (int32_t)i;
   }
 
+  tr.mark("host tables");
   plan->N = N;
   plan->n_data = nD;
   plan->ndim = nd;
@@ -880,10 +953,9 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   plan->sigma.alloc((size_t)N);
   plan->flags.alloc((size_t)N);
   plan->level.alloc((size_t)N);
-  plan->ready.alloc((size_t)N);
-
-  DevBuf<int32_t> pos;
-  pos.alloc((size_t)N);
+  plan->order.alloc((size_t)N);
+  plan->pos.alloc((size_t)N);
+  DevBuf<int32_t>& pos = plan->pos;
   DevBuf<Offset> dtable;
   dtable.alloc(table.size());
   DevBuf<double> ddata;
@@ -906,8 +978,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
                          hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->counters.p, 0, (int64_t)16);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, pos.p, -1, N);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->level.p, -1, N);
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->ready.p, 0u, N);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->level.p, 0, N);
   hipLaunchKernelGGL(k_positions, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->path.p, N,
                      pos.p, ctx->counters.p);
   GSP_HIP(hipGetLastError());
@@ -918,6 +989,12 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   GSP_REQUIRE(pos_err == 0, GSP_ERR_UNSUPPORTED,
               "the path must be a permutation of the grid ids 0..N-1");
 
+  DevBuf<int32_t> outdeg, indeg;
+  outdeg.alloc((size_t)N + 1);
+  indeg.alloc((size_t)N);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 1, 256, 4096)), dim3(256), 0, s, outdeg.p, 0,
+                     N + 1);
+  tr.mark("alloc+upload+positions");
   PlanArgs A;
   std::memset(&A, 0, sizeof(A));
   A.N = (int32_t)N;
@@ -953,34 +1030,85 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   A.sigma = plan->sigma.p;
   A.flags = plan->flags.p;
   A.counters = ctx->counters.p;
+  A.outdeg = outdeg.p;
+  A.indeg = indeg.p;
 
-  const int blocks = grid_for(N, kWaves, 16384);
   {
   TimedLaunch tl(ctx, GSP_K_PLAN);
-  if (nmax <= 8) launch_plan<8>(A, p->precision, blocks, s);
-  else if (nmax <= 16) launch_plan<16>(A, p->precision, blocks, s);
-  else if (nmax <= 24) launch_plan<24>(A, p->precision, blocks, s);
-  else if (nmax <= 32) launch_plan<32>(A, p->precision, blocks, s);
-  else if (nmax <= 48) launch_plan<48>(A, p->precision, blocks, s);
-  else launch_plan<64>(A, p->precision, blocks, s);
+  if (nmax <= 8) launch_plan<8>(A, p->precision, N, s);
+  else if (nmax <= 16) launch_plan<16>(A, p->precision, N, s);
+  else if (nmax <= 24) launch_plan<24>(A, p->precision, N, s);
+  else if (nmax <= 32) launch_plan<32>(A, p->precision, N, s);
+  else if (nmax <= 48) launch_plan<48>(A, p->precision, N, s);
+  else launch_plan<64>(A, p->precision, N, s);
   GSP_HIP(hipGetLastError());
   }
+  tr.mark("search+solve kernel");
   {
     TimedLaunch tl(ctx, GSP_K_LEVELS);
-    hipLaunchKernelGGL(k_levels, dim3(grid_for(N, 256, 2048)), dim3(256), 0, s, plan->path.p,
-                       (int32_t)N, nmax, plan->nfound.p, plan->flags.p, plan->nbr.p,
-                       plan->level.p, ctx->counters.p);
-    GSP_HIP(hipGetLastError());
+    DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
+    rowptr.alloc((size_t)N + 1);
+    cnt.alloc((size_t)N + 16);
+    off.alloc((size_t)N + 16);
+    const int nb = grid_for(N, 256, 4096);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 16, 256, 4096)), dim3(256), 0, s, cnt.p, 0,
+                       N + 16);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, off.p, 0, (int64_t)1);
+    size_t tmp_bytes = 0;
+    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, outdeg.p, rowptr.p, (int)N + 1, s));
+    DevBuf<char> tmp;
+    tmp.alloc(tmp_bytes);
+    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, outdeg.p, rowptr.p, (int)N + 1, s));
+    int32_t n_edges = 0;
+    GSP_HIP(hipMemcpyAsync(&n_edges, rowptr.p + N, 4, hipMemcpyDeviceToHost, s));
+    GSP_HIP(hipStreamSynchronize(s));
+    tr.mark("levels: count+scan");
+    deps.alloc((size_t)std::max(n_edges, 1));
+    cursor.alloc((size_t)N);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, cursor.p, 0, N);
+    hipLaunchKernelGGL(k_dep_fill, dim3(nb), dim3(256), 0, s, (int32_t)N, nmax, plan->nfound.p,
+                       plan->flags.p, plan->nbr.p, rowptr.p, cursor.p, deps.p);
+    hipLaunchKernelGGL(k_frontier0, dim3(nb), dim3(256), 0, s, (int32_t)N, indeg.p,
+                       plan->order.p, cnt.p, plan->level.p);
+    tr.mark("levels: fill");
+    // one launch per level; the frontier sizes are checked every kBatch levels
+    constexpr int kBatch = 16;
+    const int fb = grid_for(N / 16, 256, 1024);
+    int32_t depth = -1;
+    std::vector<int32_t> hc(kBatch + 1);
+    for (int32_t L = 0; depth < 0; L += kBatch) {
+      GSP_REQUIRE(L + kBatch + 1 < N + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
+      for (int b = 0; b < kBatch; ++b)
+        hipLaunchKernelGGL(k_frontier_step, dim3(fb), dim3(256), 0, s, L + b, rowptr.p, deps.p,
+                           indeg.p, plan->order.p, off.p, cnt.p, plan->level.p);
+      GSP_HIP(hipMemcpyAsync(hc.data(), cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, s));
+      GSP_HIP(hipStreamSynchronize(s));
+      for (int b = 0; b <= kBatch; ++b)
+        if (hc[(size_t)b] == 0) {
+          depth = L + b;
+          break;
+        }
+    }
+    tr.mark("levels: frontiers");
+    GSP_REQUIRE(depth >= 1 || N == 0, GSP_ERR_INTERNAL, "empty dependency levels");
+    std::vector<int32_t> hcnt((size_t)depth);
+    GSP_HIP(hipMemcpyAsync(hcnt.data(), cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, s));
+    GSP_HIP(hipStreamSynchronize(s));
+    plan->depth = depth;
+    plan->level_off.assign((size_t)depth + 1, 0);
+    for (int32_t l = 0; l < depth; ++l)
+      plan->level_off[(size_t)l + 1] = plan->level_off[(size_t)l] + hcnt[(size_t)l];
+    GSP_REQUIRE(plan->level_off[(size_t)depth] == N, GSP_ERR_INTERNAL,
+                "dependency levels cover %lld of %lld nodes",
+                (long long)plan->level_off[(size_t)depth], (long long)N);
   }
   int32_t ctr[16];
   GSP_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
   GSP_HIP(hipStreamSynchronize(s));
-  GSP_REQUIRE(ctr[CTR_ERROR] == 0, GSP_ERR_INTERNAL,
-              "dependency wait exceeded its limit while ordering the plan");
+  tr.mark("levels: offsets+free");
   flush_timings(ctx);
   plan->n_prior = ctr[CTR_PRIOR];
   plan->n_fallback = ctr[CTR_FALLBACK];
-  plan->depth = ctr[CTR_DEPTH];
 }
 
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
@@ -990,46 +1118,45 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
   const int RCmax = 64;
   const int RC0 = R < RCmax ? R : RCmax;
   ctx->X.reserve((size_t)((N + nD) * RC0));
-  ctx->counters.reserve(16);
-  auto* mplan = const_cast<gsp_seqsim_plan*>(plan);
+  ctx->Zt.reserve((size_t)(N * RC0));
   for (int r0 = 0; r0 < R; r0 += RCmax) {
     const int RC = (R - r0) < RCmax ? (R - r0) : RCmax;
     int shift = 0;
     while ((1 << shift) < RC) ++shift;
-    mplan->epoch += 1;
-    if (mplan->epoch == 0) {  // wrapped: clear the flags
-      hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s,
-                         mplan->ready.p, 0u, N);
-      mplan->epoch = 1;
-    }
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->counters.p, 0, (int64_t)16);
     if (nD > 0)
       hipLaunchKernelGGL(k_data_rows, dim3(grid_for(nD * RC, 256, 1024)), dim3(256), 0, s,
                          ctx->X.p, (int32_t)N, (int32_t)nD, RC, data_vals);
-    SweepArgs A;
-    A.path = plan->path.p;
-    A.n_path = (int32_t)N;
+    {
+      TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
+      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, z,
+                         (int32_t)N, RC, r0, ctx->Zt.p);
+    }
+    LevelArgs A;
     A.N = (int32_t)N;
     A.kmax = plan->kmax;
     A.RC = RC;
     A.shift = shift;
-    A.r0 = r0;
+    A.pos = plan->pos.p;
     A.nfound = plan->nfound.p;
     A.flags = plan->flags.p;
     A.nbr = plan->nbr.p;
     A.lam = plan->lam.p;
     A.sigma = plan->sigma.p;
     A.mean = plan->mean;
-    A.z = z;
+    A.Zt = ctx->Zt.p;
     A.X = ctx->X.p;
-    A.ready = mplan->ready.p;
-    A.epoch = mplan->epoch;
-    A.counters = ctx->counters.p;
-    const int nodes_per_wave = 64 >> shift;
-    const int64_t waves = (N + nodes_per_wave - 1) / nodes_per_wave;
     {
       TimedLaunch tl(ctx, GSP_K_SWEEP);
-      hipLaunchKernelGGL(k_sweep, dim3(grid_for(waves, kWaves, 2048)), dim3(256), 0, s, A);
+      for (int32_t l = 0; l < plan->depth; ++l) {
+        const int64_t off = plan->level_off[(size_t)l];
+        const int64_t cnt = plan->level_off[(size_t)l + 1] - off;
+        if (cnt == 0) continue;
+        A.order = plan->order.p + off;
+        A.count = (int32_t)cnt;
+        const int64_t threads = cnt << shift;
+        hipLaunchKernelGGL(k_sweep_level, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                           A);
+      }
     }
     {
       TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
