@@ -111,12 +111,24 @@ def main():
     ap.add_argument("--workload", default="seqsim-3d-128x128x64", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32r", choices=["fp32r", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: this build has no multi-GPU path "
                  "(realizations would shard across GPUs with no exchange); run with --gpus 1")
 
     from geostatsprocesses import _lib
+
+    if args.copy_calibration:
+        ctx = _lib.default_context(0)
+        nbytes = args.copy_calibration << 20
+        src = _lib.DeviceBuffer(ctx, (nbytes // 8,))
+        dst = _lib.DeviceBuffer(ctx, (nbytes // 8,))
+        for _ in range(3):
+            _lib.check(_lib.load().gsp_selftest_copy(ctx.handle, dst.ptr, src.ptr, nbytes))
+        print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
+        return
 
     spec = WORKLOADS[args.workload]
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
@@ -165,9 +177,11 @@ def main():
 
     # algorithmic bytes (realizations resident; neighbour gathers are re-reads)
     sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
-    plan_bytes = N * (8 + spec["kmax"] * (4 + 8) + 8 + 1 + 4)    # path, nbr + lambda, sigma, flags, counts
+    search_bytes = N * (8 + spec["kmax"] * 4 + 4)                # path, nbr, nfound
+    plan_bytes = N * (spec["kmax"] * (4 + 8) + 4 + 8 + 1 + 4)    # nbr + nfound in, lambda, sigma, flags, degrees
     kern = {}
-    for slot, name, nbytes in ((_lib.K_PLAN, "plan", plan_bytes), (_lib.K_SWEEP, "sweep", sweep_bytes),
+    for slot, name, nbytes in ((_lib.K_SEARCH, "search", search_bytes), (_lib.K_PLAN, "krige", plan_bytes),
+                               (_lib.K_SWEEP, "sweep", sweep_bytes),
                                (_lib.K_LEVELS, "levels", None), (_lib.K_TRANSPOSE, "transpose", 2 * 8 * N * R)):
         ms, launches = tim[slot]
         if launches == 0:
@@ -178,7 +192,7 @@ def main():
             per_launch = nbytes / (launches / args.steps)
             kern[name]["algorithmic_bytes_per_launch"] = int(per_launch)
             kern[name]["achieved_GBs"] = round(per_launch / (avg * 1e-3) / 1e9, 1)
-    dominant = max(("plan", "sweep"), key=lambda k: kern.get(k, {}).get("avg_ms", 0) *
+    dominant = max(("search", "krige", "sweep"), key=lambda k: kern.get(k, {}).get("avg_ms", 0) *
                    kern.get(k, {}).get("launches_per_step", 0))
     d = kern[dominant]
     achieved = d["achieved_GBs"]

@@ -212,6 +212,13 @@ int32_t gsp_memcpy_to_host(gsp_ctx* ctx, void* dst, const void* src, int64_t byt
   });
 }
 
+int32_t gsp_selftest_copy(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  return gsp_guard(ctx, [&] {
+    GSP_REQUIRE(dst && src && bytes >= 0 && bytes % 16 == 0, GSP_ERR_ARGUMENT, "bad copy");
+    gsp::copy_kernel(ctx, dst, src, bytes);
+  });
+}
+
 int32_t gsp_selftest_mfma(gsp_ctx* ctx, int32_t fp32, double* out) {
   return gsp_guard(ctx, [&] {
     GSP_REQUIRE(out, GSP_ERR_ARGUMENT, "null output");

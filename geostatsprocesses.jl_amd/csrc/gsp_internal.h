@@ -369,4 +369,5 @@ void lusim_factor_build(gsp_ctx* ctx, const gsp_lusim_problem* p, gsp_lusim_plan
 void lusim_sample_run(gsp_ctx* ctx, const gsp_lusim_plan* plan, int32_t R, const double* w,
                       double* out);
 void mfma_probe(gsp_ctx* ctx, int fp32, double* host_out);
+void copy_kernel(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes);
 }  // namespace gsp

@@ -304,6 +304,24 @@ void cov_matrix(hipStream_t s, const CovArgs& A, const int64_t* rows, const int6
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(256) void k_copy16(uint4* dst, const uint4* src, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+}  // namespace
+
+void copy_kernel(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  const int64_t n = bytes / 16;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  if (n > 0)
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (uint4*)dst,
+                       (const uint4*)src, n);
+  GSP_HIP(hipGetLastError());
+  GSP_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 void mfma_probe(gsp_ctx* ctx, int fp32, double* host_out) {
   DevBuf<double> d;
   d.alloc(512);

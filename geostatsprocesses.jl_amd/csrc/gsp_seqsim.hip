@@ -5,9 +5,10 @@
 // k x k simple-kriging system and draws one normal. The neighbour sets, weights
 // and variances depend only on the path and the geometry, so this file splits
 // the work into
-//   k_plan      one wavefront per node: neighbour search (offset table over the
-//               implicit grid + a cell hash for the data) fused with covariance
-//               assembly and a register-resident Cholesky solve;
+//   k_search    one wavefront per node: neighbour search (offset table over the
+//               implicit grid + a cell hash for the data);
+//   k_krige     one wavefront per node: covariance assembly and a
+//               register-resident Cholesky solve of the kriging system;
 //   k_dep_* / k_frontier_*  dependency level of every node (Kahn's algorithm
 //               on the reverse edges), giving the nodes grouped by level;
 //   k_sweep_level  the only sequential part: one launch per level, every node
@@ -18,6 +19,7 @@
 //   k_z_transpose / k_transpose  realization-major <-> node-major layouts.
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -66,6 +68,13 @@ __device__ inline bool key_less(double da, int ia, double db, int ib) {
 }
 
 // ------------------------------------------------------------ plan kernel --
+// A conditioning datum in a hash-cell list: physical coordinates inline.
+struct DataItem {
+  double x[3];
+  int32_t id;
+  int32_t pad_;
+};
+
 struct PlanArgs {
   int32_t N, ndim, n_data;
   int32_t dims[3];
@@ -82,7 +91,9 @@ struct PlanArgs {
   const Offset* table;
   const double* data;  // n_data x ndim, physical
   const int32_t* hstart;
-  const int32_t* hitems;
+  const DataItem* hitems;
+  int32_t hexp;  // 1: each hash cell lists every datum near it (read one list per node)
+  int32_t hB;    // hash cell width in grid cells
   int32_t* nfound;
   int32_t* nbr;
   double* lam;
@@ -93,14 +104,14 @@ struct PlanArgs {
   int32_t* indeg;   // grid neighbours per node
 };
 
-// Squared physical distance between datum `di` and the centre of node t.
-__device__ inline double data_d2(const PlanArgs& A, int di, const int* t) {
+// Squared physical distance between a datum and the centre of node t.
+__device__ inline double data_d2(const PlanArgs& A, const DataItem& it, const int* t) {
   double acc = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d >= A.ndim) break;
     double c = A.org[d] + ((double)t[d] + 0.5) * A.sp[d];
-    double u = A.data[(int64_t)di * A.ndim + d] - c;
+    double u = it.x[d] - c;
     acc = acc + u * u;
   }
   return acc;
@@ -284,23 +295,24 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
   return 0;
 }
 
-// Wavefronts per block of the plan kernel (LDS per wave grows with KB^2).
-constexpr int plan_waves(int KB) { return KB <= 32 ? 4 : (KB <= 48 ? 2 : 1); }
-
-template <int KB, int PREC>
-__global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) {
-  constexpr int kWaves = plan_waves(KB);
+// Neighbour search, one wavefront per node (in path order): the first kmax
+// usable grid offsets of the sorted offset table, the data of the node's hash
+// list within the k-th grid distance, their merge by (d2, index) and the mask.
+// Writes nbr[i][0..n) and nfound[i].
+__global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_gidx[kWaves][64];
   __shared__ double s_gd2[kWaves][64];
   __shared__ int s_didx[kWaves][128];
   __shared__ double s_dd2[kWaves][128];
   __shared__ int s_fidx[kWaves][64];
-  __shared__ KrigeLds<KB, PREC> s_krige[kWaves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int nx = A.dims[0], ny = A.dims[1], nz = A.dims[2];
   const int64_t wstride = (int64_t)gridDim.x * kWaves;
 
+  // the first 64 offsets serve every node of this wave
+  Offset e0 = {0.0, 0, 0, 0, 0};
+  if (lane < A.T) e0 = A.table[lane];
   for (int64_t q = (int64_t)blockIdx.x * kWaves + w; q < A.N; q += wstride) {
     const int i = (int)A.path[q];
     const int pi = (int)q;
@@ -317,7 +329,7 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) 
       int j = 0;
       double d2 = 0.0;
       if (t < A.T) {
-        const Offset e = A.table[t];
+        const Offset e = t0 == 0 ? e0 : A.table[t];
         const int x = ti[0] + e.dx, y = ti[1] + e.dy, z = ti[2] + e.dz;
         ok = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
         if (ok) {
@@ -341,78 +353,87 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) 
     // ---- data candidates with d2 <= Dk, from the cell hash
     int nd = 0;
     if (A.n_data > 0) {
-      int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-      const bool all = !(Dk < __builtin_inf());
-      const double rho = all ? 0.0 : __builtin_sqrt(Dk) * (1.0 + 0x1p-20) + 0x1p-40;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (d >= A.ndim) break;
-        if (all) {
-          lo[d] = 0;
-          hi[d] = A.hd[d] - 1;
-        } else {
-          const double c = A.org[d] + ((double)ti[d] + 0.5) * A.sp[d];
-          double fl = __builtin_floor((c - rho - A.org[d]) / A.hcs[d]);
-          double fh = __builtin_floor((c + rho - A.org[d]) / A.hcs[d]);
-          fl = fl < 0.0 ? 0.0 : (fl > A.hd[d] - 1 ? A.hd[d] - 1 : fl);
-          fh = fh < 0.0 ? 0.0 : (fh > A.hd[d] - 1 ? A.hd[d] - 1 : fh);
-          lo[d] = (int)fl;
-          hi[d] = (int)fh;
-        }
-      }
-      for (int cz = lo[2]; cz <= hi[2]; ++cz)
-        for (int cy = lo[1]; cy <= hi[1]; ++cy)
-          for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-            const int cell = cx + A.hd[0] * (cy + A.hd[1] * cz);
-            const int s = A.hstart[cell], e = A.hstart[cell + 1];
-            for (int b0 = s; b0 < e; b0 += 64) {
-              bool ok = false;
-              int di = 0;
-              double d2 = 0.0;
-              if (b0 + lane < e) {
-                di = A.hitems[b0 + lane];
-                d2 = data_d2(A, di, ti);
-                ok = d2 <= Dk && d2 <= A.r2;
-              }
-              const uint64_t b = __ballot(ok);
-              const int add = __popcll(b);
-              if (add == 0) continue;
-              if (nd + add > 128) {
-                // keep the best kmax entries found so far
-                wave_sync();
-                int keep_rank[2] = {128, 128};
-                int kid[2] = {0, 0};
-                double kd[2] = {0.0, 0.0};
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                  const int m = lane + 64 * h;
-                  if (m < nd) {
-                    kd[h] = s_dd2[w][m];
-                    kid[h] = s_didx[w][m];
-                    int r = 0;
-                    for (int m2 = 0; m2 < nd; ++m2)
-                      r += key_less(s_dd2[w][m2], s_didx[w][m2], kd[h], kid[h]);
-                    keep_rank[h] = r;
-                  }
-                }
-                wave_sync();
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-                  if (keep_rank[h] < A.kmax) {
-                    s_dd2[w][keep_rank[h]] = kd[h];
-                    s_didx[w][keep_rank[h]] = kid[h];
-                  }
-                nd = nd < A.kmax ? nd : A.kmax;
-                wave_sync();
-              }
-              if (ok) {
-                const int r = nd + mbcnt(b);
-                s_didx[w][r] = A.N + di;
-                s_dd2[w][r] = d2;
-              }
-              nd += add;
-            }
+      // scan one hash-cell list, appending data with d2 <= Dk to the LDS list
+      auto scan_list = [&](int s0, int e0l) {
+        for (int b0 = s0; b0 < e0l; b0 += 64) {
+          bool ok = false;
+          int di = 0;
+          double d2 = 0.0;
+          if (b0 + lane < e0l) {
+            const DataItem it = A.hitems[b0 + lane];
+            di = it.id;
+            d2 = data_d2(A, it, ti);
+            ok = d2 <= Dk && d2 <= A.r2;
           }
+          const uint64_t b = __ballot(ok);
+          const int add = __popcll(b);
+          if (add == 0) continue;
+          if (nd + add > 128) {
+            // keep the best kmax entries found so far
+            wave_sync();
+            int keep_rank[2] = {128, 128};
+            int kid[2] = {0, 0};
+            double kd[2] = {0.0, 0.0};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int m = lane + 64 * h;
+              if (m < nd) {
+                kd[h] = s_dd2[w][m];
+                kid[h] = s_didx[w][m];
+                int r = 0;
+                for (int m2 = 0; m2 < nd; ++m2)
+                  r += key_less(s_dd2[w][m2], s_didx[w][m2], kd[h], kid[h]);
+                keep_rank[h] = r;
+              }
+            }
+            wave_sync();
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+              if (keep_rank[h] < A.kmax) {
+                s_dd2[w][keep_rank[h]] = kd[h];
+                s_didx[w][keep_rank[h]] = kid[h];
+              }
+            nd = nd < A.kmax ? nd : A.kmax;
+            wave_sync();
+          }
+          if (ok) {
+            const int r = nd + mbcnt(b);
+            s_didx[w][r] = A.N + di;
+            s_dd2[w][r] = d2;
+          }
+          nd += add;
+        }
+      };
+      if (A.hexp) {
+        const int cell = ti[0] / A.hB + A.hd[0] * (ti[1] / A.hB + A.hd[1] * (ti[2] / A.hB));
+        scan_list(A.hstart[cell], A.hstart[cell + 1]);
+      } else {
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        const bool all = !(Dk < __builtin_inf());
+        const double rho = all ? 0.0 : __builtin_sqrt(Dk) * (1.0 + 0x1p-20) + 0x1p-40;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (d >= A.ndim) break;
+          if (all) {
+            lo[d] = 0;
+            hi[d] = A.hd[d] - 1;
+          } else {
+            const double c = A.org[d] + ((double)ti[d] + 0.5) * A.sp[d];
+            double fl = __builtin_floor((c - rho - A.org[d]) / A.hcs[d]);
+            double fh = __builtin_floor((c + rho - A.org[d]) / A.hcs[d]);
+            fl = fl < 0.0 ? 0.0 : (fl > A.hd[d] - 1 ? A.hd[d] - 1 : fl);
+            fh = fh < 0.0 ? 0.0 : (fh > A.hd[d] - 1 ? A.hd[d] - 1 : fh);
+            lo[d] = (int)fl;
+            hi[d] = (int)fh;
+          }
+        }
+        for (int cz = lo[2]; cz <= hi[2]; ++cz)
+          for (int cy = lo[1]; cy <= hi[1]; ++cy)
+            for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+              const int cell = cx + A.hd[0] * (cy + A.hd[1] * cz);
+              scan_list(A.hstart[cell], A.hstart[cell + 1]);
+            }
+      }
       wave_sync();
     }
 
@@ -459,6 +480,33 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) 
     wave_sync();
     const int nb = lane < n ? s_fidx[w][lane] : -1;
 
+    if (lane < A.kmax) A.nbr[(int64_t)i * A.kmax + lane] = nb;
+    if (lane == 0) A.nfound[i] = n;
+  }
+}
+
+// Wavefronts per block of the kriging kernel (LDS per wave grows with KB^2).
+constexpr int plan_waves(int KB) { return KB <= 32 ? 4 : (KB <= 48 ? 2 : 1); }
+
+// Kriging systems, one wavefront per node (domain order): covariance assembly,
+// Cholesky solve, weights, sigma and flags, plus the dependency degrees.
+template <int KB, int PREC>
+__global__ __launch_bounds__(64 * plan_waves(KB)) void k_krige(const PlanArgs A, const int32_t* list,
+                                                               int32_t count) {
+  constexpr int kW = plan_waves(KB);
+  __shared__ KrigeLds<KB, PREC> s_krige[kW];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int nx = A.dims[0], ny = A.dims[1];
+  const int64_t wstride = (int64_t)gridDim.x * kW;
+  for (int64_t idx = (int64_t)blockIdx.x * kW + w; idx < count; idx += wstride) {
+    const int64_t i = list[idx];
+    const int n = A.nfound[i];
+    const int nb = lane < n ? A.nbr[i * A.kmax + lane] : -1;
+    int ti[3];
+    ti[0] = (int)(i % nx);
+    ti[1] = (int)((i / nx) % ny);
+    ti[2] = (int)(i / ((int64_t)nx * ny));
     // ---- kriging
     int flag = 0;
     double lam = 0.0, sig = A.prior_sigma;
@@ -494,10 +542,7 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) 
         sig = A.prior_sigma;
       }
     }
-    if (lane < A.kmax) {
-      A.nbr[(int64_t)i * A.kmax + lane] = nb;
-      A.lam[(int64_t)i * A.kmax + lane] = lane < n ? lam : 0.0;
-    }
+    if (lane < A.kmax) A.lam[i * A.kmax + lane] = lane < n ? lam : 0.0;
     // dependency degrees for the level computation (grid neighbours only)
     const bool dep = !(flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) && lane < n && nb < A.N;
     if (dep) atomicAdd(&A.outdeg[nb], 1);
@@ -510,6 +555,334 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_plan(const PlanArgs A) 
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
+  }
+}
+
+// ---- four systems per wavefront (n <= 16) ---------------------------------
+// Lane 16 g + r holds row r of system g. Values are exchanged inside a 16-lane
+// row with DPP row broadcasts. The systems of one wavefront are padded to the
+// largest size among them with sill * identity rows, which leaves the
+// factorisation and both substitutions of every real row bit-identical.
+template <int M>
+__device__ inline double rbd(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + M, 0xF,
+                                                            0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32),
+                                                            0x150 + M, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+template <int M>
+__device__ inline float rbf(float x) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x150 + M, 0xF, 0xF, false));
+}
+
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// visits N-1, N-2, ..., 0
+template <int N, class F>
+__device__ __forceinline__ void static_for_down(F&& f) {
+  if constexpr (N > 0) {
+    f(std::integral_constant<int, N - 1>{});
+    static_for_down<N - 1>(f);
+  }
+}
+
+template <int PREC>
+struct Krige16Lds {
+  double x[4][16][3];
+  double A[4][16][17];
+  float F[PREC == GSP_FP32_REFINE ? 4 : 1][16][17];
+};
+
+// FP64 Cholesky + substitutions of the packed systems (rows in a[]). Returns
+// true in the lanes of a group whose factorisation met a non-positive pivot.
+template <int PREC>
+__device__ inline bool packed_fp64(Krige16Lds<PREC>& L, int g, int r, int nmax, double c, double C0,
+                                   double* a, double& lam_out, double& sigma_out) {
+  bool fail = false;
+  static_for<0, 16>([&](auto colc) {
+    constexpr int col = decltype(colc)::value;
+    if (col < nmax) {
+      const double piv = rbd<col>(a[col]);
+      fail = fail || !(piv > 0.0);
+      const double lcc = __builtin_sqrt(piv);
+      a[col] = r == col ? lcc : (r > col ? a[col] / lcc : a[col]);
+      static_for<col + 1, 16>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if (m < nmax) {
+          const double lmc = rbd<m>(a[col]);
+          a[m] = a[m] - a[col] * lmc;
+        }
+      });
+    }
+  });
+  double s = c, y = 0.0;
+  static_for<0, 16>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < nmax) {
+      const double ym = rbd<m>(s) / rbd<m>(a[m]);
+      if (r == m) y = ym;
+      if (r > m) s = s - a[m] * ym;
+    }
+  });
+  wave_sync();
+  static_for<0, 16>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m <= r) L.A[g][r][m] = a[m];
+  });
+  wave_sync();
+  double s2 = y, lam = 0.0;
+  static_for_down<16>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < nmax) {
+      const double lm = rbd<m>(s2) / L.A[g][m][m];
+      if (r == m) lam = lm;
+      if (r < m) s2 = s2 - L.A[g][m][r] * lm;
+    }
+  });
+  double acc = 0.0;
+  static_for<0, 16>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < nmax) acc = acc + rbd<j>(lam) * rbd<j>(c);
+  });
+  double var = C0 - acc;
+  if (var < 0.0) var = 0.0;
+  lam_out = lam;
+  sigma_out = __builtin_sqrt(var);
+  return fail;
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t* list,
+                                                 int32_t count) {
+  __shared__ Krige16Lds<PREC> s_lds[kWaves];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int nx = A.dims[0], ny = A.dims[1];
+  const double C0 = A.M.sill;
+  Krige16Lds<PREC>& L = s_lds[w];
+  const int64_t stride = (int64_t)gridDim.x * kWaves * 4;
+  for (int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 4; base < count; base += stride) {
+    const int64_t slot = base + g;
+    const bool act = slot < count;
+    const int64_t i = act ? list[slot] : 0;
+    const int n = act ? A.nfound[i] : 0;
+    const int nb = (act && r < n) ? A.nbr[i * A.kmax + r] : -1;
+    const bool solve = act && n >= A.kmin;
+    const int ns = solve ? n : 0;
+    int nmax = rli(ns, 0);
+    nmax = rli(ns, 16) > nmax ? rli(ns, 16) : nmax;
+    nmax = rli(ns, 32) > nmax ? rli(ns, 32) : nmax;
+    nmax = rli(ns, 48) > nmax ? rli(ns, 48) : nmax;
+    double lam = 0.0, sig = A.prior_sigma;
+    int flag = act && !solve ? GSP_FLAG_FEW : 0;
+    if (nmax > 0) {
+      double x0 = 0.0, x1 = 0.0, x2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+      if (r < ns) {
+        if (nb < A.N) {
+          const int c0 = nb % nx, c1 = (nb / nx) % ny, c2 = nb / (nx * ny);
+          x0 = A.ao[0] + ((double)c0 + 0.5) * A.as[0];
+          x1 = A.ao[1] + ((double)c1 + 0.5) * A.as[1];
+          x2 = A.ao[2] + ((double)c2 + 0.5) * A.as[2];
+        } else {
+          const double* p = A.data + (int64_t)(nb - A.N) * A.ndim;
+          x0 = A.alpha * p[0];
+          if (A.ndim > 1) x1 = A.alpha * p[1];
+          if (A.ndim > 2) x2 = A.alpha * p[2];
+        }
+        L.x[g][r][0] = x0;
+        L.x[g][r][1] = x1;
+        L.x[g][r][2] = x2;
+      }
+      if (solve) {
+        const int i0 = (int)(i % nx), i1 = (int)((i / nx) % ny), i2 = (int)(i / ((int64_t)nx * ny));
+        t0 = A.ao[0] + ((double)i0 + 0.5) * A.as[0];
+        t1 = A.ao[1] + ((double)i1 + 0.5) * A.as[1];
+        t2 = A.ao[2] + ((double)i2 + 0.5) * A.as[2];
+      }
+      wave_sync();
+      // covariance entries of each system's lower triangle over its 16 lanes
+      const int T = ns * (ns + 1) / 2, Tmax = nmax * (nmax + 1) / 2;
+      for (int e = r; e < Tmax; e += 16) {
+        if (e < T) {
+          int j, m;
+          tri_unpack(e, j, m);
+          const double v = cov_points(A.M, L.x[g][j][0], L.x[g][j][1], L.x[g][j][2],
+                                      L.x[g][m][0], L.x[g][m][1], L.x[g][m][2]);
+          L.A[g][j][m] = v;
+          L.A[g][m][j] = v;
+        }
+      }
+      const double c = r < ns ? cov_points(A.M, x0, x1, x2, t0, t1, t2) : 0.0;
+      wave_sync();
+      double a[16];
+      static_for<0, 16>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        a[m] = (r < ns && m < ns) ? L.A[g][r][m] : (m == r ? C0 : 0.0);
+      });
+      bool fail;
+      bool fell_back = false;
+      if constexpr (PREC == GSP_FP32_REFINE) {
+        const float tau = (float)(C0 * 0x1p-12);
+        float f[16];
+        static_for<0, 16>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          f[m] = (float)a[m];
+        });
+        bool f32fail = false;
+        static_for<0, 16>([&](auto colc) {
+          constexpr int col = decltype(colc)::value;
+          if (col < nmax) {
+            const float piv = rbf<col>(f[col]);
+            f32fail = f32fail || !(piv > tau);
+            const float lcc = __builtin_sqrtf(piv);
+            f[col] = r == col ? lcc : (r > col ? f[col] / lcc : f[col]);
+            static_for<col + 1, 16>([&](auto mc) {
+              constexpr int m = decltype(mc)::value;
+              if (m < nmax) {
+                const float lmc = rbf<m>(f[col]);
+                f[m] = f[m] - f[col] * lmc;
+              }
+            });
+          }
+        });
+        wave_sync();
+        static_for<0, 16>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          if (m <= r) L.F[g][r][m] = f[m];
+        });
+        wave_sync();
+        float rhs = (float)c, sol = 0.0f, l = 0.0f;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+          float s = rhs, y = 0.0f;
+          static_for<0, 16>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (m < nmax) {
+              const float ym = rbf<m>(s) / rbf<m>(f[m]);
+              if (r == m) y = ym;
+              if (r > m) s = s - f[m] * ym;
+            }
+          });
+          float s2 = y;
+          l = 0.0f;
+          static_for_down<16>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (m < nmax) {
+              const float lm = rbf<m>(s2) / L.F[g][m][m];
+              if (r == m) l = lm;
+              if (r < m) s2 = s2 - L.F[g][m][r] * lm;
+            }
+          });
+          if (pass == 0) {
+            sol = l;
+            double res = c;
+            static_for<0, 16>([&](auto mc) {
+              constexpr int m = decltype(mc)::value;
+              if (m < nmax) res = res - a[m] * (double)rbf<m>(l);
+            });
+            rhs = (float)res;
+          }
+        }
+        lam = (double)sol + (double)l;
+        double acc = 0.0;
+        static_for<0, 16>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < nmax) acc = acc + rbd<j>(lam) * rbd<j>(c);
+        });
+        double var = C0 - acc;
+        if (var < 0.0) var = 0.0;
+        sig = __builtin_sqrt(var);
+        fail = false;
+        // systems whose FP32 factor failed are solved again in FP64
+        if (__ballot(f32fail && r == 0)) {
+          double lam64, sig64;
+          wave_sync();
+          const bool fail64 = packed_fp64<PREC>(L, g, r, nmax, c, C0, a, lam64, sig64);
+          if (f32fail) {
+            lam = lam64;
+            sig = sig64;
+            fail = fail64;
+            fell_back = true;
+          }
+        }
+      } else {
+        fail = packed_fp64<PREC>(L, g, r, nmax, c, C0, a, lam, sig);
+      }
+      wave_sync();
+      if (solve) {
+        if (fell_back) flag |= GSP_FLAG_FP64_FALLBACK;
+        if (fail) {
+          flag |= GSP_FLAG_FAIL;
+          lam = 0.0;
+          sig = A.prior_sigma;
+        }
+      } else {
+        lam = 0.0;
+        sig = A.prior_sigma;
+      }
+    }
+    if (act) {
+      for (int col = r; col < A.kmax; col += 16) A.lam[i * A.kmax + col] = col < n ? lam : 0.0;
+    }
+    const bool dep = solve && !(flag & GSP_FLAG_FAIL) && r < n && nb < A.N;
+    if (dep) atomicAdd(&A.outdeg[nb], 1);
+    const uint64_t dm = __ballot(dep);
+    if (act && r == 0) {
+      A.sigma[i] = sig;
+      A.flags[i] = (uint8_t)flag;
+      A.indeg[i] = __popcll((dm >> (16 * g)) & 0xFFFFull);
+      if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
+      if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
+    }
+  }
+}
+
+// Nodes bucketed by neighbour count (counting sort), so packed wavefronts
+// hold systems of one size.
+__global__ __launch_bounds__(256) void k_n_hist(const int32_t* nfound, int32_t N, int32_t* hist) {
+  __shared__ int32_t h[65];
+  for (int t = threadIdx.x; t < 65; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[nfound[i]], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < 65; t += blockDim.x)
+    if (h[t]) atomicAdd(&hist[t], h[t]);
+}
+
+// Each block counts its nodes per bucket in LDS, reserves one range per bucket
+// with a single global atomic, then places its nodes.
+constexpr int kScatterChunk = 4096;
+
+__global__ __launch_bounds__(256) void k_n_scatter(const int32_t* nfound, int32_t N,
+                                                   int32_t* cursor, int32_t* list) {
+  __shared__ int32_t h[65], base[65];
+  const int64_t i0 = (int64_t)blockIdx.x * kScatterChunk;
+  const int64_t i1 = i0 + kScatterChunk < N ? i0 + kScatterChunk : N;
+  for (int t = threadIdx.x; t < 65; t += blockDim.x) h[t] = 0;
+  __syncthreads();
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) atomicAdd(&h[nfound[i]], 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < 65; t += blockDim.x) {
+    base[t] = h[t] ? atomicAdd(&cursor[t], h[t]) : 0;
+    h[t] = 0;
+  }
+  __syncthreads();
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int k = nfound[i];
+    list[base[k] + atomicAdd(&h[k], 1)] = (int)i;
   }
 }
 
@@ -857,13 +1230,17 @@ int64_t validate_grid(const gsp_grid& g) {
 }
 
 template <int KB>
-void launch_plan(const PlanArgs& A, int precision, int64_t N, hipStream_t s) {
+void launch_krige(const PlanArgs& A, int precision, const int32_t* list, int64_t count,
+                  hipStream_t s) {
+  if (count <= 0) return;
   constexpr int W = plan_waves(KB);
-  const int blocks = grid_for(N, W, 65536 / W);
+  const int blocks = grid_for(count, W, 65536 / W);
   if (precision == GSP_FP32_REFINE)
-    hipLaunchKernelGGL((k_plan<KB, GSP_FP32_REFINE>), dim3(blocks), dim3(64 * W), 0, s, A);
+    hipLaunchKernelGGL((k_krige<KB, GSP_FP32_REFINE>), dim3(blocks), dim3(64 * W), 0, s, A, list,
+                       (int32_t)count);
   else
-    hipLaunchKernelGGL((k_plan<KB, GSP_FP64>), dim3(blocks), dim3(64 * W), 0, s, A);
+    hipLaunchKernelGGL((k_krige<KB, GSP_FP64>), dim3(blocks), dim3(64 * W), 0, s, A, list,
+                       (int32_t)count);
 }
 
 void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_plan* plan) {
@@ -905,40 +1282,111 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   const double r2 = p->radius > 0 ? p->radius * p->radius : inf;
   std::vector<Offset> table = build_offsets(g, r2, nmax, p->search_mode);
 
-  // data hash: cells of B x B x B grid cells, clamped to the grid box
+  // Data hash. Expanded mode: cells of B^ndim grid cells, each listing every
+  // datum within rho_max of the cell (rho_max bounds the k-th grid distance of
+  // any node, so one list holds all candidates of a node). Box mode (large
+  // rho_max, e.g. masked search in a big ball): each datum in its own cell and
+  // nodes visit every cell overlapping their ball.
   int hd[3] = {1, 1, 1};
   double hcs[3] = {1.0, 1.0, 1.0};
-  std::vector<int32_t> hstart(2, 0), hitems(1, 0);
+  int hexp = 0;
+  int64_t hB = 1;
+  std::vector<int32_t> hstart(2, 0);
+  std::vector<DataItem> hitems(1);
   if (nD > 0) {
-    int64_t B = (int64_t)std::llround(std::pow((double)N / (double)nD, 1.0 / nd));
-    if (B < 1) B = 1;
-    int64_t ncell = 1;
-    for (int d = 0; d < nd; ++d) {
-      hd[d] = (int)((g.dims[d] + B - 1) / B);
-      hcs[d] = (double)B * g.spacing[d];
-      ncell *= hd[d];
-    }
-    std::vector<int64_t> cell((size_t)nD);
-    hstart.assign((size_t)ncell + 1, 0);
-    for (int64_t i = 0; i < nD; ++i) {
-      int64_t c = 0, stride = 1;
+    const double rho_max = table.empty() ? 0.0 : std::sqrt(table.back().d2);
+    const bool bounded = !table.empty() && (p->search_mode == GSP_KNN_THEN_MASK || r2 < inf);
+    double smin = g.spacing[0];
+    for (int d = 1; d < nd; ++d) smin = std::min(smin, g.spacing[d]);
+    auto make_item = [&](int64_t i) {
+      DataItem it;
+      std::memset(&it, 0, sizeof(it));
+      for (int d = 0; d < nd; ++d) it.x[d] = data[(size_t)(i * nd + d)];
+      it.id = (int32_t)i;
+      return it;
+    };
+    if (bounded) {
+      hB = std::max<int64_t>(1, (int64_t)std::llround(rho_max / smin));
+      int64_t ncell = 1, span = 1;
       for (int d = 0; d < nd; ++d) {
-        double f = std::floor((data[(size_t)(i * nd + d)] - g.origin[d]) / hcs[d]);
-        f = std::min(std::max(f, 0.0), (double)(hd[d] - 1));
-        c += (int64_t)f * stride;
-        stride *= hd[d];
+        hd[d] = (int)((g.dims[d] + hB - 1) / hB);
+        hcs[d] = (double)hB * g.spacing[d];
+        ncell *= hd[d];
+        span *= std::min<int64_t>(hd[d], (int64_t)std::ceil(2.0 * rho_max / hcs[d]) + 2);
       }
-      cell[(size_t)i] = c;
-      hstart[(size_t)c + 1]++;
+      // worth it when the lists stay short (about one 64-lane pass per node)
+      hexp = (span * nD <= ((int64_t)1 << 24) && ncell <= ((int64_t)1 << 26) &&
+              span * nD <= 64 * ncell)
+                 ? 1
+                 : 0;
+      if (hexp) {
+        const double rho = rho_max * (1.0 + 1e-9) + 1e-12;
+        std::vector<int64_t> lo((size_t)(nD * 3)), hi((size_t)(nD * 3));
+        hstart.assign((size_t)ncell + 1, 0);
+        for (int64_t i = 0; i < nD; ++i) {
+          int64_t l[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+          for (int d = 0; d < nd; ++d) {
+            const double x = data[(size_t)(i * nd + d)];
+            double fl = std::floor((x - rho - g.origin[d]) / hcs[d]);
+            double fh = std::floor((x + rho - g.origin[d]) / hcs[d]);
+            l[d] = (int64_t)std::min(std::max(fl, 0.0), (double)(hd[d] - 1));
+            h[d] = (int64_t)std::min(std::max(fh, 0.0), (double)(hd[d] - 1));
+          }
+          for (int d = 0; d < 3; ++d) {
+            lo[(size_t)(i * 3 + d)] = l[d];
+            hi[(size_t)(i * 3 + d)] = h[d];
+          }
+          for (int64_t cz = l[2]; cz <= h[2]; ++cz)
+            for (int64_t cy = l[1]; cy <= h[1]; ++cy)
+              for (int64_t cx = l[0]; cx <= h[0]; ++cx)
+                hstart[(size_t)(cx + hd[0] * (cy + (int64_t)hd[1] * cz)) + 1]++;
+        }
+        for (int64_t c = 0; c < ncell; ++c) hstart[(size_t)c + 1] += hstart[(size_t)c];
+        hitems.assign((size_t)std::max<int32_t>(hstart[(size_t)ncell], 1), DataItem{});
+        std::vector<int32_t> fill((size_t)ncell, 0);
+        for (int64_t i = 0; i < nD; ++i) {
+          const DataItem it = make_item(i);
+          for (int64_t cz = lo[(size_t)(i * 3 + 2)]; cz <= hi[(size_t)(i * 3 + 2)]; ++cz)
+            for (int64_t cy = lo[(size_t)(i * 3 + 1)]; cy <= hi[(size_t)(i * 3 + 1)]; ++cy)
+              for (int64_t cx = lo[(size_t)(i * 3)]; cx <= hi[(size_t)(i * 3)]; ++cx) {
+                const int64_t c = cx + hd[0] * (cy + (int64_t)hd[1] * cz);
+                hitems[(size_t)(hstart[(size_t)c] + fill[(size_t)c]++)] = it;
+              }
+        }
+      }
     }
-    for (int64_t c = 0; c < ncell; ++c) hstart[(size_t)c + 1] += hstart[(size_t)c];
-    hitems.assign((size_t)nD, 0);
-    std::vector<int32_t> fill((size_t)ncell, 0);
-    for (int64_t i = 0; i < nD; ++i)
-      hitems[(size_t)(hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] = (int32_t)i;
+    if (!hexp) {
+      int64_t B = (int64_t)std::llround(std::pow((double)N / (double)nD, 1.0 / nd));
+      if (B < 1) B = 1;
+      hB = B;
+      int64_t ncell = 1;
+      for (int d = 0; d < nd; ++d) {
+        hd[d] = (int)((g.dims[d] + B - 1) / B);
+        hcs[d] = (double)B * g.spacing[d];
+        ncell *= hd[d];
+      }
+      std::vector<int64_t> cell((size_t)nD);
+      hstart.assign((size_t)ncell + 1, 0);
+      for (int64_t i = 0; i < nD; ++i) {
+        int64_t c = 0, stride = 1;
+        for (int d = 0; d < nd; ++d) {
+          double f = std::floor((data[(size_t)(i * nd + d)] - g.origin[d]) / hcs[d]);
+          f = std::min(std::max(f, 0.0), (double)(hd[d] - 1));
+          c += (int64_t)f * stride;
+          stride *= hd[d];
+        }
+        cell[(size_t)i] = c;
+        hstart[(size_t)c + 1]++;
+      }
+      for (int64_t c = 0; c < ncell; ++c) hstart[(size_t)c + 1] += hstart[(size_t)c];
+      hitems.assign((size_t)nD, DataItem{});
+      std::vector<int32_t> fill((size_t)ncell, 0);
+      for (int64_t i = 0; i < nD; ++i)
+        hitems[(size_t)(hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] =
+            make_item(i);
+    }
   }
 
-  tr.mark("host tables");
   plan->N = N;
   plan->n_data = nD;
   plan->ndim = nd;
@@ -960,7 +1408,8 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   dtable.alloc(table.size());
   DevBuf<double> ddata;
   ddata.alloc((size_t)std::max<int64_t>(1, nD * nd));
-  DevBuf<int32_t> dhstart, dhitems;
+  DevBuf<int32_t> dhstart;
+  DevBuf<DataItem> dhitems;
   dhstart.alloc(hstart.size());
   dhitems.alloc(hitems.size());
   ctx->counters.reserve(16);
@@ -974,7 +1423,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
                            hipMemcpyHostToDevice, s));
   GSP_HIP(hipMemcpyAsync(dhstart.p, hstart.data(), hstart.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, s));
-  GSP_HIP(hipMemcpyAsync(dhitems.p, hitems.data(), hitems.size() * sizeof(int32_t),
+  GSP_HIP(hipMemcpyAsync(dhitems.p, hitems.data(), hitems.size() * sizeof(DataItem),
                          hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->counters.p, 0, (int64_t)16);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, pos.p, -1, N);
@@ -1024,6 +1473,8 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   A.data = ddata.p;
   A.hstart = dhstart.p;
   A.hitems = dhitems.p;
+  A.hexp = hexp;
+  A.hB = (int32_t)hB;
   A.nfound = plan->nfound.p;
   A.nbr = plan->nbr.p;
   A.lam = plan->lam.p;
@@ -1034,16 +1485,52 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   A.indeg = indeg.p;
 
   {
-  TimedLaunch tl(ctx, GSP_K_PLAN);
-  if (nmax <= 8) launch_plan<8>(A, p->precision, N, s);
-  else if (nmax <= 16) launch_plan<16>(A, p->precision, N, s);
-  else if (nmax <= 24) launch_plan<24>(A, p->precision, N, s);
-  else if (nmax <= 32) launch_plan<32>(A, p->precision, N, s);
-  else if (nmax <= 48) launch_plan<48>(A, p->precision, N, s);
-  else launch_plan<64>(A, p->precision, N, s);
-  GSP_HIP(hipGetLastError());
+    TimedLaunch tl(ctx, GSP_K_SEARCH);
+    hipLaunchKernelGGL(k_search, dim3(grid_for(N, kWaves, 16384)), dim3(256), 0, s, A);
+    GSP_HIP(hipGetLastError());
   }
-  tr.mark("search+solve kernel");
+  tr.mark("search kernel");
+  {
+    // bucket the nodes by neighbour count: n <= 16 run four per wavefront
+    DevBuf<int32_t> nhist, list;
+    nhist.alloc(65);
+    list.alloc((size_t)N);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, nhist.p, 0, (int64_t)65);
+    hipLaunchKernelGGL(k_n_hist, dim3(grid_for(N, 256, 1024)), dim3(256), 0, s, plan->nfound.p,
+                       (int32_t)N, nhist.p);
+    int32_t hh[65];
+    GSP_HIP(hipMemcpyAsync(hh, nhist.p, sizeof(hh), hipMemcpyDeviceToHost, s));
+    GSP_HIP(hipStreamSynchronize(s));
+    int32_t cur[65];
+    int64_t acc = 0, small = 0;
+    for (int k = 0; k <= 64; ++k) {
+      cur[k] = (int32_t)acc;
+      acc += hh[k];
+      if (k <= 16) small = acc;
+    }
+    GSP_HIP(hipMemcpyAsync(nhist.p, cur, sizeof(cur), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_n_scatter, dim3((unsigned)((N + kScatterChunk - 1) / kScatterChunk)),
+                       dim3(256), 0, s, plan->nfound.p, (int32_t)N, nhist.p, list.p);
+    tr.mark("bucket by size");
+    TimedLaunch tl(ctx, GSP_K_PLAN);
+    if (small > 0) {
+      const int blocks = grid_for((small + 3) / 4, kWaves, 16384);
+      if (p->precision == GSP_FP32_REFINE)
+        hipLaunchKernelGGL(k_krige16<GSP_FP32_REFINE>, dim3(blocks), dim3(256), 0, s, A, list.p,
+                           (int32_t)small);
+      else
+        hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(256), 0, s, A, list.p,
+                           (int32_t)small);
+    }
+    const int32_t* big = list.p + small;
+    const int64_t nbig = N - small;
+    if (nmax <= 24) launch_krige<24>(A, p->precision, big, nbig, s);
+    else if (nmax <= 32) launch_krige<32>(A, p->precision, big, nbig, s);
+    else if (nmax <= 48) launch_krige<48>(A, p->precision, big, nbig, s);
+    else launch_krige<64>(A, p->precision, big, nbig, s);
+    GSP_HIP(hipGetLastError());
+  }
+  tr.mark("kriging kernel");
   {
     TimedLaunch tl(ctx, GSP_K_LEVELS);
     DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
@@ -1111,11 +1598,27 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
   plan->n_fallback = ctr[CTR_FALLBACK];
 }
 
+// Realizations per sweep pass (GSP_SWEEP_CHUNK overrides). Smaller passes keep
+// the node-major state in the Infinity Cache, but each pass pays one launch
+// per level; on the 1M-node 3D workload 64 realizations per pass measured
+// fastest (2.6 ms vs 3.1 / 3.9 / 7.4 ms for 32 / 16 / 8), so the cap is off.
+constexpr int64_t kSweepCacheBytes = (int64_t)1 << 40;
+
+int sweep_chunk(int64_t rows, int32_t R) {
+  if (const char* env = std::getenv("GSP_SWEEP_CHUNK")) {
+    const int v = std::atoi(env);
+    if (v >= 1 && v <= 64) return v;
+  }
+  int rc = 64;
+  while (rc > 8 && rows * rc * 8 > kSweepCacheBytes) rc >>= 1;
+  return rc;
+}
+
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out) {
   hipStream_t s = ctx->stream;
   const int64_t N = plan->N, nD = plan->n_data;
-  const int RCmax = 64;
+  const int RCmax = sweep_chunk(N + nD, R);
   const int RC0 = R < RCmax ? R : RCmax;
   ctx->X.reserve((size_t)((N + nD) * RC0));
   ctx->Zt.reserve((size_t)(N * RC0));

@@ -79,7 +79,7 @@ class GspSeqsimInfo(ctypes.Structure):
                 ("n_fallback", ctypes.c_int64)]
 
 
-K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_COUNT = 0, 1, 2, 3, 4, 5, 8
+K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 8
 
 
 class GspTimings(ctypes.Structure):
@@ -116,6 +116,7 @@ SIGNATURES = {
 # test hooks (include/gsp_testing.h)
 TESTING_SIGNATURES = {
     "gsp_selftest_mfma": (_I32, [_VP, _I32, _VP]),
+    "gsp_selftest_copy": (_I32, [_VP, _VP, _VP, _I64]),
 }
 
 _lib = None

@@ -141,12 +141,13 @@ typedef struct {
 /* Per-kernel device time, measured with HIP events recorded on the context
  * stream around each launch while timing is enabled. */
 enum {
-  GSP_K_PLAN = 0,        /* neighbour search + kriging solves */
+  GSP_K_PLAN = 0,        /* kriging systems: assembly + Cholesky solves */
   GSP_K_LEVELS = 1,      /* dependency depth */
   GSP_K_SWEEP = 2,       /* dataflow draw sweep */
   GSP_K_TRANSPOSE = 3,   /* node-major -> realization-major output */
   GSP_K_LU_FACTOR = 4,   /* LUSIM assembly + factorisations (whole call) */
   GSP_K_LU_SAMPLE = 5,   /* LUSIM d2 + L22 W and scatter (whole call) */
+  GSP_K_SEARCH = 6,      /* neighbour search */
   GSP_K_COUNT = 8
 };
 typedef struct {

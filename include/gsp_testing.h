@@ -19,6 +19,11 @@ extern "C" {
  * GEMM relies on. */
 int32_t gsp_selftest_mfma(gsp_ctx* ctx, int32_t fp32, double* out);
 
+/* Plain device-to-device copy of `bytes` (a multiple of 16) with a streaming
+ * kernel, queued on the context stream: the reference point for calibrating
+ * hardware traffic counters against a kernel of known traffic. */
+int32_t gsp_selftest_copy(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,17 @@ def test_sweep_parity(ctx, i, mode):
         np.testing.assert_array_equal(got, ref)
 
 
+@pytest.mark.parametrize("chunk", [1, 3, 16])
+def test_sweep_realization_chunks(ctx, chunk, monkeypatch):
+    """Realizations swept in passes of `chunk` give the same bits."""
+    monkeypatch.setenv("GSP_SWEEP_CHUNK", str(chunk))
+    case = make_case(3, O.KNN_THEN_MASK)
+    gp = gpu_plan(ctx, case)
+    o = O.seqsim_plan(case)
+    z = np.random.default_rng(31).standard_normal((20, case.n_nodes))
+    np.testing.assert_array_equal(gp.sweep(z, case.data_vals), O.seqsim_sweep(case, o, z))
+
+
 def test_config1_parity(ctx):
     """100x100, spherical range 10, k=12, radius 30, 50 data, R=10 — conditional
     and unconditional; plan bitwise and realizations bitwise against the oracle."""
