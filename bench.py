@@ -77,6 +77,37 @@ def problem(spec, path_ptr, data_ptr, precision, N):
     return p
 
 
+# kernels of each bench name in the committed traffic profiles
+PROFILE_KERNELS = {"search": ["k_search"], "krige": ["k_krige16<1>", "k_krige<24, 1>"],
+                   "sweep": ["k_sweep_level"]}
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` (read + written, counter-calibrated) from
+    profiles/<workload>_traffic.md, which rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    runs of this script produced; None when no profile covers it."""
+    path = os.path.join(ROOT, "profiles", workload.replace("seqsim-", "seqsim_").replace("-", "_")
+                        + "_traffic.md")
+    if not os.path.exists(path):
+        return None
+    total, found = 0.0, False
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] in PROFILE_KERNELS.get(kernel, []):
+            try:
+                dispatches = int(cells[1])
+                per = (float(cells[2]) + float(cells[3])) * 1e6
+            except ValueError:
+                continue
+            # the profiled run made two plans and two sweeps (export + timed step)
+            total += per * dispatches / 2
+            found = True
+    if not found:
+        return None
+    launches = {"search": 1, "krige": 1, "sweep": 1}[kernel]
+    return int(total / launches)
+
+
 def cpu_baseline(spec, path, data, vals, precision):
     """The oracle's reference-cost restatement (KD-free cell-hash search +
     Cholesky per node per realization), OpenMP over realizations."""
@@ -197,7 +228,8 @@ def main():
     d = kern[dominant]
     achieved = d["achieved_GBs"]
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(args.workload, dominant)}
 
     cpu = None
     if not args.no_cpu:
