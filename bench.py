@@ -11,6 +11,14 @@ refinement step, realizations in FP64. One step is one full
 kriging solves) and the 64-realization sweep, with the path, the data and the
 64 x N standard normals already resident in HBM.
 
+Workload "seqsim-batch-512": 512 independent 2D requests in one plan and one
+sweep: square grids with side log-uniform in [32, 512] (about 23M nodes in
+total), spherical/exponential/gaussian variograms (sill 1) with range uniform
+in [5, 60] cells and nugget uniform in [0, 0.1], 0-500 uniform data each,
+maxneighbors from {8, 16, 32}, :range neighbourhood, random paths, 8
+realizations each, FP32 solves with refinement. One step is one batched plan
+plus the batched sweep.
+
 Prints one JSON line. `cpu_baseline` times the CPU restatement of the
 reference's cost structure (search + solve per node per realization) on a
 bounded sample of the same workload.
@@ -34,7 +42,30 @@ WORKLOADS = {
                                  nugget=0.05, n_data=2000, kmax=24, radius=40.0, R=64, seed=1003),
     "seqsim-2d-100x100": dict(dims=(100, 100), family=0, sill=1.0, ranges=(10.0, 10.0), nugget=0.0,
                               n_data=50, kmax=12, radius=30.0, R=10, seed=1001),
+    "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
 }
+
+
+def batch_specs(spec):
+    """The requests of a batch workload (same draws as the tests' mixed-batch
+    generator): one spec dict per request."""
+    rng = np.random.default_rng(spec["seed"])
+    lo, hi = spec["side"]
+    out = []
+    for _ in range(spec["batch"]):
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        fam = int(rng.integers(0, 3))
+        rang = float(rng.uniform(5.0, 60.0))
+        nug = float(rng.uniform(0.0, 0.1))
+        nd = int(rng.integers(0, 501))
+        kmax = int(rng.choice([8, 16, 32]))
+        data = rng.uniform(0, side, (nd, 2))
+        vals = rng.standard_normal(nd)
+        path = rng.permutation(side * side).astype(np.int64)
+        out.append(dict(dims=(side, side), family=fam, sill=1.0, ranges=(rang, rang), nugget=nug,
+                        n_data=nd, kmax=kmax, radius=rang, data=np.ascontiguousarray(data),
+                        vals=vals, path=path))
+    return out
 
 
 def make_inputs(spec):
@@ -108,30 +139,47 @@ def profiled_traffic(workload, kernel):
     return int(total / launches)
 
 
-def cpu_baseline(spec, path, data, vals, precision):
+def cpu_baseline(reqs, precision, R_batch=None, node_budget=None):
     """The oracle's reference-cost restatement (KD-free cell-hash search +
-    Cholesky per node per realization), OpenMP over realizations."""
+    Cholesky per node per realization), OpenMP over realizations.
+
+    Single workloads: one full realization per thread. Batches: the first
+    requests up to `node_budget` nodes, with the workload's R realizations
+    each (every path position costs differently, so samples are whole
+    realizations)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
-    dims = spec["dims"]
-    case = O.SeqsimCase(dims=dims, family=spec["family"], sill=spec["sill"],
-                        ranges=spec["ranges"][:len(dims)], nugget=spec["nugget"], kmax=spec["kmax"],
-                        radius=spec["radius"], data_coords=data, data_vals=vals, path=path,
-                        precision=precision)
-    N = case.n_nodes
-    R = threads
-    z = np.random.default_rng(7).standard_normal((R, N))
-    # every path position costs differently (early nodes have few simulated
-    # neighbours), so the sample is whole realizations: one per thread
-    limit = N
-    t0 = time.perf_counter()
-    _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=limit)
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "nodes*realizations/s", "cores": threads, "kind": "port",
-            "sample": f"{R} full realizations ({limit} path nodes each) of the same workload "
-                      f"({dt:.1f} s; search + kriging solve per node per realization, "
-                      f"OpenMP over realizations, {threads} threads)"}
+    if R_batch is not None:
+        threads = min(threads, R_batch)
+    n_total, dt_total, used, nodes = 0, 0.0, 0, 0
+    for spec in reqs:
+        dims = spec["dims"]
+        case = O.SeqsimCase(dims=dims, family=spec["family"], sill=spec["sill"],
+                            ranges=spec["ranges"][:len(dims)], nugget=spec["nugget"],
+                            kmax=spec["kmax"], radius=spec["radius"], data_coords=spec["data"],
+                            data_vals=spec["vals"], path=spec["path"], precision=precision)
+        N = case.n_nodes
+        R = R_batch if R_batch is not None else threads
+        z = np.random.default_rng(7).standard_normal((R, N))
+        t0 = time.perf_counter()
+        _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=N)
+        dt_total += time.perf_counter() - t0
+        n_total += n
+        used += 1
+        nodes += N
+        if node_budget is None or nodes >= node_budget:
+            break
+    if R_batch is None:
+        sample = (f"{threads} full realizations ({nodes} path nodes each) of the same workload "
+                  f"({dt_total:.1f} s; search + kriging solve per node per realization, "
+                  f"OpenMP over realizations, {threads} threads)")
+    else:
+        sample = (f"the first {used} requests of the batch ({nodes} nodes), {R_batch} full "
+                  f"realizations each ({dt_total:.1f} s; search + kriging solve per node per "
+                  f"realization, OpenMP over realizations, {threads} threads)")
+    return {"value": n_total / dt_total, "unit": "nodes*realizations/s", "cores": threads,
+            "kind": "port", "sample": sample}
 
 
 def main():
@@ -163,22 +211,39 @@ def main():
 
     spec = WORKLOADS[args.workload]
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
-    N, path, data, vals = make_inputs(spec)
     R = spec["R"]
+    if "batch" in spec:
+        reqs = batch_specs(spec)
+    else:
+        _, path, data, vals = make_inputs(spec)
+        reqs = [dict(spec, data=data, vals=vals, path=path)]
+    sizes = [int(np.prod(r["dims"])) for r in reqs]
+    N = int(sum(sizes))
     ctx = _lib.default_context(0)
-    # inputs resident in HBM before timing: path, data, R x N standard normals
-    z_host = np.random.default_rng(spec["seed"] + 3).standard_normal((R, N))
-    dpath = _lib.DeviceBuffer.from_array(ctx, path)
-    ddata = _lib.DeviceBuffer.from_array(ctx, data)
-    dvals = _lib.DeviceBuffer.from_array(ctx, vals)
-    z = _lib.DeviceBuffer.from_array(ctx, z_host)
-    del z_host
+    # inputs resident in HBM before timing: paths, data, R x N standard normals
+    # (one buffer each; request q's arrays at its offsets)
+    path_all = np.concatenate([r["path"] for r in reqs])
+    coords_all = np.concatenate([r["data"].ravel() for r in reqs] + [np.zeros(1)])
+    vals_all = np.concatenate([r["vals"] for r in reqs] + [np.zeros(1)])
+    dpath = _lib.DeviceBuffer.from_array(ctx, path_all)
+    ddata = _lib.DeviceBuffer.from_array(ctx, coords_all)
+    dvals = _lib.DeviceBuffer.from_array(ctx, vals_all)
+    z = _lib.DeviceBuffer(ctx, (R, N))
+    for r0 in range(R):  # row by row keeps host memory small
+        z_row = np.random.default_rng(spec["seed"] + 3 + r0).standard_normal(N)
+        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, z.ptr + r0 * N * 8,
+                                                    z_row.ctypes.data, N * 8))
     out = _lib.DeviceBuffer(ctx, (R, N))
-
-    prob = problem(spec, dpath.ptr, ddata.ptr, precision, N)
+    probs, noff, coff = [], 0, 0
+    for r, n in zip(reqs, sizes):
+        probs.append(problem(r, dpath.ptr + 8 * noff, ddata.ptr + 8 * coff if r["n_data"] else None,
+                             precision, n))
+        noff += n
+        coff += r["data"].size
+    plan_arg = probs[0] if len(probs) == 1 else probs
 
     def step():
-        plan = _lib.SeqsimPlan(ctx, prob)
+        plan = _lib.SeqsimPlan(ctx, plan_arg)
         plan.sweep(z, dvals, out, device_pointers=True)
         ctx.synchronize()
         return plan
@@ -187,7 +252,7 @@ def main():
         step().close()
     # realization-invariant statistics of the plan (for the byte counts)
     plan = step()
-    ex = plan.export()
+    ex = plan.export(fields=("nfound", "flags"))
     info = plan.info
     plan.close()
     keff = np.where(ex["flags"] & 3, 0, ex["nfound"]).astype(np.float64)
@@ -208,8 +273,9 @@ def main():
 
     # algorithmic bytes (realizations resident; neighbour gathers are re-reads)
     sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
-    search_bytes = N * (8 + spec["kmax"] * 4 + 4)                # path, nbr, nfound
-    plan_bytes = N * (spec["kmax"] * (4 + 8) + 4 + 8 + 1 + 4)    # nbr + nfound in, lambda, sigma, flags, degrees
+    K = info.maxneighbors                                        # row stride of nbr / lambda
+    search_bytes = N * (8 + K * 4 + 4)                           # path, nbr, nfound
+    plan_bytes = N * (K * (4 + 8) + 4 + 8 + 1 + 4)               # nbr + nfound in, lambda, sigma, flags, degrees
     kern = {}
     for slot, name, nbytes in ((_lib.K_SEARCH, "search", search_bytes), (_lib.K_PLAN, "krige", plan_bytes),
                                (_lib.K_SWEEP, "sweep", sweep_bytes),
@@ -233,7 +299,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(spec, path, data, vals, precision)
+        if "batch" in spec:
+            cpu = cpu_baseline(reqs, precision, R_batch=R, node_budget=1_500_000)
+        else:
+            cpu = cpu_baseline(reqs, precision)
 
     line = {
         "metric": "SEQSIM nodes*realizations/s (exact kriging)",
@@ -248,10 +317,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (solves fp32+fp64 refinement)" if precision == _lib.FP32_REFINE else "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
-                   "maxneighbors": spec["kmax"], "radius": spec["radius"], "n_data": spec["n_data"],
-                   "search": "k nearest then mask", "precision": args.precision,
-                   "depth": info.depth, "mean_neighbours": round(kbar, 2)},
+        "config": ({"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
+                    "maxneighbors": spec["kmax"], "radius": spec["radius"], "n_data": spec["n_data"]}
+                   if "batch" not in spec else
+                   {"workload": args.workload, "requests": len(reqs), "nodes": N,
+                    "realizations_per_request": R, "side_range": list(spec["side"]),
+                    "n_data": int(sum(r["n_data"] for r in reqs)), "maxneighbors": [8, 16, 32]})
+        | {"search": "k nearest then mask", "precision": args.precision,
+           "depth": info.depth, "mean_neighbours": round(kbar, 2)},
         "roofline": roofline,
         "kernels": kern,
         "cpu_baseline": cpu,

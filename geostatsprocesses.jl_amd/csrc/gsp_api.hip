@@ -229,6 +229,11 @@ int32_t gsp_selftest_mfma(gsp_ctx* ctx, int32_t fp32, double* out) {
 // ------------------------------------------------------------------ SEQSIM --
 int32_t gsp_seqsim_plan_create(gsp_ctx* ctx, const gsp_seqsim_problem* problem,
                                gsp_seqsim_plan** out) {
+  return gsp_seqsim_plan_create_batch(ctx, 1, problem, out);
+}
+
+int32_t gsp_seqsim_plan_create_batch(gsp_ctx* ctx, int32_t n, const gsp_seqsim_problem* problems,
+                                     gsp_seqsim_plan** out) {
   if (!out) {
     gsp::set_error("null output pointer");
     return GSP_ERR_ARGUMENT;
@@ -241,7 +246,7 @@ int32_t gsp_seqsim_plan_create(gsp_ctx* ctx, const gsp_seqsim_problem* problem,
   }
   int32_t rc = gsp_guard(ctx, [&] {
     plan->pool = ctx->pool;
-    gsp::seqsim_plan_build(ctx, problem, plan);
+    gsp::seqsim_plan_build(ctx, n, problems, plan);
   });
   if (rc != GSP_OK) {
     delete plan;
@@ -265,6 +270,20 @@ int32_t gsp_seqsim_plan_info(const gsp_seqsim_plan* plan, gsp_seqsim_info* info)
   info->alpha = plan->alpha;
   info->n_prior = plan->n_prior;
   info->n_fallback = plan->n_fallback;
+  return GSP_OK;
+}
+
+int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,
+                               int64_t* node_offset, int64_t* data_offset) {
+  if (!plan) {
+    gsp::set_error("null plan");
+    return GSP_ERR_ARGUMENT;
+  }
+  if (n_requests) *n_requests = plan->n_req;
+  for (size_t q = 0; q < plan->node_off.size(); ++q) {
+    if (node_offset) node_offset[q] = plan->node_off[q];
+    if (data_offset) data_offset[q] = plan->data_off[q];
+  }
   return GSP_OK;
 }
 

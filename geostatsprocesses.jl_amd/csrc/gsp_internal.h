@@ -336,12 +336,15 @@ struct Offset {
 
 struct gsp_seqsim_plan {
   std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
-  int64_t N = 0, n_data = 0;
-  int32_t ndim = 0, kmin = 0, kmax = 0, depth = 0;
-  double alpha = 0;
+  int64_t N = 0, n_data = 0;  // nodes / data of all requests
+  int32_t n_req = 0;
+  int32_t ndim = 0, kmin = 0, kmax = 0, depth = 0;  // kmax: row stride of nbr / lam
+  double alpha = 0;  // scale factor of the first request
   double mean = 0;
   int64_t n_prior = 0, n_fallback = 0;
-  gsp::DevBuf<int64_t> path;
+  std::vector<int64_t> node_off, data_off;  // n_req + 1 entries each
+  gsp::DevBuf<int32_t> node_req;            // request of every node
+  gsp::DevBuf<double> req_mean;             // process mean of every request
   gsp::DevBuf<int32_t> nfound, nbr, level;
   gsp::DevBuf<double> lam, sigma;
   gsp::DevBuf<uint8_t> flags;
@@ -362,7 +365,8 @@ struct gsp_lusim_plan {
 
 // kernels launched from gsp_seqsim.hip / gsp_lusim.hip
 namespace gsp {
-void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_plan* plan);
+void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
+                       gsp_seqsim_plan* plan);
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out);
 void lusim_factor_build(gsp_ctx* ctx, const gsp_lusim_problem* p, gsp_lusim_plan* plan);

@@ -7,8 +7,9 @@
 // the work into
 //   k_search    one wavefront per node: neighbour search (offset table over the
 //               implicit grid + a cell hash for the data);
-//   k_krige     one wavefront per node: covariance assembly and a
-//               register-resident Cholesky solve of the kriging system;
+//   k_krige*    covariance assembly and a register-resident Cholesky solve of
+//               the kriging system (four systems per wavefront up to 16
+//               neighbours, one above);
 //   k_dep_* / k_frontier_*  dependency level of every node (Kahn's algorithm
 //               on the reverse edges), giving the nodes grouped by level;
 //   k_sweep_level  the only sequential part: one launch per level, every node
@@ -17,18 +18,29 @@
 //               own L2, so per-node acquire/release inside one kernel would
 //               write back and invalidate L2 for every node);
 //   k_z_transpose / k_transpose  realization-major <-> node-major layouts.
+//
+// A plan holds one or many independent requests (grids, variograms, data,
+// paths, neighbourhoods of their own) in one global index space: request p
+// owns node ids [node_off_p, node_off_p + N_p), path positions over the same
+// range and data ids NT + [data_off_p, data_off_p + n_data_p), where NT is
+// the number of nodes of all requests. Every kernel above runs once over the
+// whole batch; the requests' levels run side by side in the sweep.
 #include <algorithm>
 #include <chrono>
-#include <type_traits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <atomic>
+#include <string>
+#include <thread>
+#include <type_traits>
 
 #include <hipcub/hipcub.hpp>
 
 #include "gsp_internal.h"
+#include "gsp_wave.h"
 
 namespace gsp {
 namespace {
@@ -37,63 +49,41 @@ constexpr int kWaves = 4;  // wavefronts per 256-thread block
 
 enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5 };
 
-// ------------------------------------------------------------ wave helpers --
-__device__ inline double rl(double x, int lane) {
-  uint64_t u = __builtin_bit_cast(uint64_t, x);
-  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
-  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-__device__ inline float rlf(float x, int lane) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), lane));
-}
-
-__device__ inline int rli(int x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
-
-// number of set bits of `mask` below this lane
-__device__ inline int mbcnt(uint64_t mask) {
-  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ inline bool key_less(double da, int ia, double db, int ib) {
-  return da < db || (da == db && ia < ib);
-}
-
-// ------------------------------------------------------------ plan kernel --
-// A conditioning datum in a hash-cell list: physical coordinates inline.
+// ------------------------------------------------------------ plan kernels --
+// A conditioning datum in a hash-cell list: physical coordinates inline, `id`
+// its global data index.
 struct DataItem {
   double x[3];
   int32_t id;
   int32_t pad_;
 };
 
-struct PlanArgs {
-  int32_t N, ndim, n_data;
-  int32_t dims[3];
-  int32_t kmin, kmax, mode, precision, T;
-  int32_t hd[3];
-  double r2;
+// One request of a plan.
+struct Req {
+  int32_t ndim, mode, kmin, kmax;
+  int32_t dims[3], T;    // grid size; entries of the offset table
+  int32_t hd[3], hexp;   // data-hash cells per axis; 1: expanded cell lists
+  int32_t hB, n_data;    // hash cell width in grid cells; conditioning data
+  int32_t node_off, data_off;
+  int64_t table_off, cell_off;  // first entry of the offset table / of hstart
+  double r2, alpha, prior_sigma, mean;
   double org[3], sp[3];  // physical grid
   double ao[3], as[3];   // scaled grid (alpha*origin, alpha*spacing)
   double hcs[3];         // data-hash cell size (physical)
-  double alpha, prior_sigma;
   DevModel M;
-  const int64_t* path;
-  const int32_t* pos;
+};
+
+struct PlanArgs {
+  int32_t NT;  // nodes of all requests; data ids start here
+  int32_t K;   // row stride of nbr / lam (largest kmax)
+  const Req* reqs;
+  const int32_t* node_req;  // request of every node
+  const int64_t* path;      // (request << 32) | global node id, in global path order
+  const int32_t* pos;       // global path position of every node
   const Offset* table;
-  const double* data;  // n_data x ndim, physical
-  const int32_t* hstart;
+  const double* data;       // n_data_total x 3, physical
+  const int32_t* hstart;    // absolute item index of every hash cell
   const DataItem* hitems;
-  int32_t hexp;  // 1: each hash cell lists every datum near it (read one list per node)
-  int32_t hB;    // hash cell width in grid cells
   int32_t* nfound;
   int32_t* nbr;
   double* lam;
@@ -105,16 +95,43 @@ struct PlanArgs {
 };
 
 // Squared physical distance between a datum and the centre of node t.
-__device__ inline double data_d2(const PlanArgs& A, const DataItem& it, const int* t) {
+__device__ inline double data_d2(const Req& R, const DataItem& it, const int* t) {
   double acc = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (d >= A.ndim) break;
-    double c = A.org[d] + ((double)t[d] + 0.5) * A.sp[d];
+    if (d >= R.ndim) break;
+    double c = R.org[d] + ((double)t[d] + 0.5) * R.sp[d];
     double u = it.x[d] - c;
     acc = acc + u * u;
   }
   return acc;
+}
+
+// Scaled coordinates of neighbour `nb` (a node of request R or a datum).
+__device__ inline void neighbour_coords(const PlanArgs& A, const Req& R, int nb, double& x0,
+                                        double& x1, double& x2) {
+  if (nb < A.NT) {
+    const int l = nb - R.node_off;
+    const int nx = R.dims[0], ny = R.dims[1];
+    const int c0 = l % nx, c1 = (l / nx) % ny, c2 = l / (nx * ny);
+    x0 = R.ao[0] + ((double)c0 + 0.5) * R.as[0];
+    x1 = R.ao[1] + ((double)c1 + 0.5) * R.as[1];
+    x2 = R.ao[2] + ((double)c2 + 0.5) * R.as[2];
+  } else {
+    const double* p = A.data + (int64_t)(nb - A.NT) * 3;
+    x0 = R.alpha * p[0];
+    x1 = R.ndim > 1 ? R.alpha * p[1] : 0.0;
+    x2 = R.ndim > 2 ? R.alpha * p[2] : 0.0;
+  }
+}
+
+// Scaled coordinates of the centre of local node l of request R.
+__device__ inline void node_coords(const Req& R, int l, double& t0, double& t1, double& t2) {
+  const int nx = R.dims[0], ny = R.dims[1];
+  const int c0 = l % nx, c1 = (l / nx) % ny, c2 = l / (nx * ny);
+  t0 = R.ao[0] + ((double)c0 + 0.5) * R.as[0];
+  t1 = R.ao[1] + ((double)c1 + 0.5) * R.as[1];
+  t2 = R.ao[2] + ((double)c2 + 0.5) * R.as[2];
 }
 
 // Per-wavefront LDS of the kriging solve: neighbour coordinates, the FP64
@@ -126,15 +143,6 @@ struct KrigeLds {
   double A[KB][KB + 1];
   float F[PREC == GSP_FP32_REFINE ? KB : 1][KB + 1];
 };
-
-// (j, m) with j >= m of the e-th entry of a packed lower triangle.
-__device__ inline void tri_unpack(int e, int& j, int& m) {
-  int jj = (int)((__builtin_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
-  while (jj * (jj + 1) / 2 > e) --jj;
-  while ((jj + 1) * (jj + 2) / 2 <= e) ++jj;
-  j = jj;
-  m = e - jj * (jj + 1) / 2;
-}
 
 // Covariance assembly spread over the entries of the lower triangle (lane e
 // computes entries e, e + 64, ...), stored to both triangles; lane j < n also
@@ -295,10 +303,10 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
   return 0;
 }
 
-// Neighbour search, one wavefront per node (in path order): the first kmax
-// usable grid offsets of the sorted offset table, the data of the node's hash
-// list within the k-th grid distance, their merge by (d2, index) and the mask.
-// Writes nbr[i][0..n) and nfound[i].
+// Neighbour search, one wavefront per path position (global order): the first
+// kmax usable grid offsets of the request's sorted offset table, the data of
+// the node's hash list within the k-th grid distance, their merge by
+// (d2, index) and the mask. Writes nbr[i][0..K) and nfound[i].
 __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_gidx[kWaves][64];
   __shared__ double s_gd2[kWaves][64];
@@ -307,15 +315,25 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_fidx[kWaves][64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int nx = A.dims[0], ny = A.dims[1], nz = A.dims[2];
   const int64_t wstride = (int64_t)gridDim.x * kWaves;
 
-  // the first 64 offsets serve every node of this wave
+  int cur = -1;
   Offset e0 = {0.0, 0, 0, 0, 0};
-  if (lane < A.T) e0 = A.table[lane];
-  for (int64_t q = (int64_t)blockIdx.x * kWaves + w; q < A.N; q += wstride) {
-    const int i = (int)A.path[q];
+  Req R;  // the current request (a register copy: the stores below cannot alias it)
+  for (int64_t q = (int64_t)blockIdx.x * kWaves + w; q < A.NT; q += wstride) {
+    const int64_t pv = A.path[q];
+    const int gi = __builtin_amdgcn_readfirstlane((int)(pv & 0xFFFFFFFF));
+    const int p = __builtin_amdgcn_readfirstlane((int)(pv >> 32));
+    if (p != cur) {  // the first 64 offsets serve every node of the request
+      cur = p;
+      R = A.reqs[p];
+      e0 = Offset{0.0, 0, 0, 0, 0};
+      if (lane < R.T) e0 = A.table[R.table_off + lane];
+    }
     const int pi = (int)q;
+    const int nx = R.dims[0], ny = R.dims[1], nz = R.dims[2];
+    const int kmax = R.kmax;
+    const int i = gi - R.node_off;
     int ti[3];
     ti[0] = i % nx;
     ti[1] = (i / nx) % ny;
@@ -323,36 +341,36 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
 
     // ---- grid candidates: the first kmax usable offsets in (d2, index) order
     int cnt = 0;
-    for (int t0 = 0; cnt < A.kmax && t0 < A.T; t0 += 64) {
+    for (int t0 = 0; cnt < kmax && t0 < R.T; t0 += 64) {
       const int t = t0 + lane;
       bool ok = false;
       int j = 0;
       double d2 = 0.0;
-      if (t < A.T) {
-        const Offset e = t0 == 0 ? e0 : A.table[t];
+      if (t < R.T) {
+        const Offset e = t0 == 0 ? e0 : A.table[R.table_off + t];
         const int x = ti[0] + e.dx, y = ti[1] + e.dy, z = ti[2] + e.dz;
         ok = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
         if (ok) {
-          j = x + nx * (y + ny * z);
+          j = R.node_off + x + nx * (y + ny * z);
           d2 = e.d2;
-          if (A.mode == GSP_MASKED_KNN && A.pos[j] >= pi) ok = false;
+          if (R.mode == GSP_MASKED_KNN && A.pos[j] >= pi) ok = false;
         }
       }
       const uint64_t b = __ballot(ok);
       const int rank = cnt + mbcnt(b);
-      if (ok && rank < A.kmax) {
+      if (ok && rank < kmax) {
         s_gidx[w][rank] = j;
         s_gd2[w][rank] = d2;
       }
       cnt += __popcll(b);
     }
-    const int ng = cnt < A.kmax ? cnt : A.kmax;
+    const int ng = cnt < kmax ? cnt : kmax;
     wave_sync();
-    const double Dk = ng == A.kmax ? s_gd2[w][ng - 1] : A.r2;
+    const double Dk = ng == kmax ? s_gd2[w][ng - 1] : R.r2;
 
     // ---- data candidates with d2 <= Dk, from the cell hash
     int nd = 0;
-    if (A.n_data > 0) {
+    if (R.n_data > 0) {
       // scan one hash-cell list, appending data with d2 <= Dk to the LDS list
       auto scan_list = [&](int s0, int e0l) {
         for (int b0 = s0; b0 < e0l; b0 += 64) {
@@ -362,8 +380,8 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
           if (b0 + lane < e0l) {
             const DataItem it = A.hitems[b0 + lane];
             di = it.id;
-            d2 = data_d2(A, it, ti);
-            ok = d2 <= Dk && d2 <= A.r2;
+            d2 = data_d2(R, it, ti);
+            ok = d2 <= Dk && d2 <= R.r2;
           }
           const uint64_t b = __ballot(ok);
           const int add = __popcll(b);
@@ -389,40 +407,41 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
             wave_sync();
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-              if (keep_rank[h] < A.kmax) {
+              if (keep_rank[h] < kmax) {
                 s_dd2[w][keep_rank[h]] = kd[h];
                 s_didx[w][keep_rank[h]] = kid[h];
               }
-            nd = nd < A.kmax ? nd : A.kmax;
+            nd = nd < kmax ? nd : kmax;
             wave_sync();
           }
           if (ok) {
             const int r = nd + mbcnt(b);
-            s_didx[w][r] = A.N + di;
+            s_didx[w][r] = A.NT + di;
             s_dd2[w][r] = d2;
           }
           nd += add;
         }
       };
-      if (A.hexp) {
-        const int cell = ti[0] / A.hB + A.hd[0] * (ti[1] / A.hB + A.hd[1] * (ti[2] / A.hB));
-        scan_list(A.hstart[cell], A.hstart[cell + 1]);
+      const int32_t* hs = A.hstart + R.cell_off;
+      if (R.hexp) {
+        const int cell = ti[0] / R.hB + R.hd[0] * (ti[1] / R.hB + R.hd[1] * (ti[2] / R.hB));
+        scan_list(hs[cell], hs[cell + 1]);
       } else {
         int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         const bool all = !(Dk < __builtin_inf());
         const double rho = all ? 0.0 : __builtin_sqrt(Dk) * (1.0 + 0x1p-20) + 0x1p-40;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          if (d >= A.ndim) break;
+          if (d >= R.ndim) break;
           if (all) {
             lo[d] = 0;
-            hi[d] = A.hd[d] - 1;
+            hi[d] = R.hd[d] - 1;
           } else {
-            const double c = A.org[d] + ((double)ti[d] + 0.5) * A.sp[d];
-            double fl = __builtin_floor((c - rho - A.org[d]) / A.hcs[d]);
-            double fh = __builtin_floor((c + rho - A.org[d]) / A.hcs[d]);
-            fl = fl < 0.0 ? 0.0 : (fl > A.hd[d] - 1 ? A.hd[d] - 1 : fl);
-            fh = fh < 0.0 ? 0.0 : (fh > A.hd[d] - 1 ? A.hd[d] - 1 : fh);
+            const double c = R.org[d] + ((double)ti[d] + 0.5) * R.sp[d];
+            double fl = __builtin_floor((c - rho - R.org[d]) / R.hcs[d]);
+            double fh = __builtin_floor((c + rho - R.org[d]) / R.hcs[d]);
+            fl = fl < 0.0 ? 0.0 : (fl > R.hd[d] - 1 ? R.hd[d] - 1 : fl);
+            fh = fh < 0.0 ? 0.0 : (fh > R.hd[d] - 1 ? R.hd[d] - 1 : fh);
             lo[d] = (int)fl;
             hi[d] = (int)fh;
           }
@@ -430,15 +449,15 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
         for (int cz = lo[2]; cz <= hi[2]; ++cz)
           for (int cy = lo[1]; cy <= hi[1]; ++cy)
             for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-              const int cell = cx + A.hd[0] * (cy + A.hd[1] * cz);
-              scan_list(A.hstart[cell], A.hstart[cell + 1]);
+              const int cell = cx + R.hd[0] * (cy + R.hd[1] * cz);
+              scan_list(hs[cell], hs[cell + 1]);
             }
       }
       wave_sync();
     }
 
     // ---- merge: top-kmax of the grid list (sorted) and the data list
-    const int nf = ng + nd < A.kmax ? ng + nd : A.kmax;
+    const int nf = ng + nd < kmax ? ng + nd : kmax;
     int my_rank[3] = {128, 128, 128};
     int my_id[3] = {0, 0, 0};
     if (lane < ng) {
@@ -465,13 +484,12 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     wave_sync();
 #pragma unroll
     for (int h = 0; h < 3; ++h)
-      if (my_rank[h] < A.kmax) s_fidx[w][my_rank[h]] = my_id[h];
+      if (my_rank[h] < kmax) s_fidx[w][my_rank[h]] = my_id[h];
     wave_sync();
 
     // ---- mask: drop points not yet simulated (k nearest first, then mask)
     const int fi = lane < nf ? s_fidx[w][lane] : -1;
-    const bool keep =
-        lane < nf && (A.mode == GSP_MASKED_KNN || fi >= A.N || A.pos[fi] < pi);
+    const bool keep = lane < nf && (R.mode == GSP_MASKED_KNN || fi >= A.NT || A.pos[fi] < pi);
     const uint64_t kb = __ballot(keep);
     const int n = __popcll(kb);
     const int kr = mbcnt(kb);
@@ -480,16 +498,17 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     wave_sync();
     const int nb = lane < n ? s_fidx[w][lane] : -1;
 
-    if (lane < A.kmax) A.nbr[(int64_t)i * A.kmax + lane] = nb;
-    if (lane == 0) A.nfound[i] = n;
+    if (lane < A.K) A.nbr[(int64_t)gi * A.K + lane] = nb;
+    if (lane == 0) A.nfound[gi] = n;
   }
 }
 
 // Wavefronts per block of the kriging kernel (LDS per wave grows with KB^2).
 constexpr int plan_waves(int KB) { return KB <= 32 ? 4 : (KB <= 48 ? 2 : 1); }
 
-// Kriging systems, one wavefront per node (domain order): covariance assembly,
-// Cholesky solve, weights, sigma and flags, plus the dependency degrees.
+// Kriging systems with more than 16 neighbours, one wavefront per node:
+// covariance assembly, Cholesky solve, weights, sigma and flags, plus the
+// dependency degrees.
 template <int KB, int PREC>
 __global__ __launch_bounds__(64 * plan_waves(KB)) void k_krige(const PlanArgs A, const int32_t* list,
                                                                int32_t count) {
@@ -497,61 +516,41 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_krige(const PlanArgs A,
   __shared__ KrigeLds<KB, PREC> s_krige[kW];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  const int nx = A.dims[0], ny = A.dims[1];
   const int64_t wstride = (int64_t)gridDim.x * kW;
   for (int64_t idx = (int64_t)blockIdx.x * kW + w; idx < count; idx += wstride) {
-    const int64_t i = list[idx];
-    const int n = A.nfound[i];
-    const int nb = lane < n ? A.nbr[i * A.kmax + lane] : -1;
-    int ti[3];
-    ti[0] = (int)(i % nx);
-    ti[1] = (int)((i / nx) % ny);
-    ti[2] = (int)(i / ((int64_t)nx * ny));
-    // ---- kriging
+    const int gi = __builtin_amdgcn_readfirstlane(list[idx]);
+    const Req& R = A.reqs[__builtin_amdgcn_readfirstlane(A.node_req[gi])];
+    const int n = A.nfound[gi];
+    const int nb = lane < n ? A.nbr[(int64_t)gi * A.K + lane] : -1;
     int flag = 0;
-    double lam = 0.0, sig = A.prior_sigma;
-    if (n < A.kmin) {
+    double lam = 0.0, sig = R.prior_sigma;
+    if (n < R.kmin) {
       flag = GSP_FLAG_FEW;
     } else {
-      double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-      if (lane < n) {
-        if (nb < A.N) {
-          const int c0 = nb % nx, c1 = (nb / nx) % ny, c2 = nb / (nx * ny);
-          x0 = A.ao[0] + ((double)c0 + 0.5) * A.as[0];
-          x1 = A.ao[1] + ((double)c1 + 0.5) * A.as[1];
-          x2 = A.ao[2] + ((double)c2 + 0.5) * A.as[2];
-        } else {
-          const double* p = A.data + (int64_t)(nb - A.N) * A.ndim;
-          x0 = A.alpha * p[0];
-          if (A.ndim > 1) x1 = A.alpha * p[1];
-          if (A.ndim > 2) x2 = A.alpha * p[2];
-        }
-      }
-      const double t0 = A.ao[0] + ((double)ti[0] + 0.5) * A.as[0];
-      const double t1 = A.ao[1] + ((double)ti[1] + 0.5) * A.as[1];
-      const double t2 = A.ao[2] + ((double)ti[2] + 0.5) * A.as[2];
+      double x0 = 0.0, x1 = 0.0, x2 = 0.0, t0, t1, t2;
+      if (lane < n) neighbour_coords(A, R, nb, x0, x1, x2);
+      node_coords(R, gi - R.node_off, t0, t1, t2);
       KrigeLds<KB, PREC>& KL = s_krige[w];
-      const double c = assemble<KB, PREC>(KL, A.M, n, lane, x0, x1, x2, t0, t1, t2);
+      const double c = assemble<KB, PREC>(KL, R.M, n, lane, x0, x1, x2, t0, t1, t2);
       if constexpr (PREC == GSP_FP32_REFINE)
-        flag = krige_fp32r<KB>(KL, n, lane, c, A.M.sill, (float)(A.M.sill * 0x1p-12), lam, sig);
+        flag = krige_fp32r<KB>(KL, n, lane, c, R.M.sill, (float)(R.M.sill * 0x1p-12), lam, sig);
       else
-        flag = krige_fp64<KB, PREC>(KL, n, lane, c, A.M.sill, lam, sig);
+        flag = krige_fp64<KB, PREC>(KL, n, lane, c, R.M.sill, lam, sig);
       wave_sync();
       if (flag & GSP_FLAG_FAIL) {
         lam = 0.0;
-        sig = A.prior_sigma;
+        sig = R.prior_sigma;
       }
     }
-    if (lane < A.kmax) A.lam[i * A.kmax + lane] = lane < n ? lam : 0.0;
+    if (lane < A.K) A.lam[(int64_t)gi * A.K + lane] = lane < n ? lam : 0.0;
     // dependency degrees for the level computation (grid neighbours only)
-    const bool dep = !(flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) && lane < n && nb < A.N;
+    const bool dep = !(flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) && lane < n && nb < A.NT;
     if (dep) atomicAdd(&A.outdeg[nb], 1);
     const int ndep = __popcll(__ballot(dep));
     if (lane == 0) {
-      A.sigma[i] = sig;
-      A.flags[i] = (uint8_t)flag;
-      A.nfound[i] = n;
-      A.indeg[i] = ndep;
+      A.sigma[gi] = sig;
+      A.flags[gi] = (uint8_t)flag;
+      A.indeg[gi] = ndep;
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
@@ -562,46 +561,18 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_krige(const PlanArgs A,
 // Lane 16 g + r holds row r of system g. Values are exchanged inside a 16-lane
 // row with DPP row broadcasts. The systems of one wavefront are padded to the
 // largest size among them with sill * identity rows, which leaves the
-// factorisation and both substitutions of every real row bit-identical.
-template <int M>
-__device__ inline double rbd(double x) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, x);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + M, 0xF,
-                                                            0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32),
-                                                            0x150 + M, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
-template <int M>
-__device__ inline float rbf(float x) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x150 + M, 0xF, 0xF, false));
-}
-
-template <int I, int N, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (I < N) {
-    f(std::integral_constant<int, I>{});
-    static_for<I + 1, N>(f);
-  }
-}
-
-// visits N-1, N-2, ..., 0
-template <int N, class F>
-__device__ __forceinline__ void static_for_down(F&& f) {
-  if constexpr (N > 0) {
-    f(std::integral_constant<int, N - 1>{});
-    static_for_down<N - 1>(f);
-  }
-}
-
+// factorisation and both substitutions of every real row bit-identical. The
+// four systems may come from different requests; each group then reads its
+// variogram from LDS.
 template <int PREC>
 struct Krige16Lds {
   double x[4][16][3];
   double A[4][16][17];
   float F[PREC == GSP_FP32_REFINE ? 4 : 1][16][17];
+  DevModel M[4];
 };
+
+static_assert(sizeof(DevModel) == 16 * sizeof(double), "DevModel is copied as 16 words");
 
 // FP64 Cholesky + substitutions of the packed systems (rows in a[]). Returns
 // true in the lanes of a group whose factorisation met a non-positive pivot.
@@ -668,62 +639,54 @@ __global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
-  const int nx = A.dims[0], ny = A.dims[1];
-  const double C0 = A.M.sill;
   Krige16Lds<PREC>& L = s_lds[w];
   const int64_t stride = (int64_t)gridDim.x * kWaves * 4;
   for (int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 4; base < count; base += stride) {
     const int64_t slot = base + g;
     const bool act = slot < count;
-    const int64_t i = act ? list[slot] : 0;
-    const int n = act ? A.nfound[i] : 0;
-    const int nb = (act && r < n) ? A.nbr[i * A.kmax + r] : -1;
-    const bool solve = act && n >= A.kmin;
+    const int gi = act ? list[slot] : list[base];
+    const int p = A.node_req[gi];
+    const Req& R = A.reqs[p];
+    const int p0 = rli(p, 0);
+    const bool uniform = __ballot(p != p0) == 0;  // all four systems of one request
+    const double C0 = R.M.sill;
+    const int n = act ? A.nfound[gi] : 0;
+    const int nb = (act && r < n) ? A.nbr[(int64_t)gi * A.K + r] : -1;
+    const bool solve = act && n >= R.kmin;
     const int ns = solve ? n : 0;
     int nmax = rli(ns, 0);
     nmax = rli(ns, 16) > nmax ? rli(ns, 16) : nmax;
     nmax = rli(ns, 32) > nmax ? rli(ns, 32) : nmax;
     nmax = rli(ns, 48) > nmax ? rli(ns, 48) : nmax;
-    double lam = 0.0, sig = A.prior_sigma;
+    double lam = 0.0, sig = R.prior_sigma;
     int flag = act && !solve ? GSP_FLAG_FEW : 0;
     if (nmax > 0) {
       double x0 = 0.0, x1 = 0.0, x2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
       if (r < ns) {
-        if (nb < A.N) {
-          const int c0 = nb % nx, c1 = (nb / nx) % ny, c2 = nb / (nx * ny);
-          x0 = A.ao[0] + ((double)c0 + 0.5) * A.as[0];
-          x1 = A.ao[1] + ((double)c1 + 0.5) * A.as[1];
-          x2 = A.ao[2] + ((double)c2 + 0.5) * A.as[2];
-        } else {
-          const double* p = A.data + (int64_t)(nb - A.N) * A.ndim;
-          x0 = A.alpha * p[0];
-          if (A.ndim > 1) x1 = A.alpha * p[1];
-          if (A.ndim > 2) x2 = A.alpha * p[2];
-        }
+        neighbour_coords(A, R, nb, x0, x1, x2);
         L.x[g][r][0] = x0;
         L.x[g][r][1] = x1;
         L.x[g][r][2] = x2;
       }
-      if (solve) {
-        const int i0 = (int)(i % nx), i1 = (int)((i / nx) % ny), i2 = (int)(i / ((int64_t)nx * ny));
-        t0 = A.ao[0] + ((double)i0 + 0.5) * A.as[0];
-        t1 = A.ao[1] + ((double)i1 + 0.5) * A.as[1];
-        t2 = A.ao[2] + ((double)i2 + 0.5) * A.as[2];
-      }
+      if (solve) node_coords(R, gi - R.node_off, t0, t1, t2);
+      if (!uniform) ((double*)&L.M[g])[r] = ((const double*)&R.M)[r];
       wave_sync();
       // covariance entries of each system's lower triangle over its 16 lanes
       const int T = ns * (ns + 1) / 2, Tmax = nmax * (nmax + 1) / 2;
-      for (int e = r; e < Tmax; e += 16) {
-        if (e < T) {
-          int j, m;
-          tri_unpack(e, j, m);
-          const double v = cov_points(A.M, L.x[g][j][0], L.x[g][j][1], L.x[g][j][2],
-                                      L.x[g][m][0], L.x[g][m][1], L.x[g][m][2]);
-          L.A[g][j][m] = v;
-          L.A[g][m][j] = v;
+      auto assemble16 = [&](const DevModel& M) {
+        for (int e = r; e < Tmax; e += 16) {
+          if (e < T) {
+            int j, m;
+            tri_unpack(e, j, m);
+            const double v = cov_points(M, L.x[g][j][0], L.x[g][j][1], L.x[g][j][2],
+                                        L.x[g][m][0], L.x[g][m][1], L.x[g][m][2]);
+            L.A[g][j][m] = v;
+            L.A[g][m][j] = v;
+          }
         }
-      }
-      const double c = r < ns ? cov_points(A.M, x0, x1, x2, t0, t1, t2) : 0.0;
+        return r < ns ? cov_points(M, x0, x1, x2, t0, t1, t2) : 0.0;
+      };
+      const double c = uniform ? assemble16(A.reqs[p0].M) : assemble16(L.M[g]);
       wave_sync();
       double a[16];
       static_for<0, 16>([&](auto mc) {
@@ -825,23 +788,23 @@ __global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t
         if (fail) {
           flag |= GSP_FLAG_FAIL;
           lam = 0.0;
-          sig = A.prior_sigma;
+          sig = R.prior_sigma;
         }
       } else {
         lam = 0.0;
-        sig = A.prior_sigma;
+        sig = R.prior_sigma;
       }
     }
     if (act) {
-      for (int col = r; col < A.kmax; col += 16) A.lam[i * A.kmax + col] = col < n ? lam : 0.0;
+      for (int col = r; col < A.K; col += 16) A.lam[(int64_t)gi * A.K + col] = col < n ? lam : 0.0;
     }
-    const bool dep = solve && !(flag & GSP_FLAG_FAIL) && r < n && nb < A.N;
+    const bool dep = solve && !(flag & GSP_FLAG_FAIL) && r < n && nb < A.NT;
     if (dep) atomicAdd(&A.outdeg[nb], 1);
     const uint64_t dm = __ballot(dep);
     if (act && r == 0) {
-      A.sigma[i] = sig;
-      A.flags[i] = (uint8_t)flag;
-      A.indeg[i] = __popcll((dm >> (16 * g)) & 0xFFFFull);
+      A.sigma[gi] = sig;
+      A.flags[gi] = (uint8_t)flag;
+      A.indeg[gi] = __popcll((dm >> (16 * g)) & 0xFFFFull);
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
@@ -893,18 +856,47 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     p[t] = v;
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+// Request p's words [0, n) of src go to dst[off ..]; blockIdx.y = request.
+struct Span8 {
+  const uint64_t* src;
+  int64_t off, n;
+};
+
+__global__ void k_gather8(const Span8* spans, uint64_t* dst) {
+  const Span8 s = spans[blockIdx.y];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < s.n;
        t += (int64_t)gridDim.x * blockDim.x)
-    p[t] = v;
+    dst[s.off + t] = s.src[t];
 }
 
-// pos[path[q]] = q, flagging out-of-range ids and repeats.
+// Request p's path (local ids, in place) becomes (p << 32) | global id;
+// node_req of the request's nodes is set. Out-of-range ids are flagged and
+// replaced by -1.
+__global__ void k_batch_path(int64_t* path, const int64_t* node_off, int32_t* node_req,
+                             int32_t* counters) {
+  const int p = blockIdx.y;
+  const int64_t off = node_off[p], n = node_off[p + 1] - off;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = path[off + t];
+    if (i < 0 || i >= n) {
+      atomicOr(&counters[CTR_POS_ERR], 1);
+      path[off + t] = -1;
+    } else {
+      path[off + t] = ((int64_t)p << 32) | (off + i);
+    }
+    node_req[off + t] = p;
+  }
+}
+
+// pos[i] = q for the node id i in the low 32 bits of path[q], flagging
+// out-of-range ids and repeats.
 __global__ void k_positions(const int64_t* path, int64_t N, int32_t* pos, int32_t* counters) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N;
        q += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = path[q];
-    if (i < 0 || i >= N) {
+    const int64_t v = path[q];
+    const int64_t i = v & 0xFFFFFFFF;
+    if (v < 0 || i >= N) {
       atomicOr(&counters[CTR_POS_ERR], 1);
       continue;
     }
@@ -921,16 +913,28 @@ __device__ inline int deps_of(const uint8_t* flags, const int32_t* nfound, int i
   return (flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : nfound[i];
 }
 
-__global__ void k_dep_fill(int32_t N, int32_t kmax, const int32_t* nfound, const uint8_t* flags,
+// Reverse edges in CSR form; each thread takes the edges of one node eight
+// at a time so that their atomics are in flight together.
+__global__ void k_dep_fill(int32_t N, int32_t K, const int32_t* nfound, const uint8_t* flags,
                            const int32_t* nbr, const int32_t* rowptr, int32_t* cursor,
                            int32_t* deps) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int k = deps_of(flags, nfound, (int)i);
-    const int32_t* nb = nbr + i * kmax;
-    for (int j = 0; j < k; ++j) {
-      const int a = nb[j];
-      if (a < N) deps[rowptr[a] + atomicAdd(&cursor[a], 1)] = (int)i;
+    const int32_t* nb = nbr + i * K;
+    for (int j0 = 0; j0 < k; j0 += 8) {
+      int a[8], slot[8], base[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = j0 + u < k ? nb[j0 + u] : N;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (a[u] < N) {
+          slot[u] = atomicAdd(&cursor[a[u]], 1);
+          base[u] = rowptr[a[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (a[u] < N) deps[base[u] + slot[u]] = (int)i;
     }
   }
 }
@@ -1010,16 +1014,17 @@ __global__ __launch_bounds__(256) void k_z_transpose(const double* z, int32_t n_
 struct LevelArgs {
   const int32_t* order;  // the nodes of this level
   int32_t count;
-  int32_t N, kmax, RC, shift;
+  int32_t K, RC, shift;
   const int32_t* pos;
   const int32_t* nfound;
   const uint8_t* flags;
   const int32_t* nbr;
   const double* lam;
   const double* sigma;
-  double mean;
-  const double* Zt;  // n_path x RC
-  double* X;         // (N + n_data) x RC
+  const int32_t* node_req;
+  const double* mean;  // per request
+  const double* Zt;    // n_path x RC
+  double* X;           // (NT + n_data) x RC
 };
 
 // All nodes of one level are independent: x_i = (mean + sum_j lambda_ij
@@ -1031,16 +1036,17 @@ __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
   const int r = (int)(t & ((1 << A.shift) - 1));
   if (slot >= A.count || r >= A.RC) return;
   const int i = A.order[slot];
+  const double mean = A.mean[A.node_req[i]];
   const int k = (A.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : A.nfound[i];
-  const int32_t* nb = A.nbr + (int64_t)i * A.kmax;
-  const double* lm = A.lam + (int64_t)i * A.kmax;
+  const int32_t* nb = A.nbr + (int64_t)i * A.K;
+  const double* lm = A.lam + (int64_t)i * A.K;
   double acc = 0.0;
-  for (int j = 0; j < k; ++j) acc = acc + lm[j] * (A.X[(int64_t)nb[j] * A.RC + r] - A.mean);
-  const double mu = A.mean + acc;
+  for (int j = 0; j < k; ++j) acc = acc + lm[j] * (A.X[(int64_t)nb[j] * A.RC + r] - mean);
+  const double mu = mean + acc;
   A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * A.Zt[(int64_t)A.pos[i] * A.RC + r];
 }
 
-// X rows N..N+n_data-1 hold the conditioning data for every realization.
+// X rows NT..NT+n_data-1 hold the conditioning data for every realization.
 __global__ void k_data_rows(double* X, int32_t N, int32_t n_data, int32_t RC,
                             const double* vals) {
   const int64_t total = (int64_t)n_data * RC;
@@ -1144,7 +1150,6 @@ std::vector<Offset> build_offsets(const gsp_grid& g, double r2, int kmax, int mo
   GSP_REQUIRE(lim[0] < 32768 && lim[1] < 32768 && lim[2] < 32768, GSP_ERR_UNSUPPORTED,
               "grid too large for the offset table");
   std::vector<Offset> v;
-  std::vector<int64_t> lin;
   v.reserve((size_t)total);
   for (int64_t dz = -lim[2]; dz <= lim[2]; ++dz)
     for (int64_t dy = -lim[1]; dy <= lim[1]; ++dy)
@@ -1175,7 +1180,7 @@ std::vector<Offset> build_offsets(const gsp_grid& g, double r2, int kmax, int mo
 }
 
 // seqsim.jl:162-189
-double scale_factor(const gsp_grid& g, const std::vector<double>& data, const gsp_model& m) {
+double scale_factor(const gsp_grid& g, const double* data, int64_t n, const gsp_model& m) {
   const int nd = g.ndim;
   double a1 = 0.0;
   for (int d = 0; d < nd; ++d) {
@@ -1184,10 +1189,133 @@ double scale_factor(const gsp_grid& g, const std::vector<double>& data, const gs
     a1 = std::max(a1, std::max(lo, hi));
   }
   double a2 = 0.0;
-  for (double v : data) a2 = std::max(a2, std::fabs(v));
+  for (int64_t t = 0; t < n; ++t) a2 = std::max(a2, std::fabs(data[t]));
   double a3 = 0.0;
   for (int d = 0; d < nd; ++d) a3 = std::max(a3, m.ranges[d]);
   return 1.0 / std::max(a1, std::max(a2, a3));
+}
+
+// Host-side preparation of one request: clamped neighbour counts, scale
+// factor, offset table and data hash (item ids local, hstart local).
+struct HostReq {
+  int64_t N = 0, nD = 0;
+  int nd = 0;
+  int32_t kmin = 0, kmax = 0;
+  double alpha = 0, r2 = 0;
+  const double* data = nullptr;  // nD x nd, host
+  std::vector<Offset> table;
+  int hd[3] = {1, 1, 1};
+  double hcs[3] = {1.0, 1.0, 1.0};
+  int hexp = 0;
+  int64_t hB = 1;
+  std::vector<int32_t> hstart;
+  std::vector<DataItem> hitems;
+};
+
+// Data hash. Expanded mode: cells of B^ndim grid cells, each listing every
+// datum within rho_max of the cell (rho_max bounds the k-th grid distance of
+// any node, so one list holds all candidates of a node). Box mode (large
+// rho_max, e.g. masked search in a big ball): each datum in its own cell and
+// nodes visit every cell overlapping their ball.
+void build_hash(const gsp_grid& g, int mode, HostReq& h) {
+  const int nd = h.nd;
+  const int64_t nD = h.nD;
+  const double* data = h.data;
+  const double inf = std::numeric_limits<double>::infinity();
+  h.hstart.assign(2, 0);
+  h.hitems.clear();
+  if (nD == 0) return;
+  const double rho_max = h.table.empty() ? 0.0 : std::sqrt(h.table.back().d2);
+  const bool bounded = !h.table.empty() && (mode == GSP_KNN_THEN_MASK || h.r2 < inf);
+  double smin = g.spacing[0];
+  for (int d = 1; d < nd; ++d) smin = std::min(smin, g.spacing[d]);
+  auto make_item = [&](int64_t i) {
+    DataItem it;
+    std::memset(&it, 0, sizeof(it));
+    for (int d = 0; d < nd; ++d) it.x[d] = data[(size_t)(i * nd + d)];
+    it.id = (int32_t)i;
+    return it;
+  };
+  if (bounded) {
+    h.hB = std::max<int64_t>(1, (int64_t)std::llround(rho_max / smin));
+    int64_t ncell = 1, span = 1;
+    for (int d = 0; d < nd; ++d) {
+      h.hd[d] = (int)((g.dims[d] + h.hB - 1) / h.hB);
+      h.hcs[d] = (double)h.hB * g.spacing[d];
+      ncell *= h.hd[d];
+      span *= std::min<int64_t>(h.hd[d], (int64_t)std::ceil(2.0 * rho_max / h.hcs[d]) + 2);
+    }
+    // worth it when the lists stay short (about one 64-lane pass per node)
+    h.hexp = (span * nD <= ((int64_t)1 << 24) && ncell <= ((int64_t)1 << 26) &&
+              span * nD <= 64 * ncell)
+                 ? 1
+                 : 0;
+    if (h.hexp) {
+      const double rho = rho_max * (1.0 + 1e-9) + 1e-12;
+      std::vector<int64_t> lo((size_t)(nD * 3)), hi((size_t)(nD * 3));
+      h.hstart.assign((size_t)ncell + 1, 0);
+      for (int64_t i = 0; i < nD; ++i) {
+        int64_t l[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+        for (int d = 0; d < nd; ++d) {
+          const double x = data[(size_t)(i * nd + d)];
+          double fl = std::floor((x - rho - g.origin[d]) / h.hcs[d]);
+          double fh = std::floor((x + rho - g.origin[d]) / h.hcs[d]);
+          l[d] = (int64_t)std::min(std::max(fl, 0.0), (double)(h.hd[d] - 1));
+          u[d] = (int64_t)std::min(std::max(fh, 0.0), (double)(h.hd[d] - 1));
+        }
+        for (int d = 0; d < 3; ++d) {
+          lo[(size_t)(i * 3 + d)] = l[d];
+          hi[(size_t)(i * 3 + d)] = u[d];
+        }
+        for (int64_t cz = l[2]; cz <= u[2]; ++cz)
+          for (int64_t cy = l[1]; cy <= u[1]; ++cy)
+            for (int64_t cx = l[0]; cx <= u[0]; ++cx)
+              h.hstart[(size_t)(cx + h.hd[0] * (cy + (int64_t)h.hd[1] * cz)) + 1]++;
+      }
+      for (int64_t c = 0; c < ncell; ++c) h.hstart[(size_t)c + 1] += h.hstart[(size_t)c];
+      h.hitems.assign((size_t)h.hstart[(size_t)ncell], DataItem{});
+      std::vector<int32_t> fill((size_t)ncell, 0);
+      for (int64_t i = 0; i < nD; ++i) {
+        const DataItem it = make_item(i);
+        for (int64_t cz = lo[(size_t)(i * 3 + 2)]; cz <= hi[(size_t)(i * 3 + 2)]; ++cz)
+          for (int64_t cy = lo[(size_t)(i * 3 + 1)]; cy <= hi[(size_t)(i * 3 + 1)]; ++cy)
+            for (int64_t cx = lo[(size_t)(i * 3)]; cx <= hi[(size_t)(i * 3)]; ++cx) {
+              const int64_t c = cx + h.hd[0] * (cy + (int64_t)h.hd[1] * cz);
+              h.hitems[(size_t)(h.hstart[(size_t)c] + fill[(size_t)c]++)] = it;
+            }
+      }
+    }
+  }
+  if (!h.hexp) {
+    int64_t B = (int64_t)std::llround(std::pow((double)h.N / (double)nD, 1.0 / nd));
+    if (B < 1) B = 1;
+    h.hB = B;
+    int64_t ncell = 1;
+    for (int d = 0; d < nd; ++d) {
+      h.hd[d] = (int)((g.dims[d] + B - 1) / B);
+      h.hcs[d] = (double)B * g.spacing[d];
+      ncell *= h.hd[d];
+    }
+    std::vector<int64_t> cell((size_t)nD);
+    h.hstart.assign((size_t)ncell + 1, 0);
+    for (int64_t i = 0; i < nD; ++i) {
+      int64_t c = 0, stride = 1;
+      for (int d = 0; d < nd; ++d) {
+        double f = std::floor((data[(size_t)(i * nd + d)] - g.origin[d]) / h.hcs[d]);
+        f = std::min(std::max(f, 0.0), (double)(h.hd[d] - 1));
+        c += (int64_t)f * stride;
+        stride *= h.hd[d];
+      }
+      cell[(size_t)i] = c;
+      h.hstart[(size_t)c + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) h.hstart[(size_t)c + 1] += h.hstart[(size_t)c];
+    h.hitems.assign((size_t)nD, DataItem{});
+    std::vector<int32_t> fill((size_t)ncell, 0);
+    for (int64_t i = 0; i < nD; ++i)
+      h.hitems[(size_t)(h.hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] =
+          make_item(i);
+  }
 }
 
 }  // namespace
@@ -1229,6 +1357,36 @@ int64_t validate_grid(const gsp_grid& g) {
   return N;
 }
 
+namespace {
+
+// Checks one problem and clamps its neighbour counts (seqsim.jl:202-208).
+void validate_problem(const gsp_seqsim_problem* p, HostReq& h) {
+  GSP_REQUIRE(p, GSP_ERR_ARGUMENT, "null problem");
+  const gsp_grid& g = p->grid;
+  h.N = validate_grid(g);
+  h.nd = g.ndim;
+  validate_model(p->model, h.nd);
+  GSP_REQUIRE(p->n_data >= 0 && h.N + p->n_data < ((int64_t)1 << 30), GSP_ERR_ARGUMENT,
+              "invalid number of data points");
+  GSP_REQUIRE(p->n_data == 0 || p->data_coords, GSP_ERR_ARGUMENT, "null data coordinates");
+  GSP_REQUIRE(p->n_path == h.N && p->path, GSP_ERR_UNSUPPORTED,
+              "the path must visit every grid node exactly once");
+  GSP_REQUIRE(p->search_mode == GSP_KNN_THEN_MASK || p->search_mode == GSP_MASKED_KNN,
+              GSP_ERR_ARGUMENT, "unknown search mode");
+  GSP_REQUIRE(p->precision == GSP_FP64 || p->precision == GSP_FP32_REFINE, GSP_ERR_ARGUMENT,
+              "unknown precision");
+  GSP_REQUIRE(std::isfinite(p->mean), GSP_ERR_ARGUMENT, "invalid mean");
+  int32_t nmin = p->minneighbors, nmax = p->maxneighbors;
+  if ((int64_t)nmax > h.N || nmax < 1) nmax = (int32_t)h.N;
+  if (nmin > nmax || nmin < 1) nmin = 1;
+  GSP_REQUIRE(nmax <= 64, GSP_ERR_UNSUPPORTED, "maxneighbors above 64 is not supported");
+  h.kmin = nmin;
+  h.kmax = nmax;
+  h.nD = p->n_data;
+  const double inf = std::numeric_limits<double>::infinity();
+  h.r2 = p->radius > 0 ? p->radius * p->radius : inf;
+}
+
 template <int KB>
 void launch_krige(const PlanArgs& A, int precision, const int32_t* list, int64_t count,
                   hipStream_t s) {
@@ -1243,238 +1401,318 @@ void launch_krige(const PlanArgs& A, int precision, const int32_t* list, int64_t
                        (int32_t)count);
 }
 
-void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_plan* plan) {
-  GSP_REQUIRE(p, GSP_ERR_ARGUMENT, "null problem");
-  const gsp_grid& g = p->grid;
-  const int64_t N = validate_grid(g);
-  const int nd = g.ndim;
-  validate_model(p->model, nd);
-  GSP_REQUIRE(p->n_data >= 0 && N + p->n_data < ((int64_t)1 << 30), GSP_ERR_ARGUMENT,
-              "invalid number of data points");
-  GSP_REQUIRE(p->n_data == 0 || p->data_coords, GSP_ERR_ARGUMENT, "null data coordinates");
-  GSP_REQUIRE(p->n_path == N && p->path, GSP_ERR_UNSUPPORTED,
-              "the path must visit every grid node exactly once");
-  GSP_REQUIRE(p->search_mode == GSP_KNN_THEN_MASK || p->search_mode == GSP_MASKED_KNN,
-              GSP_ERR_ARGUMENT, "unknown search mode");
-  GSP_REQUIRE(p->precision == GSP_FP64 || p->precision == GSP_FP32_REFINE, GSP_ERR_ARGUMENT,
-              "unknown precision");
-  GSP_REQUIRE(std::isfinite(p->mean), GSP_ERR_ARGUMENT, "invalid mean");
-  // seqsim.jl:202-208
-  int32_t nmin = p->minneighbors, nmax = p->maxneighbors;
-  if ((int64_t)nmax > N || nmax < 1) nmax = (int32_t)N;
-  if (nmin > nmax || nmin < 1) nmin = 1;
-  GSP_REQUIRE(nmax <= 64, GSP_ERR_UNSUPPORTED, "maxneighbors above 64 is not supported");
+// Runs `f` for request index q; on failure in a batch of several requests the
+// error message names the request.
+template <class F>
+void for_request(int32_t n, int32_t q, F&& f) {
+  try {
+    f();
+  } catch (const Fail&) {
+    if (n > 1) {
+      const std::string msg = gsp_last_error();
+      set_error("request %d: %s", q, msg.c_str());
+    }
+    throw;
+  }
+}
 
+// Runs f(q) for every request on up to 16 host threads (host-only work: no
+// HIP calls). The failure of the lowest request index is raised on the
+// calling thread, with its message.
+template <class F>
+void parallel_requests(int32_t n, F&& f) {
+  const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+  const int T = (int)std::min<unsigned>(std::min<unsigned>(hc, 16u), (unsigned)((n + 7) / 8));
+  if (T <= 1) {
+    for (int32_t q = 0; q < n; ++q) for_request(n, q, [&] { f(q); });
+    return;
+  }
+  std::vector<int32_t> codes((size_t)n, 0);
+  std::vector<std::string> msgs((size_t)n);
+  std::atomic<int32_t> next{0};
+  auto worker = [&] {
+    for (;;) {
+      const int32_t q = next.fetch_add(1);
+      if (q >= n) break;
+      try {
+        f(q);
+      } catch (const Fail& e) {
+        codes[(size_t)q] = e.code;
+        msgs[(size_t)q] = gsp_last_error();
+      } catch (const std::bad_alloc&) {
+        codes[(size_t)q] = GSP_ERR_MEMORY;
+        msgs[(size_t)q] = "host allocation failed";
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  for (int32_t q = 0; q < n; ++q)
+    if (codes[(size_t)q]) {
+      set_error("request %d: %s", q, msgs[(size_t)q].c_str());
+      throw Fail{codes[(size_t)q]};
+    }
+}
+
+}  // namespace
+
+void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
+                       gsp_seqsim_plan* plan) {
+  GSP_REQUIRE(problems, GSP_ERR_ARGUMENT, "null problem");
+  GSP_REQUIRE(n_req >= 1 && n_req <= 65535, GSP_ERR_ARGUMENT,
+              "number of requests must be 1..65535");
   hipStream_t s = ctx->stream;
   PhaseTrace tr(s);
-  const int64_t nD = p->n_data;
-  std::vector<double> data((size_t)(nD * nd));
-  if (nD > 0) {
-    if (p->device_pointers)
-      GSP_HIP(hipMemcpy(data.data(), p->data_coords, data.size() * sizeof(double),
-                        hipMemcpyDeviceToHost));
-    else
-      std::memcpy(data.data(), p->data_coords, data.size() * sizeof(double));
-    for (double v : data) GSP_REQUIRE(std::isfinite(v), GSP_ERR_ARGUMENT, "non-finite data coordinate");
-  }
-  const double alpha = scale_factor(g, data, p->model);
-  const double inf = std::numeric_limits<double>::infinity();
-  const double r2 = p->radius > 0 ? p->radius * p->radius : inf;
-  std::vector<Offset> table = build_offsets(g, r2, nmax, p->search_mode);
-
-  // Data hash. Expanded mode: cells of B^ndim grid cells, each listing every
-  // datum within rho_max of the cell (rho_max bounds the k-th grid distance of
-  // any node, so one list holds all candidates of a node). Box mode (large
-  // rho_max, e.g. masked search in a big ball): each datum in its own cell and
-  // nodes visit every cell overlapping their ball.
-  int hd[3] = {1, 1, 1};
-  double hcs[3] = {1.0, 1.0, 1.0};
-  int hexp = 0;
-  int64_t hB = 1;
-  std::vector<int32_t> hstart(2, 0);
-  std::vector<DataItem> hitems(1);
-  if (nD > 0) {
-    const double rho_max = table.empty() ? 0.0 : std::sqrt(table.back().d2);
-    const bool bounded = !table.empty() && (p->search_mode == GSP_KNN_THEN_MASK || r2 < inf);
-    double smin = g.spacing[0];
-    for (int d = 1; d < nd; ++d) smin = std::min(smin, g.spacing[d]);
-    auto make_item = [&](int64_t i) {
-      DataItem it;
-      std::memset(&it, 0, sizeof(it));
-      for (int d = 0; d < nd; ++d) it.x[d] = data[(size_t)(i * nd + d)];
-      it.id = (int32_t)i;
-      return it;
-    };
-    if (bounded) {
-      hB = std::max<int64_t>(1, (int64_t)std::llround(rho_max / smin));
-      int64_t ncell = 1, span = 1;
-      for (int d = 0; d < nd; ++d) {
-        hd[d] = (int)((g.dims[d] + hB - 1) / hB);
-        hcs[d] = (double)hB * g.spacing[d];
-        ncell *= hd[d];
-        span *= std::min<int64_t>(hd[d], (int64_t)std::ceil(2.0 * rho_max / hcs[d]) + 2);
-      }
-      // worth it when the lists stay short (about one 64-lane pass per node)
-      hexp = (span * nD <= ((int64_t)1 << 24) && ncell <= ((int64_t)1 << 26) &&
-              span * nD <= 64 * ncell)
-                 ? 1
-                 : 0;
-      if (hexp) {
-        const double rho = rho_max * (1.0 + 1e-9) + 1e-12;
-        std::vector<int64_t> lo((size_t)(nD * 3)), hi((size_t)(nD * 3));
-        hstart.assign((size_t)ncell + 1, 0);
-        for (int64_t i = 0; i < nD; ++i) {
-          int64_t l[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-          for (int d = 0; d < nd; ++d) {
-            const double x = data[(size_t)(i * nd + d)];
-            double fl = std::floor((x - rho - g.origin[d]) / hcs[d]);
-            double fh = std::floor((x + rho - g.origin[d]) / hcs[d]);
-            l[d] = (int64_t)std::min(std::max(fl, 0.0), (double)(hd[d] - 1));
-            h[d] = (int64_t)std::min(std::max(fh, 0.0), (double)(hd[d] - 1));
-          }
-          for (int d = 0; d < 3; ++d) {
-            lo[(size_t)(i * 3 + d)] = l[d];
-            hi[(size_t)(i * 3 + d)] = h[d];
-          }
-          for (int64_t cz = l[2]; cz <= h[2]; ++cz)
-            for (int64_t cy = l[1]; cy <= h[1]; ++cy)
-              for (int64_t cx = l[0]; cx <= h[0]; ++cx)
-                hstart[(size_t)(cx + hd[0] * (cy + (int64_t)hd[1] * cz)) + 1]++;
-        }
-        for (int64_t c = 0; c < ncell; ++c) hstart[(size_t)c + 1] += hstart[(size_t)c];
-        hitems.assign((size_t)std::max<int32_t>(hstart[(size_t)ncell], 1), DataItem{});
-        std::vector<int32_t> fill((size_t)ncell, 0);
-        for (int64_t i = 0; i < nD; ++i) {
-          const DataItem it = make_item(i);
-          for (int64_t cz = lo[(size_t)(i * 3 + 2)]; cz <= hi[(size_t)(i * 3 + 2)]; ++cz)
-            for (int64_t cy = lo[(size_t)(i * 3 + 1)]; cy <= hi[(size_t)(i * 3 + 1)]; ++cy)
-              for (int64_t cx = lo[(size_t)(i * 3)]; cx <= hi[(size_t)(i * 3)]; ++cx) {
-                const int64_t c = cx + hd[0] * (cy + (int64_t)hd[1] * cz);
-                hitems[(size_t)(hstart[(size_t)c] + fill[(size_t)c]++)] = it;
-              }
-        }
-      }
-    }
-    if (!hexp) {
-      int64_t B = (int64_t)std::llround(std::pow((double)N / (double)nD, 1.0 / nd));
-      if (B < 1) B = 1;
-      hB = B;
-      int64_t ncell = 1;
-      for (int d = 0; d < nd; ++d) {
-        hd[d] = (int)((g.dims[d] + B - 1) / B);
-        hcs[d] = (double)B * g.spacing[d];
-        ncell *= hd[d];
-      }
-      std::vector<int64_t> cell((size_t)nD);
-      hstart.assign((size_t)ncell + 1, 0);
-      for (int64_t i = 0; i < nD; ++i) {
-        int64_t c = 0, stride = 1;
-        for (int d = 0; d < nd; ++d) {
-          double f = std::floor((data[(size_t)(i * nd + d)] - g.origin[d]) / hcs[d]);
-          f = std::min(std::max(f, 0.0), (double)(hd[d] - 1));
-          c += (int64_t)f * stride;
-          stride *= hd[d];
-        }
-        cell[(size_t)i] = c;
-        hstart[(size_t)c + 1]++;
-      }
-      for (int64_t c = 0; c < ncell; ++c) hstart[(size_t)c + 1] += hstart[(size_t)c];
-      hitems.assign((size_t)nD, DataItem{});
-      std::vector<int32_t> fill((size_t)ncell, 0);
-      for (int64_t i = 0; i < nD; ++i)
-        hitems[(size_t)(hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] =
-            make_item(i);
-    }
+  std::vector<HostReq> H((size_t)n_req);
+  const int32_t precision = problems[0].precision;
+  const int32_t devp = problems[0].device_pointers ? 1 : 0;
+  int64_t NT = 0, nDT = 0, nC = 0;
+  int32_t K = 1;
+  for (int32_t q = 0; q < n_req; ++q) {
+    for_request(n_req, q, [&] {
+      const gsp_seqsim_problem* p = &problems[q];
+      validate_problem(p, H[(size_t)q]);
+      GSP_REQUIRE(p->precision == precision, GSP_ERR_ARGUMENT,
+                  "all requests of a batch must use one precision");
+      GSP_REQUIRE((p->device_pointers ? 1 : 0) == devp, GSP_ERR_ARGUMENT,
+                  "all requests of a batch must pass the same kind of pointers");
+    });
+    const HostReq& h = H[(size_t)q];
+    NT += h.N;
+    nDT += h.nD;
+    nC += h.nD * h.nd;
+    K = std::max(K, h.kmax);
+    GSP_REQUIRE(NT + nDT < ((int64_t)1 << 30), GSP_ERR_UNSUPPORTED,
+                "batch too large (nodes plus data must stay below 2^30)");
   }
 
-  plan->N = N;
-  plan->n_data = nD;
-  plan->ndim = nd;
-  plan->kmin = nmin;
-  plan->kmax = nmax;
-  plan->alpha = alpha;
-  plan->mean = p->mean;
-  plan->path.alloc((size_t)N);
-  plan->nfound.alloc((size_t)N);
-  plan->nbr.alloc((size_t)(N * nmax));
-  plan->lam.alloc((size_t)(N * nmax));
-  plan->sigma.alloc((size_t)N);
-  plan->flags.alloc((size_t)N);
-  plan->level.alloc((size_t)N);
-  plan->order.alloc((size_t)N);
-  plan->pos.alloc((size_t)N);
+  // per-request offsets of nodes, data and packed coordinates
+  std::vector<int64_t> node_off((size_t)n_req + 1, 0), data_off((size_t)n_req + 1, 0),
+      coord_off((size_t)n_req + 1, 0);
+  for (int32_t q = 0; q < n_req; ++q) {
+    const HostReq& h = H[(size_t)q];
+    node_off[(size_t)q + 1] = node_off[(size_t)q] + h.N;
+    data_off[(size_t)q + 1] = data_off[(size_t)q] + h.nD;
+    coord_off[(size_t)q + 1] = coord_off[(size_t)q] + h.nD * h.nd;
+  }
+
+  ctx->counters.reserve(16);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->counters.p, 0, (int64_t)16);
+  DevBuf<int64_t> path, dnode_off;
+  path.alloc((size_t)NT);
+  dnode_off.alloc((size_t)n_req + 1);
+  GSP_HIP(hipMemcpyAsync(dnode_off.p, node_off.data(), node_off.size() * sizeof(int64_t),
+                         hipMemcpyHostToDevice, s));
+
+  // conditioning coordinates on the host (the hash is built there)
+  std::vector<double> coords((size_t)std::max<int64_t>(nC, 1));
+  if (devp) {
+    // one gather launch per array over all requests instead of a copy each
+    std::vector<Span8> spans((size_t)n_req);
+    DevBuf<Span8> dspans;
+    dspans.alloc((size_t)n_req);
+    const int gx = grid_for(NT / n_req + 1, 256, 1024);
+    for (int32_t q = 0; q < n_req; ++q)
+      spans[(size_t)q] = Span8{(const uint64_t*)problems[q].path, node_off[(size_t)q],
+                               H[(size_t)q].N};
+    GSP_HIP(hipMemcpyAsync(dspans.p, spans.data(), spans.size() * sizeof(Span8),
+                           hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather8, dim3(gx, n_req), dim3(256), 0, s, dspans.p,
+                       (uint64_t*)path.p);
+    if (nC > 0) {
+      std::vector<Span8> cs((size_t)n_req);
+      DevBuf<Span8> cspans;
+      cspans.alloc((size_t)n_req);
+      DevBuf<double> dcoords;
+      dcoords.alloc((size_t)nC);
+      for (int32_t q = 0; q < n_req; ++q)
+        cs[(size_t)q] = Span8{(const uint64_t*)problems[q].data_coords, coord_off[(size_t)q],
+                              H[(size_t)q].nD * H[(size_t)q].nd};
+      GSP_HIP(hipMemcpyAsync(cspans.p, cs.data(), cs.size() * sizeof(Span8),
+                             hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_gather8, dim3(grid_for(nC / n_req + 1, 256, 64), n_req), dim3(256), 0,
+                         s, cspans.p, (uint64_t*)dcoords.p);
+      GSP_HIP(hipMemcpyAsync(coords.data(), dcoords.p, (size_t)nC * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    }
+    GSP_HIP(hipStreamSynchronize(s));  // the span tables leave scope
+  } else {
+    if (n_req == 1) {
+      GSP_HIP(hipMemcpyAsync(path.p, problems[0].path, (size_t)NT * sizeof(int64_t),
+                             hipMemcpyHostToDevice, s));
+    } else {
+      std::vector<int64_t> hp((size_t)NT);
+      for (int32_t q = 0; q < n_req; ++q)
+        std::memcpy(hp.data() + node_off[(size_t)q], problems[q].path,
+                    (size_t)H[(size_t)q].N * sizeof(int64_t));
+      GSP_HIP(hipMemcpyAsync(path.p, hp.data(), (size_t)NT * sizeof(int64_t),
+                             hipMemcpyHostToDevice, s));
+      GSP_HIP(hipStreamSynchronize(s));  // hp leaves scope
+    }
+    for (int32_t q = 0; q < n_req; ++q)
+      if (H[(size_t)q].nD > 0)
+        std::memcpy(coords.data() + coord_off[(size_t)q], problems[q].data_coords,
+                    (size_t)(H[(size_t)q].nD * H[(size_t)q].nd) * sizeof(double));
+  }
+  tr.mark("gather inputs");
+
+  // host preparation: scale factors, offset tables, data hashes
+  parallel_requests(n_req, [&](int32_t q) {
+    HostReq& h = H[(size_t)q];
+    const gsp_seqsim_problem* p = &problems[q];
+    h.data = coords.data() + coord_off[(size_t)q];
+    for (int64_t t = 0; t < h.nD * h.nd; ++t)
+      GSP_REQUIRE(std::isfinite(h.data[t]), GSP_ERR_ARGUMENT, "non-finite data coordinate");
+    h.alpha = scale_factor(p->grid, h.data, h.nD * h.nd, p->model);
+    h.table = build_offsets(p->grid, h.r2, h.kmax, p->search_mode);
+    build_hash(p->grid, p->search_mode, h);
+  });
+  std::vector<int64_t> table_off((size_t)n_req + 1, 0), cell_off((size_t)n_req + 1, 0),
+      item_off((size_t)n_req + 1, 0);
+  for (int32_t q = 0; q < n_req; ++q) {
+    table_off[(size_t)q + 1] = table_off[(size_t)q] + (int64_t)H[(size_t)q].table.size();
+    cell_off[(size_t)q + 1] = cell_off[(size_t)q] + (int64_t)H[(size_t)q].hstart.size();
+    item_off[(size_t)q + 1] = item_off[(size_t)q] + (int64_t)H[(size_t)q].hitems.size();
+  }
+  const int64_t n_table = table_off.back(), n_cells = cell_off.back(), n_items = item_off.back();
+  GSP_REQUIRE(n_items < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "data hash too large");
+
+  // concatenated device tables
+  std::vector<Req> reqs((size_t)n_req);
+  std::vector<Offset> table((size_t)std::max<int64_t>(n_table, 1));
+  std::vector<int32_t> hstart((size_t)n_cells);
+  std::vector<DataItem> hitems((size_t)std::max<int64_t>(n_items, 1));
+  std::vector<double> data3((size_t)std::max<int64_t>(nDT * 3, 1), 0.0);
+  std::vector<double> means((size_t)n_req);
+  {
+    parallel_requests(n_req, [&](int32_t q) {
+      const int64_t t_off = table_off[(size_t)q], c_off = cell_off[(size_t)q],
+                    i_off = item_off[(size_t)q];
+      const HostReq& h = H[(size_t)q];
+      const gsp_seqsim_problem* p = &problems[q];
+      const gsp_grid& g = p->grid;
+      Req& R = reqs[(size_t)q];
+      std::memset(&R, 0, sizeof(R));
+      R.ndim = h.nd;
+      R.mode = p->search_mode;
+      R.kmin = h.kmin;
+      R.kmax = h.kmax;
+      R.T = (int32_t)h.table.size();
+      R.hexp = h.hexp;
+      R.hB = (int32_t)h.hB;
+      R.n_data = (int32_t)h.nD;
+      R.node_off = (int32_t)node_off[(size_t)q];
+      R.data_off = (int32_t)data_off[(size_t)q];
+      R.table_off = t_off;
+      R.cell_off = c_off;
+      R.r2 = h.r2;
+      R.alpha = h.alpha;
+      R.prior_sigma = std::sqrt(p->model.sill);
+      R.mean = p->mean;
+      for (int d = 0; d < 3; ++d) {
+        const bool in = d < h.nd;
+        R.dims[d] = in ? (int32_t)g.dims[d] : 1;
+        R.org[d] = in ? g.origin[d] : 0.0;
+        R.sp[d] = in ? g.spacing[d] : 1.0;
+        R.ao[d] = in ? h.alpha * g.origin[d] : 0.0;
+        R.as[d] = in ? h.alpha * g.spacing[d] : 0.0;
+        R.hd[d] = h.hd[d];
+        R.hcs[d] = h.hcs[d];
+      }
+      R.M = make_dev_model(p->model, h.nd, h.alpha);
+      means[(size_t)q] = p->mean;
+      std::copy(h.table.begin(), h.table.end(), table.begin() + t_off);
+      for (size_t c = 0; c < h.hstart.size(); ++c)
+        hstart[(size_t)c_off + c] = (int32_t)(i_off + h.hstart[c]);
+      for (size_t t = 0; t < h.hitems.size(); ++t) {
+        DataItem it = h.hitems[t];
+        it.id += R.data_off;
+        hitems[(size_t)i_off + t] = it;
+      }
+      for (int64_t i = 0; i < h.nD; ++i)
+        for (int d = 0; d < h.nd; ++d)
+          data3[(size_t)((data_off[(size_t)q] + i) * 3 + d)] = h.data[(size_t)(i * h.nd + d)];
+    });
+  }
+  tr.mark("host tables");
+
+  plan->N = NT;
+  plan->n_data = nDT;
+  plan->n_req = n_req;
+  plan->ndim = H[0].nd;
+  plan->kmin = H[0].kmin;
+  for (const HostReq& h : H) plan->kmin = std::min(plan->kmin, h.kmin);
+  plan->kmax = K;
+  plan->alpha = H[0].alpha;
+  plan->mean = problems[0].mean;
+  plan->node_off = node_off;
+  plan->data_off = data_off;
+  plan->nfound.alloc((size_t)NT);
+  plan->nbr.alloc((size_t)(NT * K));
+  plan->lam.alloc((size_t)(NT * K));
+  plan->sigma.alloc((size_t)NT);
+  plan->flags.alloc((size_t)NT);
+  plan->level.alloc((size_t)NT);
+  plan->order.alloc((size_t)NT);
+  plan->pos.alloc((size_t)NT);
+  plan->node_req.alloc((size_t)NT);
+  plan->req_mean.alloc((size_t)n_req);
   DevBuf<int32_t>& pos = plan->pos;
+  DevBuf<Req> dreqs;
+  dreqs.alloc((size_t)n_req);
   DevBuf<Offset> dtable;
   dtable.alloc(table.size());
   DevBuf<double> ddata;
-  ddata.alloc((size_t)std::max<int64_t>(1, nD * nd));
+  ddata.alloc(data3.size());
   DevBuf<int32_t> dhstart;
   DevBuf<DataItem> dhitems;
   dhstart.alloc(hstart.size());
   dhitems.alloc(hitems.size());
-  ctx->counters.reserve(16);
-
-  GSP_HIP(hipMemcpyAsync(plan->path.p, p->path, (size_t)N * sizeof(int64_t),
-                         p->device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  GSP_HIP(hipMemcpyAsync(dreqs.p, reqs.data(), reqs.size() * sizeof(Req), hipMemcpyHostToDevice, s));
+  GSP_HIP(hipMemcpyAsync(plan->req_mean.p, means.data(), means.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
   GSP_HIP(hipMemcpyAsync(dtable.p, table.data(), table.size() * sizeof(Offset),
                          hipMemcpyHostToDevice, s));
-  if (nD > 0)
-    GSP_HIP(hipMemcpyAsync(ddata.p, data.data(), data.size() * sizeof(double),
-                           hipMemcpyHostToDevice, s));
+  GSP_HIP(hipMemcpyAsync(ddata.p, data3.data(), data3.size() * sizeof(double),
+                         hipMemcpyHostToDevice, s));
   GSP_HIP(hipMemcpyAsync(dhstart.p, hstart.data(), hstart.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, s));
   GSP_HIP(hipMemcpyAsync(dhitems.p, hitems.data(), hitems.size() * sizeof(DataItem),
                          hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, ctx->counters.p, 0, (int64_t)16);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, pos.p, -1, N);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->level.p, 0, N);
-  hipLaunchKernelGGL(k_positions, dim3(grid_for(N, 256, 4096)), dim3(256), 0, s, plan->path.p, N,
-                     pos.p, ctx->counters.p);
+  const int gn = grid_for(NT, 256, 4096);
+  hipLaunchKernelGGL(k_batch_path, dim3(grid_for(NT / n_req + 1, 256, 1024), n_req), dim3(256), 0,
+                     s, path.p, dnode_off.p, plan->node_req.p, ctx->counters.p);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, pos.p, -1, NT);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, plan->level.p, 0, NT);
+  hipLaunchKernelGGL(k_positions, dim3(gn), dim3(256), 0, s, path.p, NT, pos.p, ctx->counters.p);
   GSP_HIP(hipGetLastError());
   int32_t pos_err = 0;
   GSP_HIP(hipMemcpyAsync(&pos_err, ctx->counters.p + CTR_POS_ERR, sizeof(int32_t),
                          hipMemcpyDeviceToHost, s));
-  GSP_HIP(hipStreamSynchronize(s));
+  GSP_HIP(hipStreamSynchronize(s));  // also ends the host tables' uploads
   GSP_REQUIRE(pos_err == 0, GSP_ERR_UNSUPPORTED,
               "the path must be a permutation of the grid ids 0..N-1");
 
   DevBuf<int32_t> outdeg, indeg;
-  outdeg.alloc((size_t)N + 1);
-  indeg.alloc((size_t)N);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 1, 256, 4096)), dim3(256), 0, s, outdeg.p, 0,
-                     N + 1);
-  tr.mark("alloc+upload+positions");
+  outdeg.alloc((size_t)NT + 1);
+  indeg.alloc((size_t)NT);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 1, 256, 4096)), dim3(256), 0, s, outdeg.p, 0,
+                     NT + 1);
+  tr.mark("upload+positions");
   PlanArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.N = (int32_t)N;
-  A.ndim = nd;
-  A.n_data = (int32_t)nD;
-  for (int d = 0; d < 3; ++d) {
-    A.dims[d] = d < nd ? (int32_t)g.dims[d] : 1;
-    A.org[d] = d < nd ? g.origin[d] : 0.0;
-    A.sp[d] = d < nd ? g.spacing[d] : 1.0;
-    A.ao[d] = d < nd ? alpha * g.origin[d] : 0.0;
-    A.as[d] = d < nd ? alpha * g.spacing[d] : 0.0;
-    A.hd[d] = hd[d];
-    A.hcs[d] = hcs[d];
-  }
-  A.kmin = nmin;
-  A.kmax = nmax;
-  A.mode = p->search_mode;
-  A.precision = p->precision;
-  A.T = (int32_t)table.size();
-  A.r2 = r2;
-  A.alpha = alpha;
-  A.prior_sigma = std::sqrt(p->model.sill);
-  A.M = make_dev_model(p->model, nd, alpha);
-  A.path = plan->path.p;
+  A.NT = (int32_t)NT;
+  A.K = K;
+  A.reqs = dreqs.p;
+  A.node_req = plan->node_req.p;
+  A.path = path.p;
   A.pos = pos.p;
   A.table = dtable.p;
   A.data = ddata.p;
   A.hstart = dhstart.p;
   A.hitems = dhitems.p;
-  A.hexp = hexp;
-  A.hB = (int32_t)hB;
   A.nfound = plan->nfound.p;
   A.nbr = plan->nbr.p;
   A.lam = plan->lam.p;
@@ -1486,85 +1724,87 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
 
   {
     TimedLaunch tl(ctx, GSP_K_SEARCH);
-    hipLaunchKernelGGL(k_search, dim3(grid_for(N, kWaves, 16384)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_search, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
     GSP_HIP(hipGetLastError());
   }
   tr.mark("search kernel");
   {
-    // bucket the nodes by neighbour count: n <= 16 run four per wavefront
+    // bucket the nodes by neighbour count: n <= 16 run four per wavefront,
+    // larger systems in the smallest register tile that holds them
     DevBuf<int32_t> nhist, list;
     nhist.alloc(65);
-    list.alloc((size_t)N);
+    list.alloc((size_t)NT);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, nhist.p, 0, (int64_t)65);
-    hipLaunchKernelGGL(k_n_hist, dim3(grid_for(N, 256, 1024)), dim3(256), 0, s, plan->nfound.p,
-                       (int32_t)N, nhist.p);
+    hipLaunchKernelGGL(k_n_hist, dim3(grid_for(NT, 256, 1024)), dim3(256), 0, s, plan->nfound.p,
+                       (int32_t)NT, nhist.p);
     int32_t hh[65];
     GSP_HIP(hipMemcpyAsync(hh, nhist.p, sizeof(hh), hipMemcpyDeviceToHost, s));
     GSP_HIP(hipStreamSynchronize(s));
     int32_t cur[65];
-    int64_t acc = 0, small = 0;
+    int64_t upto[65];  // nodes with at most k neighbours
+    int64_t acc = 0;
     for (int k = 0; k <= 64; ++k) {
       cur[k] = (int32_t)acc;
       acc += hh[k];
-      if (k <= 16) small = acc;
+      upto[k] = acc;
     }
     GSP_HIP(hipMemcpyAsync(nhist.p, cur, sizeof(cur), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_n_scatter, dim3((unsigned)((N + kScatterChunk - 1) / kScatterChunk)),
-                       dim3(256), 0, s, plan->nfound.p, (int32_t)N, nhist.p, list.p);
+    hipLaunchKernelGGL(k_n_scatter, dim3((unsigned)((NT + kScatterChunk - 1) / kScatterChunk)),
+                       dim3(256), 0, s, plan->nfound.p, (int32_t)NT, nhist.p, list.p);
     tr.mark("bucket by size");
     TimedLaunch tl(ctx, GSP_K_PLAN);
+    const int64_t small = upto[16];
     if (small > 0) {
       const int blocks = grid_for((small + 3) / 4, kWaves, 16384);
-      if (p->precision == GSP_FP32_REFINE)
+      if (precision == GSP_FP32_REFINE)
         hipLaunchKernelGGL(k_krige16<GSP_FP32_REFINE>, dim3(blocks), dim3(256), 0, s, A, list.p,
                            (int32_t)small);
       else
         hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(256), 0, s, A, list.p,
                            (int32_t)small);
     }
-    const int32_t* big = list.p + small;
-    const int64_t nbig = N - small;
-    if (nmax <= 24) launch_krige<24>(A, p->precision, big, nbig, s);
-    else if (nmax <= 32) launch_krige<32>(A, p->precision, big, nbig, s);
-    else if (nmax <= 48) launch_krige<48>(A, p->precision, big, nbig, s);
-    else launch_krige<64>(A, p->precision, big, nbig, s);
+    launch_krige<24>(A, precision, list.p + upto[16], upto[24] - upto[16], s);
+    launch_krige<32>(A, precision, list.p + upto[24], upto[32] - upto[24], s);
+    launch_krige<48>(A, precision, list.p + upto[32], upto[48] - upto[32], s);
+    launch_krige<64>(A, precision, list.p + upto[48], upto[64] - upto[48], s);
     GSP_HIP(hipGetLastError());
   }
   tr.mark("kriging kernel");
   {
     TimedLaunch tl(ctx, GSP_K_LEVELS);
     DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
-    rowptr.alloc((size_t)N + 1);
-    cnt.alloc((size_t)N + 16);
-    off.alloc((size_t)N + 16);
-    const int nb = grid_for(N, 256, 4096);
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N + 16, 256, 4096)), dim3(256), 0, s, cnt.p, 0,
-                       N + 16);
+    rowptr.alloc((size_t)NT + 1);
+    cnt.alloc((size_t)NT + 16);
+    off.alloc((size_t)NT + 16);
+    const int nb = grid_for(NT, 256, 4096);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 16, 256, 4096)), dim3(256), 0, s, cnt.p, 0,
+                       NT + 16);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, off.p, 0, (int64_t)1);
     size_t tmp_bytes = 0;
-    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, outdeg.p, rowptr.p, (int)N + 1, s));
+    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, outdeg.p, rowptr.p, (int)NT + 1,
+                                             s));
     DevBuf<char> tmp;
     tmp.alloc(tmp_bytes);
-    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, outdeg.p, rowptr.p, (int)N + 1, s));
+    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, outdeg.p, rowptr.p, (int)NT + 1, s));
     int32_t n_edges = 0;
-    GSP_HIP(hipMemcpyAsync(&n_edges, rowptr.p + N, 4, hipMemcpyDeviceToHost, s));
+    GSP_HIP(hipMemcpyAsync(&n_edges, rowptr.p + NT, 4, hipMemcpyDeviceToHost, s));
     GSP_HIP(hipStreamSynchronize(s));
     tr.mark("levels: count+scan");
     deps.alloc((size_t)std::max(n_edges, 1));
-    cursor.alloc((size_t)N);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, cursor.p, 0, N);
-    hipLaunchKernelGGL(k_dep_fill, dim3(nb), dim3(256), 0, s, (int32_t)N, nmax, plan->nfound.p,
+    cursor.alloc((size_t)NT);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, cursor.p, 0, NT);
+    hipLaunchKernelGGL(k_dep_fill, dim3(nb), dim3(256), 0, s, (int32_t)NT, K, plan->nfound.p,
                        plan->flags.p, plan->nbr.p, rowptr.p, cursor.p, deps.p);
-    hipLaunchKernelGGL(k_frontier0, dim3(nb), dim3(256), 0, s, (int32_t)N, indeg.p,
+    hipLaunchKernelGGL(k_frontier0, dim3(nb), dim3(256), 0, s, (int32_t)NT, indeg.p,
                        plan->order.p, cnt.p, plan->level.p);
     tr.mark("levels: fill");
     // one launch per level; the frontier sizes are checked every kBatch levels
     constexpr int kBatch = 16;
-    const int fb = grid_for(N / 16, 256, 1024);
+    const int fb = grid_for(NT / 16, 256, 1024);
     int32_t depth = -1;
     std::vector<int32_t> hc(kBatch + 1);
     for (int32_t L = 0; depth < 0; L += kBatch) {
-      GSP_REQUIRE(L + kBatch + 1 < N + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
+      GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
       for (int b = 0; b < kBatch; ++b)
         hipLaunchKernelGGL(k_frontier_step, dim3(fb), dim3(256), 0, s, L + b, rowptr.p, deps.p,
                            indeg.p, plan->order.p, off.p, cnt.p, plan->level.p);
@@ -1577,7 +1817,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
         }
     }
     tr.mark("levels: frontiers");
-    GSP_REQUIRE(depth >= 1 || N == 0, GSP_ERR_INTERNAL, "empty dependency levels");
+    GSP_REQUIRE(depth >= 1, GSP_ERR_INTERNAL, "empty dependency levels");
     std::vector<int32_t> hcnt((size_t)depth);
     GSP_HIP(hipMemcpyAsync(hcnt.data(), cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, s));
     GSP_HIP(hipStreamSynchronize(s));
@@ -1585,9 +1825,9 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
     plan->level_off.assign((size_t)depth + 1, 0);
     for (int32_t l = 0; l < depth; ++l)
       plan->level_off[(size_t)l + 1] = plan->level_off[(size_t)l] + hcnt[(size_t)l];
-    GSP_REQUIRE(plan->level_off[(size_t)depth] == N, GSP_ERR_INTERNAL,
+    GSP_REQUIRE(plan->level_off[(size_t)depth] == NT, GSP_ERR_INTERNAL,
                 "dependency levels cover %lld of %lld nodes",
-                (long long)plan->level_off[(size_t)depth], (long long)N);
+                (long long)plan->level_off[(size_t)depth], (long long)NT);
   }
   int32_t ctr[16];
   GSP_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
@@ -1605,6 +1845,7 @@ void seqsim_plan_build(gsp_ctx* ctx, const gsp_seqsim_problem* p, gsp_seqsim_pla
 constexpr int64_t kSweepCacheBytes = (int64_t)1 << 40;
 
 int sweep_chunk(int64_t rows, int32_t R) {
+  (void)R;
   if (const char* env = std::getenv("GSP_SWEEP_CHUNK")) {
     const int v = std::atoi(env);
     if (v >= 1 && v <= 64) return v;
@@ -1635,8 +1876,7 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
                          (int32_t)N, RC, r0, ctx->Zt.p);
     }
     LevelArgs A;
-    A.N = (int32_t)N;
-    A.kmax = plan->kmax;
+    A.K = plan->kmax;
     A.RC = RC;
     A.shift = shift;
     A.pos = plan->pos.p;
@@ -1645,7 +1885,8 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
     A.nbr = plan->nbr.p;
     A.lam = plan->lam.p;
     A.sigma = plan->sigma.p;
-    A.mean = plan->mean;
+    A.node_req = plan->node_req.p;
+    A.mean = plan->req_mean.p;
     A.Zt = ctx->Zt.p;
     A.X = ctx->X.p;
     {

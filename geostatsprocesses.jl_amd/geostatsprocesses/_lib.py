@@ -106,6 +106,9 @@ SIGNATURES = {
     "gsp_seqsim_plan_export": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "gsp_seqsim_sweep": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP, _I32]),
     "gsp_seqsim_plan_free": (_I32, [_VP]),
+    "gsp_seqsim_plan_create_batch": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
+                                            ctypes.POINTER(_VP)]),
+    "gsp_seqsim_plan_layout": (_I32, [_VP, ctypes.POINTER(_I32), _VP, _VP]),
     "gsp_lusim_factor": (_I32, [_VP, ctypes.POINTER(GspLusimProblem), ctypes.POINTER(_VP)]),
     "gsp_lusim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
     "gsp_lusim_dims": (_I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
@@ -276,16 +279,36 @@ def default_context(device: int | None = None) -> Context:
 
 
 class SeqsimPlan:
-    """gsp_seqsim_plan: realization-invariant neighbours, weights and sigmas."""
+    """gsp_seqsim_plan: realization-invariant neighbours, weights and sigmas.
 
-    def __init__(self, ctx: Context, problem: GspSeqsimProblem, keep=()):
+    `problem` is one GspSeqsimProblem, or a sequence of them for a batch of
+    independent requests sharing one plan (gsp_seqsim_plan_create_batch)."""
+
+    def __init__(self, ctx: Context, problem, keep=()):
         self.ctx = ctx
         self._keep = keep
         self._h = ctypes.c_void_p()
-        check(load().gsp_seqsim_plan_create(ctx.handle, ctypes.byref(problem), ctypes.byref(self._h)))
+        if isinstance(problem, GspSeqsimProblem):
+            check(load().gsp_seqsim_plan_create(ctx.handle, ctypes.byref(problem),
+                                                ctypes.byref(self._h)))
+        else:
+            arr = (GspSeqsimProblem * len(problem))(*problem)
+            self._keep = (keep, arr)
+            check(load().gsp_seqsim_plan_create_batch(ctx.handle, len(problem), arr,
+                                                      ctypes.byref(self._h)))
         info = GspSeqsimInfo()
         check(load().gsp_seqsim_plan_info(self._h, ctypes.byref(info)))
         self.info = info
+        n = ctypes.c_int32()
+        check(load().gsp_seqsim_plan_layout(self._h, ctypes.byref(n), None, None))
+        self.node_offset = np.zeros(n.value + 1, np.int64)
+        self.data_offset = np.zeros(n.value + 1, np.int64)
+        check(load().gsp_seqsim_plan_layout(self._h, ctypes.byref(n), ptr(self.node_offset),
+                                            ptr(self.data_offset)))
+
+    @property
+    def n_requests(self):
+        return len(self.node_offset) - 1
 
     @property
     def n_nodes(self):
@@ -295,12 +318,13 @@ class SeqsimPlan:
     def kmax(self):
         return self.info.maxneighbors
 
-    def export(self):
+    def export(self, fields=("nfound", "nbr", "lam", "sigma", "flags", "level")):
         N, K = self.info.n_nodes, self.info.maxneighbors
-        out = dict(nfound=np.zeros(N, np.int32), nbr=np.zeros((N, K), np.int32),
-                   lam=np.zeros((N, K)), sigma=np.zeros(N), flags=np.zeros(N, np.uint8),
-                   level=np.zeros(N, np.int32))
-        check(load().gsp_seqsim_plan_export(self.ctx.handle, self._h, *(ptr(out[k]) for k in
+        make = dict(nfound=lambda: np.zeros(N, np.int32), nbr=lambda: np.zeros((N, K), np.int32),
+                    lam=lambda: np.zeros((N, K)), sigma=lambda: np.zeros(N),
+                    flags=lambda: np.zeros(N, np.uint8), level=lambda: np.zeros(N, np.int32))
+        out = {k: make[k]() for k in fields}
+        check(load().gsp_seqsim_plan_export(self.ctx.handle, self._h, *(ptr(out.get(k)) for k in
                                             ("nfound", "nbr", "lam", "sigma", "flags", "level"))))
         return out
 

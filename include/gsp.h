@@ -218,6 +218,33 @@ int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                          int32_t device_pointers);
 int32_t gsp_seqsim_plan_free(gsp_seqsim_plan* plan);
 
+/* ---- SEQSIM batches --------------------------------------------------------
+ * gsp_seqsim_plan_create_batch builds one plan for n independent SEQSIM
+ * requests, each with its own grid, variogram, data, path and neighbourhood:
+ * the work of n calls of preprocess(rng, process, ::SEQSIM, ...) (seqsim.jl:
+ * 50-70), which the reference runs one process at a time. The requests share
+ * one global index space: request p owns node ids and path positions
+ * [node_offset[p], node_offset[p+1]) and data ids [data_offset[p],
+ * data_offset[p+1]) (see gsp_seqsim_plan_layout); every kernel runs once over
+ * the whole batch. All requests must use one `precision` and one kind of
+ * pointers (`device_pointers`). A plan made by gsp_seqsim_plan_create is a
+ * batch of one.
+ *
+ * For a batch, gsp_seqsim_sweep takes z as R x N_total (request p's path
+ * position q in column node_offset[p] + q), data_vals as the requests' data
+ * values one after another (n_data_total), and writes out as R x N_total
+ * (request p's grid id i in column node_offset[p] + i); every request draws
+ * R realizations. gsp_seqsim_plan_export returns the global arrays: neighbour
+ * ids are global node ids, or N_total + global data id, in rows of stride
+ * info.maxneighbors (the largest maxneighbors of the batch). */
+int32_t gsp_seqsim_plan_create_batch(gsp_ctx* ctx, int32_t n,
+                                     const gsp_seqsim_problem* problems,
+                                     gsp_seqsim_plan** out);
+/* n_requests: number of requests; node_offset / data_offset: n + 1 entries
+ * each (either may be NULL). */
+int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,
+                               int64_t* node_offset, int64_t* data_offset);
+
 /* ---- LUSIM ---------------------------------------------------------------
  * gsp_lusim_factor replaces preprocess(rng, process, ::LUSIM, ...) for one
  * variable (lusim.jl:66-107): C22, C11, C12 from the covariance (utils.jl:50-62),

@@ -83,3 +83,54 @@ def config3(dims=(128, 128, 64), n_data=2000, seed=1003, precision=O.FP32_REFINE
     return O.SeqsimCase(dims=dims, family=O.EXPONENTIAL, sill=1.0, ranges=(40.0, 40.0, 8.0),
                         nugget=0.05, kmax=24, radius=40.0, data_coords=data, data_vals=vals,
                         path=path, precision=precision)
+
+
+def config5(n_req=512, side_lo=32, side_hi=512, seed=1005, precision=O.FP32_REFINE):
+    """Mixed batch of independent 2D requests: square grids with side
+    log-uniform in [side_lo, side_hi]; spherical/exponential/gaussian
+    variograms (sill 1) with range uniform in [5, 60] cells and nugget uniform
+    in [0, 0.1]; 0-500 uniform data with N(0,1) values; maxneighbors from
+    {8, 16, 32}; :range neighbourhood; random paths."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for q in range(n_req):
+        side = int(round(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi)))))
+        fam = int(rng.integers(0, 3))
+        rang = float(rng.uniform(5.0, 60.0))
+        nug = float(rng.uniform(0.0, 0.1))
+        nd = int(rng.integers(0, 501))
+        kmax = int(rng.choice([8, 16, 32]))
+        data = rng.uniform(0, side, (nd, 2))
+        vals = rng.standard_normal(nd)
+        path = rng.permutation(side * side)
+        cases.append(O.SeqsimCase(dims=(side, side), family=fam, sill=1.0, ranges=(rang,),
+                                  nugget=nug, kmax=kmax, radius=rang, data_coords=data,
+                                  data_vals=vals, path=path, precision=precision))
+    return cases
+
+
+def batch_arrays(cases, node_offset, z_list=None):
+    """Concatenated data values (and normals (R, N_total)) of a batch."""
+    vals = np.concatenate([c.data_vals for c in cases]) if cases else np.zeros(0)
+    if z_list is None:
+        return vals
+    R = z_list[0].shape[0]
+    z = np.empty((R, int(node_offset[-1])))
+    for q, zq in enumerate(z_list):
+        z[:, node_offset[q]:node_offset[q + 1]] = zq
+    return vals, z
+
+
+def local_plan(g, q, plan, cases):
+    """Request q's rows of an exported batch plan with ids made local
+    (grid ids 0..N-1, data N + d), columns cut to its maxneighbors."""
+    no, do = plan.node_offset, plan.data_offset
+    NT = int(no[-1])
+    N = int(no[q + 1] - no[q])
+    k = min(cases[q].kmax, N)
+    rows = slice(int(no[q]), int(no[q + 1]))
+    nbr = g["nbr"][rows].astype(np.int64)
+    loc = np.where(nbr < 0, -1, np.where(nbr < NT, nbr - no[q], nbr - NT - do[q] + N))
+    return dict(nfound=g["nfound"][rows], nbr=loc[:, :k], nbr_tail=loc[:, k:],
+                lam=g["lam"][rows][:, :k], lam_tail=g["lam"][rows][:, k:],
+                sigma=g["sigma"][rows], flags=g["flags"][rows], level=g["level"][rows])
