@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """SEQSIM throughput on one MI355X: nodes x realizations per second.
 
-Workload (default, "seqsim-3d-128x128x64"): a 128 x 128 x 64 unit grid
+Workload "seqsim-3d-128x128x64": a 128 x 128 x 64 unit grid
 (1,048,576 nodes), anisotropic exponential variogram sill 1, ranges
 (40, 40, 8), nugget 0.05, 2000 conditioning points uniform in the domain with
 N(0,1) values, maxneighbors 24 with the :range neighbourhood (radius 40), a
@@ -10,6 +10,19 @@ refinement step, realizations in FP64. One step is one full
 `rand(process, grid, 64; method=SEQSIM(...))`: plan (neighbour search +
 kriging solves) and the 64-realization sweep, with the path, the data and the
 64 x N standard normals already resident in HBM.
+
+Workload "seqsim-learned-256x256x128" (default): a 256 x 256 x 128 unit grid
+(8,388,608 nodes), exponential variogram sill 1, range 20, nugget 0.01,
+maxneighbors 24 with the :range neighbourhood, no conditioning data, a random
+path, 16 realizations. The kriging weights and variances come from the learned
+surrogate (BF16 MLP 73-256-256-256-25 on the matrix cores) wherever it applies
+and its variance check passes, from exact FP32 + refinement solves elsewhere.
+The surrogate is trained before timing on 2M kriging systems of exact plans of
+the same variogram (three 128 x 128 x 64 grids). One step is one learned plan
+plus the 16-realization sweep; the same steps with exact kriging are timed
+beside it, and the line reports the RMS weight error against the exact FP64
+plan and the paired experimental-variogram error (learned vs exact
+realizations from the same normals).
 
 Workload "seqsim-batch-512": 512 independent 2D requests in one plan and one
 sweep: square grids with side log-uniform in [32, 512] (about 23M nodes in
@@ -43,6 +56,11 @@ WORKLOADS = {
     "seqsim-2d-100x100": dict(dims=(100, 100), family=0, sill=1.0, ranges=(10.0, 10.0), nugget=0.0,
                               n_data=50, kmax=12, radius=30.0, R=10, seed=1001),
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
+    "seqsim-learned-256x256x128": dict(
+        dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,
+        kmax=24, radius=20.0, R=16, seed=1004, cpu_dims=(64, 64, 32),
+        learned=dict(train_dims=(128, 128, 64), train_plans=3, samples=2_000_000, epochs=20,
+                     batch=1024, lr=2e-3, tolerance=0.05)),
 }
 
 
@@ -182,14 +200,160 @@ def cpu_baseline(reqs, precision, R_batch=None, node_budget=None):
             "kind": "port", "sample": sample}
 
 
+MLP_FLOPS = 2 * (73 * 256 + 2 * 256 * 256 + 256 * 25)   # per node, one forward pass
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def upload_normals(ctx, _lib, R, N, seed):
+    z = _lib.DeviceBuffer(ctx, (R, N))
+    for r0 in range(R):  # row by row keeps host memory small
+        z_row = np.random.default_rng(seed + 3 + r0).standard_normal(N)
+        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, z.ptr + r0 * N * 8,
+                                                    z_row.ctypes.data, N * 8))
+    return z
+
+
+def timed(ctx, step, steps, warmup):
+    """W untimed steps, then K steps between device synchronisations."""
+    for _ in range(warmup):
+        step().close()
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.timings(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step().close()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    tim = ctx.timings(reset=True)
+    ctx.enable_timing(False)
+    return (t1 - t0) * 1e3 / steps, tim
+
+
+def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0):
+    """Average launch time per kernel family and its algorithmic bytes (flops for
+    the surrogate) per launch."""
+    sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
+    search_bytes = N * (8 + K * 4 + 4)                           # path, nbr, nfound
+    plan_bytes = N * (K * (4 + 8) + 4 + 8 + 1 + 4)               # nbr + nfound in, lambda, sigma, flags, degrees
+    kern = {}
+    for slot, name, nbytes in ((_lib.K_SEARCH, "search", search_bytes), (_lib.K_PLAN, "krige", plan_bytes),
+                               (_lib.K_LEARNED, "learned", None), (_lib.K_SWEEP, "sweep", sweep_bytes),
+                               (_lib.K_LEVELS, "levels", None), (_lib.K_TRANSPOSE, "transpose", 2 * 8 * N * R)):
+        ms, launches = tim[slot]
+        if launches == 0:
+            continue
+        avg = ms / launches
+        kern[name] = {"avg_ms": round(avg, 4), "launches_per_step": launches // steps}
+        if nbytes is not None:
+            per_launch = nbytes / (launches / steps)
+            kern[name]["algorithmic_bytes_per_launch"] = int(per_launch)
+            kern[name]["achieved_GBs"] = round(per_launch / (avg * 1e-3) / 1e9, 1)
+        if name == "learned":
+            kern[name]["algorithmic_flops_per_launch"] = int(MLP_FLOPS * n_mlp)
+            kern[name]["achieved_TFLOPs"] = round(MLP_FLOPS * n_mlp / (avg * 1e-3) / 1e12, 2)
+    return kern
+
+
+def roofline_of(kern, workload):
+    dominant = max(("search", "krige", "learned", "sweep"),
+                   key=lambda k: kern.get(k, {}).get("avg_ms", 0) * kern.get(k, {}).get("launches_per_step", 0))
+    d = kern[dominant]
+    if dominant == "learned":
+        return {"bound": "mfma", "kernel": dominant, "achieved": d["achieved_TFLOPs"],
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(d["achieved_TFLOPs"] / BF16_PEAK_TFLOPS, 4), "traffic": None}
+    return {"bound": "hbm", "kernel": dominant, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic(workload, dominant)}
+
+
+def model_variogram(spec, h):
+    u = np.asarray(h, float) / spec["ranges"][0]
+    shape = 1.0 - np.exp(-3.0 * u)
+    return (spec["sill"] - spec["nugget"]) * shape + spec["nugget"]
+
+
+def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, S, tol, z, N, R):
+    """RMS weight error of a learned plan against the exact FP64 plan, and the
+    experimental variograms of learned and exact realizations drawn from the
+    same normals (paired), along each axis at lags 1..20."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    lp = _lib.SeqsimPlan(ctx, prob_fp32r, surrogate=S, tolerance=tol)
+    ep = _lib.SeqsimPlan(ctx, prob_fp64)
+    gl = lp.export(fields=("nbr", "lam", "sigma", "flags"))
+    ge = ep.export(fields=("lam", "sigma"))
+    learned = (gl["flags"] & _lib.FLAG_LEARNED) != 0
+    valid = gl["nbr"] >= 0
+    d = gl["lam"] - ge["lam"]
+    rms_learned = float(np.sqrt(np.mean(d[learned][valid[learned]] ** 2)))
+    rms_all = float(np.sqrt(np.mean(d[valid] ** 2)))
+    sig = np.abs(gl["sigma"] - ge["sigma"])[learned] / np.maximum(ge["sigma"][learned], 1e-300)
+    del gl, ge, d, valid
+    Rv = min(R, 4)
+    dims = spec["dims"]
+    outs = []
+    for plan in (lp, ep):
+        out = _lib.DeviceBuffer(ctx, (R, N))
+        plan.sweep(z, None, out, device_pointers=True)
+        ctx.synchronize()
+        host = np.empty((Rv, N))
+        _lib.check(_lib.load().gsp_memcpy_to_host(ctx.handle, host.ctypes.data, out.ptr, Rv * N * 8))
+        outs.append(host.astype(np.float32))
+        out.free()
+    lags = list(range(1, 21))
+    paired, vs_model = [], []
+    for axis in range(len(dims)):
+        gl_ = O.axis_variogram(outs[0], dims, axis, lags)
+        ge_ = O.axis_variogram(outs[1], dims, axis, lags)
+        paired.append(float(np.linalg.norm(gl_ - ge_) / np.linalg.norm(ge_)))
+        gm = model_variogram(spec, lags)
+        vs_model.append(float(np.linalg.norm(gl_ - gm) / np.linalg.norm(gm)))
+    lp.close()
+    ep.close()
+    return {"rms_weight_error_learned_nodes": rms_learned, "rms_weight_error_all_nodes": rms_all,
+            "sigma_rel_error_median": float(np.median(sig)), "sigma_rel_error_p99": float(np.quantile(sig, 0.99)),
+            "learned_fraction": float(learned.mean()),
+            "variogram_error_paired_per_axis": [round(e, 5) for e in paired],
+            "variogram_error_vs_model_per_axis": [round(e, 5) for e in vs_model],
+            "variogram_sample": f"{Rv} realizations, lags 1..20 cells along each axis"}
+
+
+def train_surrogate(ctx, _lib, spec, precision):
+    """Exact plans of the same variogram on smaller grids (random paths), then
+    device training on up to `samples` of their kriging systems."""
+    L = spec["learned"]
+    plans, bufs = [], []
+    for t in range(L["train_plans"]):
+        tdims = L["train_dims"]
+        n = int(np.prod(tdims))
+        path = np.random.default_rng(2000 + t).permutation(n).astype(np.int64)
+        dp = _lib.DeviceBuffer.from_array(ctx, path)
+        bufs.append(dp)
+        plans.append(_lib.SeqsimPlan(ctx, problem(dict(spec, dims=tdims), dp.ptr, None, precision, n)))
+    S = _lib.Surrogate(ctx, plans, max_samples=L["samples"], epochs=L["epochs"], batch=L["batch"],
+                       lr=L["lr"], seed=L.get("seed", 0))
+    r = S.report
+    for p in plans:
+        p.close()
+    return S, {"systems": int(r.n_samples), "candidates": int(r.n_candidates), "steps": int(r.steps),
+               "epochs": L["epochs"], "batch": L["batch"], "ms": round(r.ms, 1),
+               "rms_weight_first_epoch": r.rms_weight_first_epoch,
+               "rms_weight_last_epoch": r.rms_weight_last_epoch,
+               "source": f"{L['train_plans']} exact plans of a {'x'.join(map(str, L['train_dims']))} grid, "
+                         "same variogram, random paths"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="seqsim-3d-128x128x64", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="seqsim-learned-256x256x128", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32r", choices=["fp32r", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the learned-vs-exact accuracy pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
@@ -228,28 +392,31 @@ def main():
     dpath = _lib.DeviceBuffer.from_array(ctx, path_all)
     ddata = _lib.DeviceBuffer.from_array(ctx, coords_all)
     dvals = _lib.DeviceBuffer.from_array(ctx, vals_all)
-    z = _lib.DeviceBuffer(ctx, (R, N))
-    for r0 in range(R):  # row by row keeps host memory small
-        z_row = np.random.default_rng(spec["seed"] + 3 + r0).standard_normal(N)
-        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, z.ptr + r0 * N * 8,
-                                                    z_row.ctypes.data, N * 8))
+    del path_all
+    z = upload_normals(ctx, _lib, R, N, spec["seed"])
     out = _lib.DeviceBuffer(ctx, (R, N))
-    probs, noff, coff = [], 0, 0
-    for r, n in zip(reqs, sizes):
-        probs.append(problem(r, dpath.ptr + 8 * noff, ddata.ptr + 8 * coff if r["n_data"] else None,
-                             precision, n))
-        noff += n
-        coff += r["data"].size
-    plan_arg = probs[0] if len(probs) == 1 else probs
+
+    def problems(prec):
+        probs, noff, coff = [], 0, 0
+        for r, n in zip(reqs, sizes):
+            probs.append(problem(r, dpath.ptr + 8 * noff, ddata.ptr + 8 * coff if r["n_data"] else None,
+                                 prec, n))
+            noff += n
+            coff += r["data"].size
+        return probs[0] if len(probs) == 1 else probs
+
+    plan_arg = problems(precision)
+    S, training, tol = None, None, None
+    if "learned" in spec:
+        S, training = train_surrogate(ctx, _lib, spec, precision)
+        tol = spec["learned"]["tolerance"]
 
     def step():
-        plan = _lib.SeqsimPlan(ctx, plan_arg)
+        plan = _lib.SeqsimPlan(ctx, plan_arg, surrogate=S, tolerance=tol if tol is not None else 0.05)
         plan.sweep(z, dvals, out, device_pointers=True)
         ctx.synchronize()
         return plan
 
-    for _ in range(args.warmup):
-        step().close()
     # realization-invariant statistics of the plan (for the byte counts)
     plan = step()
     ex = plan.export(fields=("nfound", "flags"))
@@ -257,55 +424,52 @@ def main():
     plan.close()
     keff = np.where(ex["flags"] & 3, 0, ex["nfound"]).astype(np.float64)
     kbar = float(keff.mean())
+    del ex
 
-    ctx.synchronize()
-    ctx.enable_timing(True)
-    ctx.timings(reset=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step().close()
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    tim = ctx.timings(reset=True)
-    ctx.enable_timing(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step, tim = timed(ctx, step, args.steps, args.warmup)
     value = N * R / (ms_per_step / 1e3)
-
-    # algorithmic bytes (realizations resident; neighbour gathers are re-reads)
-    sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
     K = info.maxneighbors                                        # row stride of nbr / lambda
-    search_bytes = N * (8 + K * 4 + 4)                           # path, nbr, nfound
-    plan_bytes = N * (K * (4 + 8) + 4 + 8 + 1 + 4)               # nbr + nfound in, lambda, sigma, flags, degrees
-    kern = {}
-    for slot, name, nbytes in ((_lib.K_SEARCH, "search", search_bytes), (_lib.K_PLAN, "krige", plan_bytes),
-                               (_lib.K_SWEEP, "sweep", sweep_bytes),
-                               (_lib.K_LEVELS, "levels", None), (_lib.K_TRANSPOSE, "transpose", 2 * 8 * N * R)):
-        ms, launches = tim[slot]
-        if launches == 0:
-            continue
-        avg = ms / launches
-        kern[name] = {"avg_ms": round(avg, 4), "launches_per_step": launches // args.steps}
-        if nbytes is not None:
-            per_launch = nbytes / (launches / args.steps)
-            kern[name]["algorithmic_bytes_per_launch"] = int(per_launch)
-            kern[name]["achieved_GBs"] = round(per_launch / (avg * 1e-3) / 1e9, 1)
-    dominant = max(("search", "krige", "sweep"), key=lambda k: kern.get(k, {}).get("avg_ms", 0) *
-                   kern.get(k, {}).get("launches_per_step", 0))
-    d = kern[dominant]
-    achieved = d["achieved_GBs"]
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(args.workload, dominant)}
+    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned)
+    roofline = roofline_of(kern, args.workload)
+
+    exact = accuracy = None
+    if S is not None:
+        def step_exact():
+            plan = _lib.SeqsimPlan(ctx, plan_arg)
+            plan.sweep(z, dvals, out, device_pointers=True)
+            ctx.synchronize()
+            return plan
+        ems, etim = timed(ctx, step_exact, args.steps, args.warmup)
+        exact = {"value": round(N * R / (ems / 1e3), 1), "ms_per_step": round(ems, 3),
+                 "kernels": kernel_table(_lib, etim, args.steps, N, R, K, kbar)}
+        if not args.no_accuracy:
+            accuracy = learned_accuracy(ctx, _lib, spec, plan_arg, problems(_lib.FP64), S, tol, z, N, R)
+            accuracy["tolerance"] = tol
 
     cpu = None
     if not args.no_cpu:
         if "batch" in spec:
             cpu = cpu_baseline(reqs, precision, R_batch=R, node_budget=1_500_000)
+        elif "learned" in spec:
+            sub = dict(reqs[0], dims=spec["cpu_dims"],
+                       path=np.random.default_rng(7).permutation(int(np.prod(spec["cpu_dims"]))))
+            cpu = cpu_baseline([sub], precision)
+            cpu["sample"] = f"a {'x'.join(map(str, spec['cpu_dims']))} grid of the same variogram: " + cpu["sample"]
         else:
             cpu = cpu_baseline(reqs, precision)
 
+    if "batch" in spec:
+        config = {"workload": args.workload, "requests": len(reqs), "nodes": N,
+                  "realizations_per_request": R, "side_range": list(spec["side"]),
+                  "n_data": int(sum(r["n_data"] for r in reqs)), "maxneighbors": [8, 16, 32]}
+    else:
+        config = {"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
+                  "maxneighbors": spec["kmax"], "radius": spec["radius"], "n_data": spec["n_data"]}
+    config |= {"search": "k nearest then mask", "precision": args.precision,
+               "depth": info.depth, "mean_neighbours": round(kbar, 2)}
     line = {
-        "metric": "SEQSIM nodes*realizations/s (exact kriging)",
+        "metric": ("SEQSIM nodes*realizations/s (learned kriging weights, exact fallback)"
+                   if S is not None else "SEQSIM nodes*realizations/s (exact kriging)"),
         "value": round(value, 1),
         "unit": "nodes*realizations/s",
         "n_gpus": 1,
@@ -317,18 +481,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (solves fp32+fp64 refinement)" if precision == _lib.FP32_REFINE else "f64",
         "data": "synthetic",
-        "config": ({"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
-                    "maxneighbors": spec["kmax"], "radius": spec["radius"], "n_data": spec["n_data"]}
-                   if "batch" not in spec else
-                   {"workload": args.workload, "requests": len(reqs), "nodes": N,
-                    "realizations_per_request": R, "side_range": list(spec["side"]),
-                    "n_data": int(sum(r["n_data"] for r in reqs)), "maxneighbors": [8, 16, 32]})
-        | {"search": "k nearest then mask", "precision": args.precision,
-           "depth": info.depth, "mean_neighbours": round(kbar, 2)},
+        "config": config,
         "roofline": roofline,
         "kernels": kern,
         "cpu_baseline": cpu,
     }
+    if S is not None:
+        line["dtype"] = "bf16 mlp (fp32 accumulate), f64 sweep; fallback solves fp32+fp64 refinement"
+        line["exact"] = exact
+        line["speedup_vs_exact"] = round(value / exact["value"], 3)
+        line["learned_nodes"] = int(info.n_learned)
+        line["training"] = training
+        line["accuracy"] = accuracy
     print(json.dumps(line), flush=True)
 
 

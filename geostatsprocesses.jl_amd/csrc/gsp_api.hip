@@ -3,7 +3,7 @@
 #include <cstdio>
 #include <new>
 
-#include "gsp_internal.h"
+#include "gsp_seqsim.h"
 #include "gsp_testing.h"
 
 namespace gsp {
@@ -222,7 +222,11 @@ int32_t gsp_selftest_copy(gsp_ctx* ctx, void* dst, const void* src, int64_t byte
 int32_t gsp_selftest_mfma(gsp_ctx* ctx, int32_t fp32, double* out) {
   return gsp_guard(ctx, [&] {
     GSP_REQUIRE(out, GSP_ERR_ARGUMENT, "null output");
-    gsp::mfma_probe(ctx, fp32, out);
+    GSP_REQUIRE(fp32 >= 0 && fp32 <= 2, GSP_ERR_ARGUMENT, "unknown probe");
+    if (fp32 == 2)
+      gsp::mfma_probe_bf16(ctx, out);
+    else
+      gsp::mfma_probe(ctx, fp32, out);
   });
 }
 
@@ -246,7 +250,7 @@ int32_t gsp_seqsim_plan_create_batch(gsp_ctx* ctx, int32_t n, const gsp_seqsim_p
   }
   int32_t rc = gsp_guard(ctx, [&] {
     plan->pool = ctx->pool;
-    gsp::seqsim_plan_build(ctx, n, problems, plan);
+    gsp::seqsim_plan_build(ctx, n, problems, plan, nullptr, 0.0);
   });
   if (rc != GSP_OK) {
     delete plan;
@@ -270,7 +274,72 @@ int32_t gsp_seqsim_plan_info(const gsp_seqsim_plan* plan, gsp_seqsim_info* info)
   info->alpha = plan->alpha;
   info->n_prior = plan->n_prior;
   info->n_fallback = plan->n_fallback;
+  info->n_learned = plan->n_learned;
   return GSP_OK;
+}
+
+int32_t gsp_seqsim_plan_create_learned(gsp_ctx* ctx, int32_t n, const gsp_seqsim_problem* problems,
+                                       const gsp_surrogate* surrogate, double tolerance,
+                                       gsp_seqsim_plan** out) {
+  if (!out || !surrogate) {
+    gsp::set_error("null surrogate or output pointer");
+    return GSP_ERR_ARGUMENT;
+  }
+  if (!(tolerance >= 0)) {
+    gsp::set_error("tolerance must be non-negative");
+    return GSP_ERR_ARGUMENT;
+  }
+  *out = nullptr;
+  gsp_seqsim_plan* plan = new (std::nothrow) gsp_seqsim_plan();
+  if (!plan) {
+    gsp::set_error("host allocation failed");
+    return GSP_ERR_MEMORY;
+  }
+  int32_t rc = gsp_guard(ctx, [&] {
+    plan->pool = ctx->pool;
+    gsp::seqsim_plan_build(ctx, n, problems, plan, surrogate, tolerance);
+  });
+  if (rc != GSP_OK) {
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return GSP_OK;
+}
+
+int32_t gsp_surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
+                            const gsp_train_options* options, gsp_surrogate** out,
+                            gsp_train_report* report) {
+  if (!out) {
+    gsp::set_error("null output pointer");
+    return GSP_ERR_ARGUMENT;
+  }
+  *out = nullptr;
+  gsp_surrogate* s = nullptr;
+  int32_t rc = gsp_guard(ctx, [&] {
+    s = gsp::surrogate_new(ctx);
+    gsp::surrogate_train(ctx, n_plans, plans, options, s, report);
+  });
+  if (rc != GSP_OK) {
+    if (s) gsp::surrogate_delete(s);
+    return rc;
+  }
+  *out = s;
+  return GSP_OK;
+}
+
+int32_t gsp_surrogate_free(gsp_surrogate* surrogate) {
+  if (surrogate) gsp::surrogate_delete(surrogate);
+  return GSP_OK;
+}
+
+int32_t gsp_surrogate_export(gsp_ctx* ctx, const gsp_surrogate* surrogate, float* params,
+                             int64_t* count, double* slots) {
+  if (!surrogate) {
+    gsp::set_error("null surrogate");
+    return GSP_ERR_ARGUMENT;
+  }
+  return gsp_guard(ctx, [&] { gsp::surrogate_export(ctx, surrogate, params, count, slots); });
 }
 
 int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,

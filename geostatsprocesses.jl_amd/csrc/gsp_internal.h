@@ -334,25 +334,6 @@ struct Offset {
 
 }  // namespace gsp
 
-struct gsp_seqsim_plan {
-  std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
-  int64_t N = 0, n_data = 0;  // nodes / data of all requests
-  int32_t n_req = 0;
-  int32_t ndim = 0, kmin = 0, kmax = 0, depth = 0;  // kmax: row stride of nbr / lam
-  double alpha = 0;  // scale factor of the first request
-  double mean = 0;
-  int64_t n_prior = 0, n_fallback = 0;
-  std::vector<int64_t> node_off, data_off;  // n_req + 1 entries each
-  gsp::DevBuf<int32_t> node_req;            // request of every node
-  gsp::DevBuf<double> req_mean;             // process mean of every request
-  gsp::DevBuf<int32_t> nfound, nbr, level;
-  gsp::DevBuf<double> lam, sigma;
-  gsp::DevBuf<uint8_t> flags;
-  gsp::DevBuf<int32_t> pos;     // path position of every node
-  gsp::DevBuf<int32_t> order;   // nodes grouped by level
-  std::vector<int64_t> level_off;  // level l holds order[level_off[l] .. level_off[l+1])
-};
-
 struct gsp_lusim_plan {
   std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
   int64_t N = 0, n_s = 0, n_d = 0;
@@ -366,12 +347,19 @@ struct gsp_lusim_plan {
 // kernels launched from gsp_seqsim.hip / gsp_lusim.hip
 namespace gsp {
 void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
-                       gsp_seqsim_plan* plan);
+                       gsp_seqsim_plan* plan, const gsp_surrogate* surrogate, double tolerance);
+void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
+                     const gsp_train_options* opt, gsp_surrogate* s, gsp_train_report* rep);
+gsp_surrogate* surrogate_new(gsp_ctx* ctx);
+void surrogate_delete(gsp_surrogate* s);
+void surrogate_export(gsp_ctx* ctx, const gsp_surrogate* s, float* params, int64_t* count,
+                      double* slots);
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out);
 void lusim_factor_build(gsp_ctx* ctx, const gsp_lusim_problem* p, gsp_lusim_plan* plan);
 void lusim_sample_run(gsp_ctx* ctx, const gsp_lusim_plan* plan, int32_t R, const double* w,
                       double* out);
 void mfma_probe(gsp_ctx* ctx, int fp32, double* host_out);
+void mfma_probe_bf16(gsp_ctx* ctx, double* host_out);
 void copy_kernel(gsp_ctx* ctx, void* dst, const void* src, int64_t bytes);
 }  // namespace gsp

@@ -1,0 +1,98 @@
+// gsp_seqsim.h — SEQSIM internals shared by the plan builder and the learned
+// kriging-weight surrogate (gfx950 only).
+#pragma once
+
+#include "gsp_internal.h"
+
+namespace gsp {
+
+// A conditioning datum in a hash-cell list: physical coordinates inline, `id`
+// its global data index.
+struct DataItem {
+  double x[3];
+  int32_t id;
+  int32_t pad_;
+};
+
+// One request of a plan, as the kernels see it.
+struct Req {
+  int32_t ndim, mode, kmin, kmax;
+  int32_t dims[3], T;    // grid size; entries of the offset table
+  int32_t hd[3], hexp;   // data-hash cells per axis; 1: expanded cell lists
+  int32_t hB, n_data;    // hash cell width in grid cells; conditioning data
+  int32_t node_off, data_off;
+  int32_t learned;       // 1: the plan's surrogate may give this request's weights
+  int32_t pad_;
+  int64_t table_off, cell_off;  // first entry of the offset table / of hstart
+  double r2, alpha, prior_sigma, mean;
+  double org[3], sp[3];  // physical grid
+  double ao[3], as[3];   // scaled grid (alpha*origin, alpha*spacing)
+  double hcs[3];         // data-hash cell size (physical)
+  DevModel M;
+};
+
+// Host-side description of one request, kept with the plan.
+struct ReqInfo {
+  gsp_model model;
+  int32_t ndim = 0, mode = 0, kmin = 0, kmax = 0, T = 0;
+  double spacing[3] = {1.0, 1.0, 1.0};
+  int16_t table24[24][3] = {};  // the first min(T, 24) offsets of the offset table
+};
+
+// Kernel-side view of a plan's per-node results (the outputs of the kriging
+// step and the inputs of the dependency levels).
+struct NodeOut {
+  int32_t NT, K;
+  const Req* reqs;
+  const int32_t* node_req;
+  const Offset* table;
+  const int32_t* nfound;
+  const int32_t* nbr;
+  const uint8_t* nbr_slot;  // learned plans: table index of each grid neighbour, 255 otherwise
+  double* lam;
+  double* sigma;
+  uint8_t* flags;
+  int32_t* counters;
+  int32_t* outdeg;
+  int32_t* indeg;
+};
+
+enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5, CTR_LEARNED = 6, CTR_EXACT = 7 };
+
+}  // namespace gsp
+
+struct gsp_seqsim_plan {
+  std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
+  int64_t N = 0, n_data = 0;  // nodes / data of all requests
+  int32_t n_req = 0;
+  int32_t ndim = 0, kmin = 0, kmax = 0, depth = 0;  // kmax: row stride of nbr / lam
+  double alpha = 0;  // scale factor of the first request
+  double mean = 0;
+  int64_t n_prior = 0, n_fallback = 0, n_learned = 0;
+  std::vector<int64_t> node_off, data_off;  // n_req + 1 entries each
+  std::vector<gsp::ReqInfo> info;           // per request
+  gsp::DevBuf<gsp::Req> reqs;               // per request, as the kernels see it
+  gsp::DevBuf<gsp::Offset> table;           // the requests' offset tables
+  gsp::DevBuf<int32_t> node_req;            // request of every node
+  gsp::DevBuf<double> req_mean;             // process mean of every request
+  gsp::DevBuf<int32_t> nfound, nbr, level;
+  gsp::DevBuf<double> lam, sigma;
+  gsp::DevBuf<uint8_t> flags;
+  gsp::DevBuf<int32_t> pos;     // path position of every node
+  gsp::DevBuf<int32_t> order;   // nodes grouped by level
+  std::vector<int64_t> level_off;  // level l holds order[level_off[l] .. level_off[l+1])
+};
+
+namespace gsp {
+
+// The learned path of the plan builder (gsp_surrogate.hip): runs the
+// surrogate on every node of the eligible requests, writes the accepted
+// nodes' weights, sigmas, flags and dependency degrees, and lists the other
+// nodes in `exact_list` (count in counters[CTR_EXACT]).
+void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, const NodeOut& out, double tolerance,
+                     int32_t* exact_list);
+// Whether request `r` matches the surrogate (family, nugget ratio, no
+// rotation, k nearest then mask, maxneighbors <= 24, the same 24 nearest
+// offsets in range units).
+bool surrogate_accepts(const gsp_surrogate* s, const ReqInfo& r);
+}  // namespace gsp

@@ -39,7 +39,7 @@
 
 #include <hipcub/hipcub.hpp>
 
-#include "gsp_internal.h"
+#include "gsp_seqsim.h"
 #include "gsp_wave.h"
 
 namespace gsp {
@@ -47,32 +47,7 @@ namespace {
 
 constexpr int kWaves = 4;  // wavefronts per 256-thread block
 
-enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5 };
-
 // ------------------------------------------------------------ plan kernels --
-// A conditioning datum in a hash-cell list: physical coordinates inline, `id`
-// its global data index.
-struct DataItem {
-  double x[3];
-  int32_t id;
-  int32_t pad_;
-};
-
-// One request of a plan.
-struct Req {
-  int32_t ndim, mode, kmin, kmax;
-  int32_t dims[3], T;    // grid size; entries of the offset table
-  int32_t hd[3], hexp;   // data-hash cells per axis; 1: expanded cell lists
-  int32_t hB, n_data;    // hash cell width in grid cells; conditioning data
-  int32_t node_off, data_off;
-  int64_t table_off, cell_off;  // first entry of the offset table / of hstart
-  double r2, alpha, prior_sigma, mean;
-  double org[3], sp[3];  // physical grid
-  double ao[3], as[3];   // scaled grid (alpha*origin, alpha*spacing)
-  double hcs[3];         // data-hash cell size (physical)
-  DevModel M;
-};
-
 struct PlanArgs {
   int32_t NT;  // nodes of all requests; data ids start here
   int32_t K;   // row stride of nbr / lam (largest kmax)
@@ -86,6 +61,7 @@ struct PlanArgs {
   const DataItem* hitems;
   int32_t* nfound;
   int32_t* nbr;
+  uint8_t* nbr_slot;  // optional: table index of each grid neighbour (255: datum or >= 255)
   double* lam;
   double* sigma;
   uint8_t* flags;
@@ -313,6 +289,8 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_didx[kWaves][128];
   __shared__ double s_dd2[kWaves][128];
   __shared__ int s_fidx[kWaves][64];
+  __shared__ uint8_t s_gt[kWaves][64];
+  __shared__ uint8_t s_ft[kWaves][64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int64_t wstride = (int64_t)gridDim.x * kWaves;
@@ -360,6 +338,7 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
       const int rank = cnt + mbcnt(b);
       if (ok && rank < kmax) {
         s_gidx[w][rank] = j;
+        s_gt[w][rank] = (uint8_t)(t < 255 ? t : 255);
         s_gd2[w][rank] = d2;
       }
       cnt += __popcll(b);
@@ -460,6 +439,7 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     const int nf = ng + nd < kmax ? ng + nd : kmax;
     int my_rank[3] = {128, 128, 128};
     int my_id[3] = {0, 0, 0};
+    uint8_t my_t[3] = {255, 255, 255};
     if (lane < ng) {
       const double d = s_gd2[w][lane];
       const int id = s_gidx[w][lane];
@@ -467,6 +447,7 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
       for (int m = 0; m < nd; ++m) r += key_less(s_dd2[w][m], s_didx[w][m], d, id);
       my_rank[0] = r;
       my_id[0] = id;
+      my_t[0] = s_gt[w][lane];
     }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -484,21 +465,30 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     wave_sync();
 #pragma unroll
     for (int h = 0; h < 3; ++h)
-      if (my_rank[h] < kmax) s_fidx[w][my_rank[h]] = my_id[h];
+      if (my_rank[h] < kmax) {
+        s_fidx[w][my_rank[h]] = my_id[h];
+        s_ft[w][my_rank[h]] = my_t[h];
+      }
     wave_sync();
 
     // ---- mask: drop points not yet simulated (k nearest first, then mask)
     const int fi = lane < nf ? s_fidx[w][lane] : -1;
+    const uint8_t ft = lane < nf ? s_ft[w][lane] : (uint8_t)255;
     const bool keep = lane < nf && (R.mode == GSP_MASKED_KNN || fi >= A.NT || A.pos[fi] < pi);
     const uint64_t kb = __ballot(keep);
     const int n = __popcll(kb);
     const int kr = mbcnt(kb);
     wave_sync();
-    if (keep) s_fidx[w][kr] = fi;
+    if (keep) {
+      s_fidx[w][kr] = fi;
+      s_ft[w][kr] = ft;
+    }
     wave_sync();
     const int nb = lane < n ? s_fidx[w][lane] : -1;
 
     if (lane < A.K) A.nbr[(int64_t)gi * A.K + lane] = nb;
+    if (A.nbr_slot && lane < A.K)
+      A.nbr_slot[(int64_t)gi * A.K + lane] = lane < n ? s_ft[w][lane] : (uint8_t)255;
     if (lane == 0) A.nfound[gi] = n;
   }
 }
@@ -811,15 +801,16 @@ __global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t
   }
 }
 
-// Nodes bucketed by neighbour count (counting sort), so packed wavefronts
-// hold systems of one size.
-__global__ __launch_bounds__(256) void k_n_hist(const int32_t* nfound, int32_t N, int32_t* hist) {
+// Nodes (ids[0..N), or 0..N-1 without ids) bucketed by neighbour count
+// (counting sort), so packed wavefronts hold systems of one size.
+__global__ __launch_bounds__(256) void k_n_hist(const int32_t* nfound, const int32_t* ids,
+                                                int32_t N, int32_t* hist) {
   __shared__ int32_t h[65];
   for (int t = threadIdx.x; t < 65; t += blockDim.x) h[t] = 0;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&h[nfound[i]], 1);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < N;
+       t += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[nfound[ids ? ids[t] : t]], 1);
   __syncthreads();
   for (int t = threadIdx.x; t < 65; t += blockDim.x)
     if (h[t]) atomicAdd(&hist[t], h[t]);
@@ -829,23 +820,25 @@ __global__ __launch_bounds__(256) void k_n_hist(const int32_t* nfound, int32_t N
 // with a single global atomic, then places its nodes.
 constexpr int kScatterChunk = 4096;
 
-__global__ __launch_bounds__(256) void k_n_scatter(const int32_t* nfound, int32_t N,
-                                                   int32_t* cursor, int32_t* list) {
+__global__ __launch_bounds__(256) void k_n_scatter(const int32_t* nfound, const int32_t* ids,
+                                                   int32_t N, int32_t* cursor, int32_t* list) {
   __shared__ int32_t h[65], base[65];
   const int64_t i0 = (int64_t)blockIdx.x * kScatterChunk;
   const int64_t i1 = i0 + kScatterChunk < N ? i0 + kScatterChunk : N;
   for (int t = threadIdx.x; t < 65; t += blockDim.x) h[t] = 0;
   __syncthreads();
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) atomicAdd(&h[nfound[i]], 1);
+  for (int64_t t = i0 + threadIdx.x; t < i1; t += blockDim.x)
+    atomicAdd(&h[nfound[ids ? ids[t] : t]], 1);
   __syncthreads();
   for (int t = threadIdx.x; t < 65; t += blockDim.x) {
     base[t] = h[t] ? atomicAdd(&cursor[t], h[t]) : 0;
     h[t] = 0;
   }
   __syncthreads();
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+  for (int64_t t = i0 + threadIdx.x; t < i1; t += blockDim.x) {
+    const int i = ids ? ids[t] : (int)t;
     const int k = nfound[i];
-    list[base[k] + atomicAdd(&h[k], 1)] = (int)i;
+    list[base[k] + atomicAdd(&h[k], 1)] = i;
   }
 }
 
@@ -1459,7 +1452,7 @@ void parallel_requests(int32_t n, F&& f) {
 }  // namespace
 
 void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
-                       gsp_seqsim_plan* plan) {
+                       gsp_seqsim_plan* plan, const gsp_surrogate* surrogate, double tolerance) {
   GSP_REQUIRE(problems, GSP_ERR_ARGUMENT, "null problem");
   GSP_REQUIRE(n_req >= 1 && n_req <= 65535, GSP_ERR_ARGUMENT,
               "number of requests must be 1..65535");
@@ -1586,6 +1579,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   std::vector<DataItem> hitems((size_t)std::max<int64_t>(n_items, 1));
   std::vector<double> data3((size_t)std::max<int64_t>(nDT * 3, 1), 0.0);
   std::vector<double> means((size_t)n_req);
+  std::vector<ReqInfo> info((size_t)n_req);
   {
     parallel_requests(n_req, [&](int32_t q) {
       const int64_t t_off = table_off[(size_t)q], c_off = cell_off[(size_t)q],
@@ -1622,6 +1616,20 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
         R.hcs[d] = h.hcs[d];
       }
       R.M = make_dev_model(p->model, h.nd, h.alpha);
+      ReqInfo& I = info[(size_t)q];
+      I.model = p->model;
+      I.ndim = h.nd;
+      I.mode = p->search_mode;
+      I.kmin = h.kmin;
+      I.kmax = h.kmax;
+      I.T = R.T;
+      for (int d = 0; d < 3; ++d) I.spacing[d] = d < h.nd ? g.spacing[d] : 1.0;
+      for (int t = 0; t < 24 && t < R.T; ++t) {
+        I.table24[t][0] = h.table[(size_t)t].dx;
+        I.table24[t][1] = h.table[(size_t)t].dy;
+        I.table24[t][2] = h.table[(size_t)t].dz;
+      }
+      R.learned = surrogate_accepts(surrogate, I) ? 1 : 0;
       means[(size_t)q] = p->mean;
       std::copy(h.table.begin(), h.table.end(), table.begin() + t_off);
       for (size_t c = 0; c < h.hstart.size(); ++c)
@@ -1649,6 +1657,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   plan->mean = problems[0].mean;
   plan->node_off = node_off;
   plan->data_off = data_off;
+  plan->info = std::move(info);
   plan->nfound.alloc((size_t)NT);
   plan->nbr.alloc((size_t)(NT * K));
   plan->lam.alloc((size_t)(NT * K));
@@ -1660,9 +1669,9 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   plan->node_req.alloc((size_t)NT);
   plan->req_mean.alloc((size_t)n_req);
   DevBuf<int32_t>& pos = plan->pos;
-  DevBuf<Req> dreqs;
+  DevBuf<Req>& dreqs = plan->reqs;
   dreqs.alloc((size_t)n_req);
-  DevBuf<Offset> dtable;
+  DevBuf<Offset>& dtable = plan->table;
   dtable.alloc(table.size());
   DevBuf<double> ddata;
   ddata.alloc(data3.size());
@@ -1715,6 +1724,9 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   A.hitems = dhitems.p;
   A.nfound = plan->nfound.p;
   A.nbr = plan->nbr.p;
+  DevBuf<uint8_t> nbr_slot;
+  if (surrogate) nbr_slot.alloc((size_t)(NT * K));
+  A.nbr_slot = surrogate ? nbr_slot.p : nullptr;
   A.lam = plan->lam.p;
   A.sigma = plan->sigma.p;
   A.flags = plan->flags.p;
@@ -1729,14 +1741,46 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   }
   tr.mark("search kernel");
   {
+    // learned plans: the surrogate takes the nodes it can, the rest are listed
+    // for the exact solver
+    DevBuf<int32_t> exact_ids;
+    int64_t n_exact = NT;
+    if (surrogate) {
+      exact_ids.alloc((size_t)NT);
+      NodeOut O;
+      O.NT = (int32_t)NT;
+      O.K = K;
+      O.reqs = dreqs.p;
+      O.node_req = plan->node_req.p;
+      O.table = dtable.p;
+      O.nfound = plan->nfound.p;
+      O.nbr = plan->nbr.p;
+      O.nbr_slot = nbr_slot.p;
+      O.lam = plan->lam.p;
+      O.sigma = plan->sigma.p;
+      O.flags = plan->flags.p;
+      O.counters = ctx->counters.p;
+      O.outdeg = outdeg.p;
+      O.indeg = indeg.p;
+      {
+        TimedLaunch tl(ctx, GSP_K_LEARNED);
+        surrogate_infer(ctx, surrogate, O, tolerance, exact_ids.p);
+      }
+      int32_t ne = 0;
+      GSP_HIP(hipMemcpyAsync(&ne, ctx->counters.p + CTR_EXACT, 4, hipMemcpyDeviceToHost, s));
+      GSP_HIP(hipStreamSynchronize(s));
+      n_exact = ne;
+      tr.mark("learned weights");
+    }
+    const int32_t* ids = surrogate ? exact_ids.p : nullptr;
     // bucket the nodes by neighbour count: n <= 16 run four per wavefront,
     // larger systems in the smallest register tile that holds them
     DevBuf<int32_t> nhist, list;
     nhist.alloc(65);
-    list.alloc((size_t)NT);
+    list.alloc((size_t)std::max<int64_t>(n_exact, 1));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, nhist.p, 0, (int64_t)65);
-    hipLaunchKernelGGL(k_n_hist, dim3(grid_for(NT, 256, 1024)), dim3(256), 0, s, plan->nfound.p,
-                       (int32_t)NT, nhist.p);
+    hipLaunchKernelGGL(k_n_hist, dim3(grid_for(n_exact, 256, 1024)), dim3(256), 0, s,
+                       plan->nfound.p, ids, (int32_t)n_exact, nhist.p);
     int32_t hh[65];
     GSP_HIP(hipMemcpyAsync(hh, nhist.p, sizeof(hh), hipMemcpyDeviceToHost, s));
     GSP_HIP(hipStreamSynchronize(s));
@@ -1749,8 +1793,10 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       upto[k] = acc;
     }
     GSP_HIP(hipMemcpyAsync(nhist.p, cur, sizeof(cur), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_n_scatter, dim3((unsigned)((NT + kScatterChunk - 1) / kScatterChunk)),
-                       dim3(256), 0, s, plan->nfound.p, (int32_t)NT, nhist.p, list.p);
+    if (n_exact > 0)
+      hipLaunchKernelGGL(k_n_scatter,
+                         dim3((unsigned)((n_exact + kScatterChunk - 1) / kScatterChunk)), dim3(256),
+                         0, s, plan->nfound.p, ids, (int32_t)n_exact, nhist.p, list.p);
     tr.mark("bucket by size");
     TimedLaunch tl(ctx, GSP_K_PLAN);
     const int64_t small = upto[16];
@@ -1836,6 +1882,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   flush_timings(ctx);
   plan->n_prior = ctr[CTR_PRIOR];
   plan->n_fallback = ctr[CTR_FALLBACK];
+  plan->n_learned = ctr[CTR_LEARNED];
 }
 
 // Realizations per sweep pass (GSP_SWEEP_CHUNK overrides). Smaller passes keep

@@ -22,7 +22,7 @@ VARIOGRAM, COVARIANCE = 0, 1
 KNN_THEN_MASK, MASKED_KNN = 0, 1
 FP64, FP32_REFINE = 0, 1
 LU_FP64, LU_MIXED = 0, 1
-FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK = 1, 2, 4
+FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK, FLAG_LEARNED = 1, 2, 4, 8
 
 
 class ArgumentError(ValueError):
@@ -76,10 +76,24 @@ class GspSeqsimInfo(ctypes.Structure):
                 ("minneighbors", ctypes.c_int32), ("maxneighbors", ctypes.c_int32),
                 ("depth", ctypes.c_int32), ("pad_", ctypes.c_int32),
                 ("alpha", ctypes.c_double), ("n_prior", ctypes.c_int64),
-                ("n_fallback", ctypes.c_int64)]
+                ("n_fallback", ctypes.c_int64), ("n_learned", ctypes.c_int64)]
 
 
-K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_COUNT = 0, 1, 2, 3, 4, 5, 6, 8
+K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_LEARNED, K_COUNT = \
+    0, 1, 2, 3, 4, 5, 6, 7, 8
+
+
+class GspTrainOptions(ctypes.Structure):
+    _fields_ = [("max_samples", ctypes.c_int64), ("epochs", ctypes.c_int32),
+                ("batch", ctypes.c_int32), ("lr", ctypes.c_double), ("var_weight", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+class GspTrainReport(ctypes.Structure):
+    _fields_ = [("n_samples", ctypes.c_int64), ("n_candidates", ctypes.c_int64),
+                ("steps", ctypes.c_int64), ("rms_weight_first_epoch", ctypes.c_double),
+                ("rms_weight_last_epoch", ctypes.c_double),
+                ("rms_logvar_last_epoch", ctypes.c_double), ("ms", ctypes.c_double)]
 
 
 class GspTimings(ctypes.Structure):
@@ -109,6 +123,12 @@ SIGNATURES = {
     "gsp_seqsim_plan_create_batch": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
                                             ctypes.POINTER(_VP)]),
     "gsp_seqsim_plan_layout": (_I32, [_VP, ctypes.POINTER(_I32), _VP, _VP]),
+    "gsp_surrogate_train": (_I32, [_VP, _I32, _VP, ctypes.POINTER(GspTrainOptions),
+                                   ctypes.POINTER(_VP), ctypes.POINTER(GspTrainReport)]),
+    "gsp_surrogate_free": (_I32, [_VP]),
+    "gsp_surrogate_export": (_I32, [_VP, _VP, _VP, ctypes.POINTER(_I64), _VP]),
+    "gsp_seqsim_plan_create_learned": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem), _VP,
+                                              ctypes.c_double, ctypes.POINTER(_VP)]),
     "gsp_lusim_factor": (_I32, [_VP, ctypes.POINTER(GspLusimProblem), ctypes.POINTER(_VP)]),
     "gsp_lusim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
     "gsp_lusim_dims": (_I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
@@ -215,9 +235,12 @@ class DeviceBuffer:
             pass
 
 
-def selftest_mfma(ctx, fp32: bool):
-    out = np.zeros((2, 64, 4))
-    check(load().gsp_selftest_mfma(ctx.handle, int(bool(fp32)), out.ctypes.data))
+def selftest_mfma(ctx, fp32):
+    """fp32: False/True for the FP64/FP32 16x16x4 probe, "bf16" for the
+    16x16x32 BF16 probe (three setups)."""
+    mode = 2 if fp32 == "bf16" else int(bool(fp32))
+    out = np.zeros((3 if mode == 2 else 2, 64, 4))
+    check(load().gsp_selftest_mfma(ctx.handle, mode, out.ctypes.data))
     return out
 
 
@@ -284,11 +307,18 @@ class SeqsimPlan:
     `problem` is one GspSeqsimProblem, or a sequence of them for a batch of
     independent requests sharing one plan (gsp_seqsim_plan_create_batch)."""
 
-    def __init__(self, ctx: Context, problem, keep=()):
+    def __init__(self, ctx: Context, problem, keep=(), surrogate=None, tolerance=0.05):
         self.ctx = ctx
         self._keep = keep
         self._h = ctypes.c_void_p()
-        if isinstance(problem, GspSeqsimProblem):
+        if surrogate is not None:
+            probs = [problem] if isinstance(problem, GspSeqsimProblem) else list(problem)
+            arr = (GspSeqsimProblem * len(probs))(*probs)
+            self._keep = (keep, arr, surrogate)
+            check(load().gsp_seqsim_plan_create_learned(ctx.handle, len(probs), arr,
+                                                        surrogate.handle, float(tolerance),
+                                                        ctypes.byref(self._h)))
+        elif isinstance(problem, GspSeqsimProblem):
             check(load().gsp_seqsim_plan_create(ctx.handle, ctypes.byref(problem),
                                                 ctypes.byref(self._h)))
         else:
@@ -353,6 +383,47 @@ class SeqsimPlan:
     def close(self):
         if self._h and self._h.value:
             load().gsp_seqsim_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Surrogate:
+    """gsp_surrogate: the learned kriging-weight MLP, trained on the systems of
+    exact plans (gsp_surrogate_train)."""
+
+    def __init__(self, ctx: Context, plans, max_samples=0, epochs=20, batch=1024, lr=2e-3,
+                 var_weight=0.01, seed=0):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        opts = GspTrainOptions(max_samples=int(max_samples), epochs=int(epochs), batch=int(batch),
+                               lr=float(lr), var_weight=float(var_weight), seed=int(seed))
+        handles = (ctypes.c_void_p * len(plans))(*[p._h.value for p in plans])
+        self.report = GspTrainReport()
+        check(load().gsp_surrogate_train(ctx.handle, len(plans), handles, ctypes.byref(opts),
+                                         ctypes.byref(self._h), ctypes.byref(self.report)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def export(self):
+        """(params FP32 flat array, slot offsets in ranges (24, 3))."""
+        n = ctypes.c_int64()
+        check(load().gsp_surrogate_export(self.ctx.handle, self._h, None, ctypes.byref(n), None))
+        params = np.zeros(n.value, np.float32)
+        slots = np.zeros((24, 3))
+        check(load().gsp_surrogate_export(self.ctx.handle, self._h, params.ctypes.data,
+                                          ctypes.byref(n), slots.ctypes.data))
+        return params, slots
+
+    def close(self):
+        if self._h and self._h.value:
+            load().gsp_surrogate_free(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

@@ -64,7 +64,8 @@ typedef enum { GSP_LU_FP64 = 0, GSP_LU_MIXED = 1 } gsp_lu_precision;
 enum {
   GSP_FLAG_FEW = 1,          /* n < minneighbors: prior draw (seqsim.jl:109-111) */
   GSP_FLAG_FAIL = 2,         /* kriging Cholesky failed: prior (seqsim.jl:127-131) */
-  GSP_FLAG_FP64_FALLBACK = 4 /* FP32 factor failed, FP64 solve used */
+  GSP_FLAG_FP64_FALLBACK = 4, /* FP32 factor failed, FP64 solve used */
+  GSP_FLAG_LEARNED = 8        /* weights and sigma from the learned surrogate */
 };
 
 /* A GeoStatsFunctions variogram or covariance (un-vendored, 0.15):
@@ -136,6 +137,7 @@ typedef struct {
   double alpha;                 /* scaling factor of seqsim.jl:162-166 */
   int64_t n_prior;              /* nodes drawn from the prior */
   int64_t n_fallback;           /* nodes whose FP32 factor fell back to FP64 */
+  int64_t n_learned;            /* nodes whose weights came from the surrogate */
 } gsp_seqsim_info;
 
 /* Per-kernel device time, measured with HIP events recorded on the context
@@ -148,6 +150,7 @@ enum {
   GSP_K_LU_FACTOR = 4,   /* LUSIM assembly + factorisations (whole call) */
   GSP_K_LU_SAMPLE = 5,   /* LUSIM d2 + L22 W and scatter (whole call) */
   GSP_K_SEARCH = 6,      /* neighbour search */
+  GSP_K_LEARNED = 7,     /* surrogate inference (learned plans) */
   GSP_K_COUNT = 8
 };
 typedef struct {
@@ -155,9 +158,30 @@ typedef struct {
   int64_t launches[GSP_K_COUNT];
 } gsp_timings;
 
+/* Training of the learned kriging-weight surrogate. */
+typedef struct {
+  int64_t max_samples;          /* kriging systems drawn from the plans (0: all usable) */
+  int32_t epochs;
+  int32_t batch;                /* samples per step, a multiple of 64 */
+  double lr;                    /* peak Adam step (one-cycle: warm-up, cosine decay) */
+  double var_weight;            /* weight of the log-variance term in the loss */
+  uint64_t seed;
+} gsp_train_options;
+
+typedef struct {
+  int64_t n_samples;            /* systems trained on */
+  int64_t n_candidates;         /* usable systems in the plans */
+  int64_t steps;
+  double rms_weight_first_epoch;/* RMS weight error over the samples of an epoch */
+  double rms_weight_last_epoch;
+  double rms_logvar_last_epoch;
+  double ms;                    /* wall time of the training loop */
+} gsp_train_report;
+
 typedef struct gsp_ctx gsp_ctx;
 typedef struct gsp_seqsim_plan gsp_seqsim_plan;
 typedef struct gsp_lusim_plan gsp_lusim_plan;
+typedef struct gsp_surrogate gsp_surrogate;
 
 /* ---- context -------------------------------------------------------------
  * One per (process, device): owns a HIP stream and reusable device workspace.
@@ -244,6 +268,41 @@ int32_t gsp_seqsim_plan_create_batch(gsp_ctx* ctx, int32_t n,
  * each (either may be NULL). */
 int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,
                                int64_t* node_offset, int64_t* data_offset);
+
+/* ---- learned kriging weights ----------------------------------------------
+ * No counterpart in the reference: a surrogate for the fit / predictprob
+ * steps of seqsim.jl:124-131. A BF16 multilayer perceptron 73 -> 256 -> 256 ->
+ * 256 -> 25 (GELU) maps a node's neighbour offsets (divided by the ranges, in
+ * slots given by the rank of the offset in the search's sorted offset table)
+ * to its 24 kriging weights and log kriging variance.
+ *
+ * gsp_surrogate_train draws training systems from exact plans (every node of
+ * a request with the first request's variogram family, nugget/sill ratio and
+ * 24 nearest offsets in range units, solved exactly, with only grid neighbours
+ * among those 24 offsets) and trains on the device. */
+int32_t gsp_surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
+                            const gsp_train_options* options, gsp_surrogate** out,
+                            gsp_train_report* report);
+int32_t gsp_surrogate_free(gsp_surrogate* surrogate);
+/* The surrogate's FP32 master parameters, layer by layer: W1 [256][128]
+ * (inputs 73..127 zero), b1 [256], W2 [256][256], b2, W3 [256][256], b3,
+ * W4 [64][256] (outputs 25..63 zero), b4 [64]; and its 24 slot offsets in
+ * ranges (slots[24][3]). Inference uses the weights rounded to BF16. Either
+ * pointer may be NULL; *count receives the number of parameters. */
+int32_t gsp_surrogate_export(gsp_ctx* ctx, const gsp_surrogate* surrogate, float* params,
+                             int64_t* count, double* slots);
+/* gsp_seqsim_plan_create_learned: gsp_seqsim_plan_create_batch whose nodes
+ * take their weights and sigma from the surrogate where it applies (requests
+ * matching its variogram and offsets, k nearest then mask, maxneighbors <= 24;
+ * nodes whose neighbours all lie in the 24 table offsets). A node is solved
+ * exactly instead when the variance implied by the predicted weights,
+ * sill - sum_j lambda_j C(x_j, x_0), differs from the predicted variance by
+ * more than `tolerance` (relative). Learned nodes carry GSP_FLAG_LEARNED;
+ * gsp_seqsim_plan_info counts them in n_learned. */
+int32_t gsp_seqsim_plan_create_learned(gsp_ctx* ctx, int32_t n,
+                                       const gsp_seqsim_problem* problems,
+                                       const gsp_surrogate* surrogate, double tolerance,
+                                       gsp_seqsim_plan** out);
 
 /* ---- LUSIM ---------------------------------------------------------------
  * gsp_lusim_factor replaces preprocess(rng, process, ::LUSIM, ...) for one

@@ -10,6 +10,10 @@ library never loads it.
   src/utils.jl:50-62 (LAPACK potrf/trsm through scipy).
 * NearestInit / ExplicitInit: src/initialization/nearest.jl:12-34 and
   src/initialization/explicit.jl:19-46.
+* Learned kriging weights (no counterpart in the reference): a numpy
+  restatement of the surrogate's forward pass (BF16 rounding of weights and
+  activations, FP32 accumulation, tanh-form GELU) and the experimental
+  variogram used to compare learned and exact realizations.
 """
 from __future__ import annotations
 
@@ -25,7 +29,7 @@ _LIB = None
 SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
 KNN_THEN_MASK, MASKED_KNN = 0, 1
 FP64, FP32_REFINE = 0, 1
-FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK = 1, 2, 4
+FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK, FLAG_LEARNED = 1, 2, 4, 8
 
 
 class OrModel(ctypes.Structure):
@@ -376,3 +380,117 @@ def lusim_sample(d2, L22, dinds, sinds, z1, mean, w):
     else:
         out += mean
     return out
+
+
+# ------------------------------------------------------ learned kriging weights
+MLP_IN, MLP_HID, MLP_OUT = 128, 256, 64
+MLP_FEAT_SCALE, MLP_DATA_FEATURE = 8.0, 4.0
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest, ties to even), as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def mlp_unpack(params):
+    """Flat FP32 parameters -> [(W, b)] per layer (W: out x in)."""
+    p = np.asarray(params, dtype=np.float32)
+    shapes = [(MLP_HID, MLP_IN), (MLP_HID, MLP_HID), (MLP_HID, MLP_HID), (MLP_OUT, MLP_HID)]
+    out, off = [], 0
+    for o, i in shapes:
+        W = p[off:off + o * i].reshape(o, i)
+        off += o * i
+        b = p[off:off + o]
+        off += o
+        out.append((W, b))
+    assert off == len(p)
+    return out
+
+
+def mlp_features(slots, masks):
+    """Inputs of the surrogate for slot masks (bit s: neighbour in slot s)."""
+    masks = np.asarray(masks, dtype=np.uint32)
+    X = np.zeros((len(masks), MLP_IN), np.float32)
+    occ = ((masks[:, None] >> np.arange(24, dtype=np.uint32)[None, :]) & 1).astype(np.float32)
+    X[:, :72] = (occ[:, :, None] * (np.asarray(slots) * MLP_FEAT_SCALE)[None, :, :].astype(np.float32)
+                 ).reshape(len(masks), 72)
+    X[:, 72] = MLP_DATA_FEATURE
+    return X
+
+
+def gelu_tanh(z):
+    z = z.astype(np.float32)
+    u = np.float32(0.7978845608028654) * (z + np.float32(0.044715) * z * z * z)
+    return z / (np.float32(1.0) + np.exp(np.float32(-2.0) * u))
+
+
+def mlp_forward(params, slots, masks):
+    """Outputs (n, 25): 24 slot weights and log(sigma^2 / sill)."""
+    h = bf16(mlp_features(slots, masks))
+    layers = mlp_unpack(params)
+    for li, (W, b) in enumerate(layers):
+        z = h @ bf16(W).T + b
+        h = bf16(gelu_tanh(z)) if li < 3 else z
+    return h[:, :25]
+
+
+def grid_slots(case: SeqsimCase, k=24):
+    """The first k grid offsets (cells) of the search's offset table: squared
+    distance, then linear index delta. The zero offset (the node itself, a
+    candidate of the k-nearest-then-mask search that the mask always drops)
+    comes first."""
+    nd = case.ndim
+    nx = case.dims[0]
+    ny = case.dims[1] if nd > 1 else 1
+    r = 3
+    rng = [range(-r, r + 1) if d < nd else range(0, 1) for d in range(3)]
+    offs = [(dx, dy, dz) for dz in rng[2] for dy in rng[1] for dx in rng[0]]
+    sp = list(case.spacing) + [1.0] * (3 - nd)
+    offs.sort(key=lambda o: (sum((o[d] * sp[d]) ** 2 for d in range(nd)), o[0] + nx * (o[1] + ny * o[2])))
+    return np.array(offs[:k])
+
+
+def node_slot_masks(case: SeqsimCase, plan: Plan, nodes, slot_cells):
+    """Slot mask and per-neighbour slot of each node (-1 mask when a neighbour
+    is a datum or outside the slot offsets)."""
+    nd = case.ndim
+    nx = case.dims[0]
+    ny = case.dims[1] if nd > 1 else 1
+    key = {tuple(o): s for s, o in enumerate(slot_cells)}
+    N = case.n_nodes
+    masks = np.zeros(len(nodes), np.int64)
+    slots = np.full((len(nodes), plan.kmax), -1, np.int64)
+    for t, i in enumerate(nodes):
+        ci = (i % nx, (i // nx) % ny, i // (nx * ny))
+        m = 0
+        for j in range(plan.nfound[i]):
+            a = int(plan.nbr[i, j])
+            if a >= N:
+                m = -1
+                break
+            c = (a % nx - ci[0], (a // nx) % ny - ci[1], a // (nx * ny) - ci[2])
+            s = key.get(c)
+            if s is None:
+                m = -1
+                break
+            slots[t, j] = s
+            m |= 1 << s
+        masks[t] = m
+    return masks, slots
+
+
+def axis_variogram(fields, dims, axis, lags):
+    """Experimental semivariogram of realizations (R, N) along one grid axis,
+    averaged over realizations: gamma(h) = mean (x(i + h e) - x(i))^2 / 2."""
+    R = fields.shape[0]
+    shape = (R,) + tuple(reversed(dims))          # (R, nz, ny, nx) for column-major ids
+    X = fields.reshape(shape)
+    ax = len(dims) - axis                          # numpy axis of grid axis `axis`
+    Xm = np.moveaxis(X, ax, -1)
+    out = []
+    for h in lags:
+        d = Xm[..., h:] - Xm[..., :-h]
+        out.append(0.5 * float(np.mean(d * d, dtype=np.float64)))
+    return np.array(out)
