@@ -59,6 +59,13 @@ struct NodeOut {
 
 enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5, CTR_LEARNED = 6, CTR_EXACT = 7 };
 
+// A table the kernels only read (request descriptors), viewed through the
+// constant address space so that wave-uniform reads become scalar loads.
+template <class T>
+__device__ inline const __attribute__((address_space(4))) T* const_view(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
 }  // namespace gsp
 
 struct gsp_seqsim_plan {

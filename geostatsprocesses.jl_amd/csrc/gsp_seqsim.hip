@@ -71,7 +71,8 @@ struct PlanArgs {
 };
 
 // Squared physical distance between a datum and the centre of node t.
-__device__ inline double data_d2(const Req& R, const DataItem& it, const int* t) {
+template <class RQ>
+__device__ inline double data_d2(const RQ& R, const DataItem& it, const int* t) {
   double acc = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -297,14 +298,14 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
 
   int cur = -1;
   Offset e0 = {0.0, 0, 0, 0, 0};
-  Req R;  // the current request (a register copy: the stores below cannot alias it)
   for (int64_t q = (int64_t)blockIdx.x * kWaves + w; q < A.NT; q += wstride) {
     const int64_t pv = A.path[q];
     const int gi = __builtin_amdgcn_readfirstlane((int)(pv & 0xFFFFFFFF));
     const int p = __builtin_amdgcn_readfirstlane((int)(pv >> 32));
+    // the request, read through the scalar cache (the stores below cannot alias it)
+    const auto& R = const_view(A.reqs)[p];
     if (p != cur) {  // the first 64 offsets serve every node of the request
       cur = p;
-      R = A.reqs[p];
       e0 = Offset{0.0, 0, 0, 0, 0};
       if (lane < R.T) e0 = A.table[R.table_off + lane];
     }
@@ -932,56 +933,65 @@ __global__ void k_dep_fill(int32_t N, int32_t K, const int32_t* nfound, const ui
   }
 }
 
-__global__ void k_frontier0(int32_t N, const int32_t* indeg, int32_t* order, int32_t* cnt,
-                            int32_t* level) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (indeg[i] == 0) {
-      order[atomicAdd(&cnt[0], 1)] = (int)i;
-      level[i] = 0;
-    }
+// Level L + 1 in two launches. Push: every node of frontier L (order[off[L]
+// .. off[L] + cnt[L])) decrements the counters of its dependents; the atomics'
+// results are not used, so they pipeline. Collect: every node not yet placed
+// whose counter reached zero gets level L + 1 and joins frontier L + 1 (one
+// atomic per wavefront; ids ascend inside each wavefront's run). L = -1
+// collects level 0 (nodes without grid neighbours).
+__global__ __launch_bounds__(256) void k_level_push(int32_t L, const int32_t* rowptr,
+                                                    const int32_t* deps, int32_t* indeg,
+                                                    const int32_t* order, const int32_t* off,
+                                                    const int32_t* cnt) {
+  const int32_t start = off[L], size = cnt[L];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < size;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int a = order[start + t];
+    const int e1 = rowptr[a + 1];
+    for (int e = rowptr[a]; e < e1; ++e) atomicSub(&indeg[deps[e]], 1);
   }
 }
 
-// Frontier L occupies order[off[L] .. off[L] + cnt[L]); both are final when
-// this launch starts. Dependents whose last unfinished neighbour is in the
-// frontier join frontier L + 1, appended after it: each block gathers its
-// appends in an LDS queue and reserves space with one global atomic.
-constexpr int kQueue = 2048;
-
-__global__ __launch_bounds__(256) void k_frontier_step(int32_t L, const int32_t* rowptr,
-                                                       const int32_t* deps, int32_t* indeg,
-                                                       int32_t* order, int32_t* off, int32_t* cnt,
-                                                       int32_t* level) {
-  __shared__ int32_t q[kQueue];
-  __shared__ int32_t qn, base;
-  const int32_t start = off[L], size = cnt[L], next = start + size;
+// Each block takes a contiguous chunk of node ids: it counts its ready nodes,
+// reserves their slots with one atomic, then writes them in id order (same-
+// address atomics serialise, so one per block rather than one per wavefront).
+__global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, const int32_t* indeg,
+                                                       int32_t* level, int32_t* order,
+                                                       int32_t* off, int32_t* cnt) {
+  __shared__ int32_t s_red[4];
+  __shared__ int32_t s_base;
+  const int32_t next = L < 0 ? 0 : off[L] + cnt[L];
   if (blockIdx.x == 0 && threadIdx.x == 0) off[L + 1] = next;
-  for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x; t0 < size;
-       t0 += (int64_t)gridDim.x * blockDim.x) {
-    if (threadIdx.x == 0) qn = 0;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t chunk = ((N + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk;
+  const int64_t i1 = i0 + chunk < N ? i0 + chunk : N;
+  auto ready = [&](int64_t i) { return i < i1 && level[i] < 0 && indeg[i] == 0; };
+  int c = 0;
+  for (int64_t i = i0 + tid; i < i1; i += 256) c += ready(i);
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0) s_red[w] = c;
+  __syncthreads();
+  if (tid == 0) {
+    const int total = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    s_base = total ? atomicAdd(&cnt[L + 1], total) : 0;
+  }
+  __syncthreads();
+  int running = s_base;
+  for (int64_t b0 = i0; b0 < i1; b0 += 256) {
+    const int64_t i = b0 + tid;
+    const bool r = ready(i);
+    const uint64_t b = __ballot(r);
+    __syncthreads();  // s_red of the previous step has been read
+    if (lane == 0) s_red[w] = __popcll(b);
     __syncthreads();
-    const int64_t t = t0 + threadIdx.x;
-    if (t < size) {
-      const int i = order[start + t];
-      for (int e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-        const int d = deps[e];
-        if (atomicSub(&indeg[d], 1) == 1) {
-          level[d] = L + 1;
-          const int slot = atomicAdd(&qn, 1);
-          if (slot < kQueue)
-            q[slot] = d;
-          else
-            order[next + atomicAdd(&cnt[L + 1], 1)] = d;
-        }
-      }
+    int before = 0;
+    for (int v = 0; v < w; ++v) before += s_red[v];
+    if (r) {
+      level[i] = L + 1;
+      order[next + running + before + mbcnt(b)] = (int32_t)i;
     }
-    __syncthreads();
-    const int n = qn < kQueue ? qn : kQueue;
-    if (threadIdx.x == 0) base = n ? atomicAdd(&cnt[L + 1], n) : 0;
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) order[next + base + j] = q[j];
-    __syncthreads();
+    running += s_red[0] + s_red[1] + s_red[2] + s_red[3];
   }
 }
 
@@ -1023,8 +1033,16 @@ struct LevelArgs {
 // All nodes of one level are independent: x_i = (mean + sum_j lambda_ij
 // (x_nbr(j) - mean)) + sigma_i z_i for every realization, 1 << shift lanes per
 // node (lane = realization), neighbours from earlier levels.
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);
+// block b works on chunk (b % 8) * per + b / 8 of the level, so every XCD
+// sweeps one contiguous run of the level's (id-ordered) nodes and the
+// neighbour rows those nodes share stay in one L2. The grid is 8 * per blocks.
+constexpr int kXcds = 8;
+
 __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int per = gridDim.x / kXcds;
+  const int64_t blk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  const int64_t t = blk * blockDim.x + threadIdx.x;
   const int64_t slot = t >> A.shift;
   const int r = (int)(t & ((1 << A.shift) - 1));
   if (slot >= A.count || r >= A.RC) return;
@@ -1033,8 +1051,23 @@ __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
   const int k = (A.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : A.nfound[i];
   const int32_t* nb = A.nbr + (int64_t)i * A.K;
   const double* lm = A.lam + (int64_t)i * A.K;
+  // eight neighbours at a time: indices and weights, then their eight row
+  // gathers in flight together, then the accumulation in neighbour order
   double acc = 0.0;
-  for (int j = 0; j < k; ++j) acc = acc + lm[j] * (A.X[(int64_t)nb[j] * A.RC + r] - mean);
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    int nj[8];
+    double lj[8], xj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      nj[u] = j0 + u < k ? nb[j0 + u] : 0;
+      lj[u] = j0 + u < k ? lm[j0 + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xj[u] = j0 + u < k ? A.X[(int64_t)nj[u] * A.RC + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + u < k) acc = acc + lj[u] * (xj[u] - mean);
+  }
   const double mu = mean + acc;
   A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * A.Zt[(int64_t)A.pos[i] * A.RC + r];
 }
@@ -1694,7 +1727,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   hipLaunchKernelGGL(k_batch_path, dim3(grid_for(NT / n_req + 1, 256, 1024), n_req), dim3(256), 0,
                      s, path.p, dnode_off.p, plan->node_req.p, ctx->counters.p);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, pos.p, -1, NT);
-  hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, plan->level.p, 0, NT);
+  hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, plan->level.p, -1, NT);
   hipLaunchKernelGGL(k_positions, dim3(gn), dim3(256), 0, s, path.p, NT, pos.p, ctx->counters.p);
   GSP_HIP(hipGetLastError());
   int32_t pos_err = 0;
@@ -1841,19 +1874,23 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, cursor.p, 0, NT);
     hipLaunchKernelGGL(k_dep_fill, dim3(nb), dim3(256), 0, s, (int32_t)NT, K, plan->nfound.p,
                        plan->flags.p, plan->nbr.p, rowptr.p, cursor.p, deps.p);
-    hipLaunchKernelGGL(k_frontier0, dim3(nb), dim3(256), 0, s, (int32_t)NT, indeg.p,
-                       plan->order.p, cnt.p, plan->level.p);
+    const int cb = grid_for(NT, 2048, 1024);
+    hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, s, (int32_t)NT, -1, indeg.p,
+                       plan->level.p, plan->order.p, off.p, cnt.p);
     tr.mark("levels: fill");
-    // one launch per level; the frontier sizes are checked every kBatch levels
+    // two launches per level; the frontier sizes are checked every kBatch levels
     constexpr int kBatch = 16;
     const int fb = grid_for(NT / 16, 256, 1024);
     int32_t depth = -1;
     std::vector<int32_t> hc(kBatch + 1);
     for (int32_t L = 0; depth < 0; L += kBatch) {
       GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
-      for (int b = 0; b < kBatch; ++b)
-        hipLaunchKernelGGL(k_frontier_step, dim3(fb), dim3(256), 0, s, L + b, rowptr.p, deps.p,
-                           indeg.p, plan->order.p, off.p, cnt.p, plan->level.p);
+      for (int b = 0; b < kBatch; ++b) {
+        hipLaunchKernelGGL(k_level_push, dim3(fb), dim3(256), 0, s, L + b, rowptr.p, deps.p,
+                           indeg.p, plan->order.p, off.p, cnt.p);
+        hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, s, (int32_t)NT, L + b, indeg.p,
+                           plan->level.p, plan->order.p, off.p, cnt.p);
+      }
       GSP_HIP(hipMemcpyAsync(hc.data(), cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, s));
       GSP_HIP(hipStreamSynchronize(s));
       for (int b = 0; b <= kBatch; ++b)
@@ -1945,8 +1982,8 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
         A.order = plan->order.p + off;
         A.count = (int32_t)cnt;
         const int64_t threads = cnt << shift;
-        hipLaunchKernelGGL(k_sweep_level, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
-                           A);
+        const int64_t per = ((threads + 255) / 256 + kXcds - 1) / kXcds;
+        hipLaunchKernelGGL(k_sweep_level, dim3((unsigned)(per * kXcds)), dim3(256), 0, s, A);
       }
     }
     {
