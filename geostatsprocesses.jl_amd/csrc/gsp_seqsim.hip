@@ -1129,6 +1129,7 @@ int grid_for(int64_t work, int per_block, int cap) {
 // ball when there is one; otherwise far enough that every node finds kmax
 // nodes (k nearest first), or the whole grid (masked search).
 std::vector<Offset> build_offsets(const gsp_grid& g, double r2, int kmax, int mode) {
+  if (mode == GSP_DATA_ONLY) return {};  // no grid candidates
   const int nd = g.ndim;
   int64_t ext[3] = {0, 0, 0};
   for (int d = 0; d < nd; ++d) ext[d] = g.dims[d] - 1;
@@ -1397,7 +1398,8 @@ void validate_problem(const gsp_seqsim_problem* p, HostReq& h) {
   GSP_REQUIRE(p->n_data == 0 || p->data_coords, GSP_ERR_ARGUMENT, "null data coordinates");
   GSP_REQUIRE(p->n_path == h.N && p->path, GSP_ERR_UNSUPPORTED,
               "the path must visit every grid node exactly once");
-  GSP_REQUIRE(p->search_mode == GSP_KNN_THEN_MASK || p->search_mode == GSP_MASKED_KNN,
+  GSP_REQUIRE(p->search_mode == GSP_KNN_THEN_MASK || p->search_mode == GSP_MASKED_KNN ||
+                  p->search_mode == GSP_DATA_ONLY,
               GSP_ERR_ARGUMENT, "unknown search mode");
   GSP_REQUIRE(p->precision == GSP_FP64 || p->precision == GSP_FP32_REFINE, GSP_ERR_ARGUMENT,
               "unknown precision");

@@ -92,8 +92,11 @@ __device__ inline uint32_t pack2(float lo, float hi) {
 // the same function in training and inference.
 constexpr float kGeluC1 = 1.5957691216057308f;   // 2 sqrt(2/pi)
 constexpr float kGeluC2 = 0.0713548162726009f;   // 2 sqrt(2/pi) 0.044715
+constexpr float kLog2e = 1.4426950408889634f;
 __device__ inline float gelu_sig(float z) {
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-z * (kGeluC1 + kGeluC2 * (z * z))));
+  // exp(-s) as 2^(-s log2 e), log2 e folded into the constants (v_exp_f32)
+  const float t = z * ((-kGeluC1 * kLog2e) + (-kGeluC2 * kLog2e) * (z * z));
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
 }
 __device__ inline float gelu(float z) { return z * gelu_sig(z); }
 __device__ inline float gelu_grad(float z) {
@@ -454,10 +457,11 @@ struct InferArgs {
 
 constexpr int kInferWaves = 4;
 
-// Weight fragments of one output tile, staged in LDS for the block's four
-// wavefronts (two buffers: tile rt + 1 is copied while tile rt is used).
+// Weight fragments of output tiles, staged in LDS for the block's four
+// wavefronts: three buffers, so that the copy of tile rt + 2 runs while tile
+// rt is multiplied and tile rt - 1 goes through its activation.
 struct FragLds {
-  u32x4 w[2][8][64];
+  u32x4 w[3][8][64];
 };
 
 // Asynchronous copy of the fragments of tile `tile` into buffer `buf`: one
@@ -476,47 +480,61 @@ __device__ __forceinline__ void copy_tile(FragLds& F, int buf, const uint16_t* w
 // One layer of the fused inference: out^T = act(W in^T + b) for the wave's
 // 32 nodes (2 column tiles); in/out hold the packed bf16 pairs of the
 // operand fragments, word q of step s, column tile ct at [s][ct][q]. Every
-// wavefront of the block runs it (block barriers inside); the copy of tile
-// rt + 1 into LDS runs while tile rt is computed.
+// wavefront of the block runs it (block barriers inside). Step rt issues the
+// products of tile rt, then the activation of tile rt - 1 (the matrix cores
+// and the vector ALU overlap), while tile rt + 2 is copied into LDS.
 template <int STEPS, int TILES, bool ACT>
 __device__ __forceinline__ void mlp_layer(FragLds& F, const uint32_t (&in)[8][2][4],
                                           uint32_t (&out)[8][2][4], float (&last)[2][2][4],
                                           const uint16_t* wp, const float* bias, int lane, int w) {
   const int g = lane >> 4;
   copy_tile<STEPS>(F, 0, wp, 0, lane, w);
+  if constexpr (TILES > 1) copy_tile<STEPS>(F, 1, wp, 1, lane, w);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  static_for<0, TILES>([&](auto rtc) {
+  f32x4 acc[2][2];
+  static_for<0, TILES + 1>([&](auto rtc) {
     constexpr int rt = decltype(rtc)::value;
-    constexpr int cur = rt & 1;
-    if constexpr (rt + 1 < TILES) copy_tile<STEPS>(F, cur ^ 1, wp, rt + 1, lane, w);
-    f32x4 acc[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
-    static_for<0, STEPS>([&](auto sc) {
-      constexpr int s = decltype(sc)::value;
-      const u32x4 a = F.w[cur][s][lane];
-      acc[0] = mfma(a, u32x4{in[s][0][0], in[s][0][1], in[s][0][2], in[s][0][3]}, acc[0]);
-      acc[1] = mfma(a, u32x4{in[s][1][0], in[s][1][1], in[s][1][2], in[s][1][3]}, acc[1]);
-    });
-    const f32x4 b = *(const f32x4*)(bias + 16 * rt + 4 * g);
-    if constexpr (ACT) {
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        uint32_t w0 = pack2(gelu(acc[ct][0] + b[0]), gelu(acc[ct][1] + b[1]));
-        uint32_t w1 = pack2(gelu(acc[ct][2] + b[2]), gelu(acc[ct][3] + b[3]));
-        // materialise the packed words here: the activation math is pure and
-        // would otherwise be sunk to the next layer, keeping FP32 values live
-        asm volatile("" : "+v"(w0), "+v"(w1));
-        out[rt >> 1][ct][(rt & 1) * 2] = w0;
-        out[rt >> 1][ct][(rt & 1) * 2 + 1] = w1;
-      }
-    } else {
-#pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) last[rt][ct][v] = acc[ct][v] + b[v];
+    if constexpr (rt + 2 < TILES) copy_tile<STEPS>(F, (rt + 2) % 3, wp, rt + 2, lane, w);
+    if constexpr (rt < TILES) {
+      // the accumulators start at the bias of their output rows
+      const f32x4 b = *(const f32x4*)(bias + 16 * rt + 4 * g);
+      acc[rt & 1][0] = b;
+      acc[rt & 1][1] = b;
+      static_for<0, STEPS>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const u32x4 a = F.w[rt % 3][s][lane];
+        acc[rt & 1][0] = mfma(a, u32x4{in[s][0][0], in[s][0][1], in[s][0][2], in[s][0][3]},
+                              acc[rt & 1][0]);
+        acc[rt & 1][1] = mfma(a, u32x4{in[s][1][0], in[s][1][1], in[s][1][2], in[s][1][3]},
+                              acc[rt & 1][1]);
+      });
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    if constexpr (rt >= 1) {
+      constexpr int pt = rt - 1;
+      if constexpr (ACT) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const f32x4 z = acc[pt & 1][ct];
+          uint32_t w0 = pack2(gelu(z[0]), gelu(z[1]));
+          uint32_t w1 = pack2(gelu(z[2]), gelu(z[3]));
+          // materialise the packed words here: the activation math is pure
+          // and would otherwise be sunk to the next layer
+          asm volatile("" : "+v"(w0), "+v"(w1));
+          out[pt >> 1][ct][(pt & 1) * 2] = w0;
+          out[pt >> 1][ct][(pt & 1) * 2 + 1] = w1;
+        }
+      } else {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) last[pt][ct][v] = acc[pt & 1][ct][v];
+      }
+    }
+    if constexpr (rt < TILES) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
     __builtin_amdgcn_sched_barrier(0);
   });
 }

@@ -5,11 +5,14 @@ simulation by SEQSIM and LUSIM, behind the reference's
 The arithmetic runs in libgsp_hip.so (HIP, gfx950) through its C ABI
 (include/gsp.h); this package mirrors the reference's Julia interface so the
 same calls and the same plugin contract (preprocess / randsingle) apply.
+The Gaussian posterior mean (src/expectation/field/gaussian.jl:21-25) runs the
+same neighbour search and kriging kernels over the data only.
 """
 from . import _lib
 from ._lib import (ArgumentError, Context, GspError, PosDefException, UnsupportedError,
                    default_context)
 from .ensembles import Ensemble
+from .expectation import KrigingPlan, expectedvalue, mean, posteriormean, priormean
 from .functions import (ExponentialCovariance, ExponentialVariogram, GaussianCovariance,
                         GaussianVariogram, SphericalCovariance, SphericalVariogram)
 from .geometry import CartesianGrid, GeoTable, PointSet, georef
@@ -20,6 +23,7 @@ from .simulation import (FFTSIM, LUSIM, SEQSIM, MetricBall, defaultsimulation, p
 
 KNN_THEN_MASK = _lib.KNN_THEN_MASK
 MASKED_KNN = _lib.MASKED_KNN
+DATA_ONLY = _lib.DATA_ONLY
 FP64 = _lib.FP64
 FP32_REFINE = _lib.FP32_REFINE
 LU_FP64 = _lib.LU_FP64
@@ -32,5 +36,6 @@ __all__ = [
     "CartesianGrid", "GeoTable", "PointSet", "georef", "ExplicitInit", "NearestInit",
     "initialize", "FixedPath", "GaussianProcess", "LinearPath", "RandomPath", "SourcePath",
     "FFTSIM", "LUSIM", "SEQSIM", "MetricBall", "defaultsimulation", "preprocess", "rand",
-    "randsingle", "KNN_THEN_MASK", "MASKED_KNN", "FP64", "FP32_REFINE", "LU_FP64", "LU_MIXED",
+    "randsingle", "KrigingPlan", "expectedvalue", "mean", "posteriormean", "priormean",
+    "KNN_THEN_MASK", "MASKED_KNN", "DATA_ONLY", "FP64", "FP32_REFINE", "LU_FP64", "LU_MIXED",
 ]

@@ -19,7 +19,7 @@ GSP_OK, GSP_ERR_ARGUMENT, GSP_ERR_ASSERTION, GSP_ERR_POSDEF = 0, 1, 2, 3
 GSP_ERR_DEVICE, GSP_ERR_MEMORY, GSP_ERR_UNSUPPORTED, GSP_ERR_INTERNAL = 4, 5, 6, 7
 SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
 VARIOGRAM, COVARIANCE = 0, 1
-KNN_THEN_MASK, MASKED_KNN = 0, 1
+KNN_THEN_MASK, MASKED_KNN, DATA_ONLY = 0, 1, 2
 FP64, FP32_REFINE = 0, 1
 LU_FP64, LU_MIXED = 0, 1
 FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK, FLAG_LEARNED = 1, 2, 4, 8

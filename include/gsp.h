@@ -49,8 +49,13 @@ typedef enum { GSP_VARIOGRAM = 0, GSP_COVARIANCE = 1 } gsp_kind;
  * (called at seqsim.jl:107): the k nearest of all augmented points, then drop
  * those outside the ball or not yet simulated (GSP_KNN_THEN_MASK, default), or
  * the k nearest among the simulated points in the ball (GSP_MASKED_KNN). Ties
- * in distance go to the lower augmented index. */
-typedef enum { GSP_KNN_THEN_MASK = 0, GSP_MASKED_KNN = 1 } gsp_search_mode;
+ * in distance go to the lower augmented index. GSP_DATA_ONLY takes the k
+ * nearest conditioning data in the ball and no grid nodes: every node is then
+ * an independent simple-kriging estimate from the data, the posterior mean of
+ * expectation/field/gaussian.jl:21-25 (GeoStatsModels.fitpredict with
+ * Kriging(f, mean)); a sweep of one realization with z = 0 returns that mean
+ * and the plan's sigma^2 is the kriging variance. */
+typedef enum { GSP_KNN_THEN_MASK = 0, GSP_MASKED_KNN = 1, GSP_DATA_ONLY = 2 } gsp_search_mode;
 
 /* Precision of the local kriging solves: FP64 Cholesky, or FP32 Cholesky with
  * one FP64-residual refinement step (FP64 retry if the FP32 factor fails). */

@@ -232,7 +232,10 @@ static void or_topk_push(or_topk* t, double d2, int32_t idx) {
  *     `mask[inds]`). This is the default.
  *   OR_MASKED_KNN: the k nearest among points that are simulated and in the ball.
  * Which one Meshes 0.57 uses is unpinned; both are kept. A grid node j is
- * simulated before target i iff pos[j] < pos[i]; data points always are. */
+ * simulated before target i iff pos[j] < pos[i]; data points always are.
+ *   OR_DATA_ONLY: the k nearest data points in the ball, no grid nodes (simple
+ *     kriging from the data: the posterior mean of
+ *     expectation/field/gaussian.jl:21-25). */
 static int or_eligible(int64_t N, const int64_t* pos, int64_t target, int64_t a) {
   return a >= N || pos[a] < pos[target];
 }
@@ -260,6 +263,7 @@ int32_t or_search_brute(const or_seqsim_problem* p, int32_t nmax, const int64_t*
   for (int64_t a = 0; a < N + p->n_data; ++a) {
     double d2;
     if (a < N) {
+      if (p->search_mode == OR_DATA_ONLY) continue;
       or_unravel(g, a, ji);
       d2 = or_d2_grid(g, ji, ti);
     } else {
@@ -372,7 +376,8 @@ int32_t or_search_bucket(const or_seqsim_problem* p, void* vb, int32_t nmax,
           ji[0] = x; ji[1] = y; ji[2] = z;
           int64_t id = x + (nd >= 2 ? g->dims[0] * (y + (nd >= 3 ? g->dims[1] * z : 0)) : 0);
           double d2 = or_d2_grid(g, ji, ti);
-          if (d2 <= r2 && (!masked || or_eligible(N, pos, target, id)))
+          if (d2 <= r2 && p->search_mode != OR_DATA_ONLY &&
+              (!masked || or_eligible(N, pos, target, id)))
             or_topk_push(&t, d2, (int32_t)id);
           for (int64_t q = b->cell_start[id]; q < b->cell_start[id + 1]; ++q) {
             int32_t di = b->cell_data[q];

@@ -27,7 +27,7 @@ extern "C" {
 #endif
 
 enum { OR_SPHERICAL = 0, OR_EXPONENTIAL = 1, OR_GAUSSIAN = 2 };
-enum { OR_KNN_THEN_MASK = 0, OR_MASKED_KNN = 1 };
+enum { OR_KNN_THEN_MASK = 0, OR_MASKED_KNN = 1, OR_DATA_ONLY = 2 };
 enum { OR_FP64 = 0, OR_FP32_REFINE = 1 };
 enum { OR_FLAG_FEW = 1, OR_FLAG_FAIL = 2, OR_FLAG_FP64_FALLBACK = 4 };
 
@@ -62,7 +62,7 @@ typedef struct {
   int32_t minneighbors;
   int32_t maxneighbors;
   double radius;             /* > 0: MetricBall radius; <= 0: plain k-nearest */
-  int32_t search_mode;       /* OR_KNN_THEN_MASK | OR_MASKED_KNN */
+  int32_t search_mode;       /* OR_KNN_THEN_MASK | OR_MASKED_KNN | OR_DATA_ONLY */
   int32_t precision;         /* OR_FP64 | OR_FP32_REFINE */
   int32_t device_pointers;   /* ignored by the oracle */
   int32_t pad_;

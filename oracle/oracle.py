@@ -27,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
-KNN_THEN_MASK, MASKED_KNN = 0, 1
+KNN_THEN_MASK, MASKED_KNN, DATA_ONLY = 0, 1, 2
 FP64, FP32_REFINE = 0, 1
 FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK, FLAG_LEARNED = 1, 2, 4, 8
 
@@ -291,6 +291,40 @@ def seqsim_sweep(case: SeqsimCase, plan: Plan, z: np.ndarray) -> np.ndarray:
                           _p(z, ctypes.c_double), _p(case.data_vals, ctypes.c_double),
                           _p(out, ctypes.c_double))
     return out
+
+
+def simple_kriging_np(case: SeqsimCase, kmax=None):
+    """Simple kriging of the data over the grid centroids, restated in numpy
+    independently of the C oracle: GeoStatsModels 0.13 `fitpredict(Kriging(f,
+    mu), data, domain)` (the posterior mean of expectation/field/gaussian.jl:
+    21-25; GeoStatsModels is not vendored, its published algorithm is: the k
+    nearest data of each centroid, within the search ball when there is one,
+    then lambda = C^-1 c, mean = mu + lambda . (z - mu), variance = C(0) -
+    lambda . c). Neighbours are ordered by (distance, data index). Nodes with
+    fewer than kmin data are NaN. Returns (mean, variance, neighbour ids)."""
+    kmax = case.kmax if kmax is None else kmax
+    X = grid_centroids(case.dims, case.origin, case.spacing)
+    D = case.data_coords
+    k = min(kmax, len(D))
+    r2 = case.radius ** 2 if case.radius > 0 else np.inf
+    mean = np.full(len(X), np.nan)
+    var = np.full(len(X), np.nan)
+    ids = np.full((len(X), k), -1, np.int64)
+    for i, x in enumerate(X):
+        d2 = np.sum((D - x) ** 2, axis=1)
+        order = np.lexsort((np.arange(len(D)), d2))[:k]
+        order = order[d2[order] <= r2]
+        if len(order) < max(case.kmin, 1):
+            continue
+        ids[i, :len(order)] = order
+        xd = D[order]
+        C = covariance_np(case.family, case.sill, case.nugget, case.ranges, xd, xd, case.rotation)
+        c = covariance_np(case.family, case.sill, case.nugget, case.ranges, xd, x[None, :],
+                          case.rotation)[:, 0]
+        lam = np.linalg.solve(C, c)
+        mean[i] = case.mean + lam @ (case.data_vals[order] - case.mean)
+        var[i] = case.sill - lam @ c
+    return mean, var, ids
 
 
 def seqsim_reference_cost(case: SeqsimCase, z: np.ndarray, threads=1, n_nodes_limit=0):

@@ -86,6 +86,8 @@ def numpy_search(case, target, pos):
     _, k = O.clamp_neighbors(N, case.kmin, case.kmax)
     order = np.lexsort((idx, d2))
     order = order[inball[order]]
+    if case.mode == O.DATA_ONLY:
+        return order[order >= N][:k].astype(np.int32)
     if case.mode == O.KNN_THEN_MASK:
         top = order[:k]
         return top[elig[top]].astype(np.int32)
@@ -102,12 +104,14 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
+@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN, O.DATA_ONLY])
 @pytest.mark.parametrize("ci", range(len(CASES)))
 def test_search_brute_and_bucket_match_numpy(ci, mode):
     spec = dict(CASES[ci])
     rng = np.random.default_rng(ci)
     nd = spec.pop("nd")
+    if mode == O.DATA_ONLY and nd == 0:
+        nd = 7
     dims = spec["dims"]
     N = int(np.prod(dims))
     sp = np.asarray(spec.get("spacing", (1.0,) * len(dims)))
@@ -333,3 +337,62 @@ def test_nearest_init_reference_fixtures():
     dest = O.nearest_init(cents, d2[:, :2])
     for i, j in {5076: 3, 2526: 1, 7551: 2}.items():
         assert dest[j - 1] == i - 1
+
+
+# -------------------------------------------------------- posterior mean ----
+def test_posterior_mean_reference_case():
+    """test/field.jl:163-170: the simple kriging posterior mean reproduces the
+    data at their cells (isapprox with Julia's default tolerance)."""
+    case = O.SeqsimCase(dims=(100, 100), origin=(0.5, 0.5), family=O.SPHERICAL, ranges=(35.0,),
+                        data_coords=np.array([[25.0, 25.0], [50.0, 75.0], [75.0, 50.0]]),
+                        data_vals=np.array([1.0, 0.0, 1.0]), kmax=10, mode=O.DATA_ONLY)
+    plan = O.seqsim_plan(case)
+    m = O.seqsim_sweep(case, plan, np.zeros((1, case.n_nodes)))[0]
+    for (i, j), v in {(24, 24): 1.0, (49, 74): 0.0, (74, 49): 1.0}.items():
+        assert m[i + 100 * j] == pytest.approx(v, rel=np.sqrt(np.finfo(float).eps), abs=0.0)
+    ref, var, _ = O.simple_kriging_np(case)
+    np.testing.assert_allclose(m, ref, rtol=1e-10, atol=1e-12)
+    assert np.all(plan.level == 0)
+
+
+def test_simple_kriging_np_single_datum_closed_form():
+    """One datum: lambda = C(h) / C(0), mean = mu + lambda (z - mu),
+    variance = C(0) - C(h)^2 / C(0)."""
+    x0 = np.array([[3.3, 4.1]])
+    case = O.SeqsimCase(dims=(8, 6), family=O.EXPONENTIAL, sill=2.0, nugget=0.3, ranges=(5.0,),
+                        mean=0.4, data_coords=x0, data_vals=np.array([1.7]), kmax=1,
+                        mode=O.DATA_ONLY)
+    m, v, ids = O.simple_kriging_np(case)
+    X = O.grid_centroids(case.dims)
+    h = np.linalg.norm(X - x0, axis=1)
+    ch = (2.0 - 0.3) * np.exp(-3.0 * h / 5.0)
+    np.testing.assert_allclose(m, 0.4 + ch / 2.0 * (1.7 - 0.4), rtol=1e-13)
+    np.testing.assert_allclose(v, 2.0 - ch ** 2 / 2.0, rtol=1e-13)
+    plan = O.seqsim_plan(case)
+    mo = O.seqsim_sweep(case, plan, np.zeros((1, case.n_nodes)))[0]
+    np.testing.assert_allclose(mo, m, rtol=1e-12)
+    np.testing.assert_allclose(plan.sigma ** 2, v, rtol=1e-12)
+
+
+@pytest.mark.parametrize("fam,kmax,radius", [(O.SPHERICAL, 10, 0.0), (O.GAUSSIAN, 6, 0.0),
+                                             (O.EXPONENTIAL, 12, 4.0)])
+def test_posterior_mean_oracle_matches_numpy(fam, kmax, radius):
+    rng = np.random.default_rng(kmax)
+    dims = (14, 11, 5)
+    data = rng.uniform(-0.05, 1.05, (40, 3)) * np.asarray(dims)
+    case = O.SeqsimCase(dims=dims, family=fam, sill=1.5, nugget=0.05, ranges=(6.0, 5.0, 3.0),
+                        mean=0.2, data_coords=data, data_vals=rng.standard_normal(40),
+                        kmin=2, kmax=kmax, radius=radius, mode=O.DATA_ONLY)
+    plan = O.seqsim_plan(case)
+    ref, var, ids = O.simple_kriging_np(case)
+    N = case.n_nodes
+    few = np.isnan(ref)
+    assert radius == 0.0 or few.any()
+    np.testing.assert_array_equal((plan.flags & 1) != 0, few)
+    for i in np.flatnonzero(~few):
+        n = plan.nfound[i]
+        np.testing.assert_array_equal(plan.nbr[i, :n] - N, ids[i, :n])
+    m = O.seqsim_sweep(case, plan, np.zeros((1, N)))[0]
+    np.testing.assert_allclose(m[~few], ref[~few], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(plan.sigma[~few] ** 2, var[~few], rtol=1e-9, atol=1e-12)
+    assert np.all(plan.level == 0)
