@@ -32,6 +32,13 @@ maxneighbors from {8, 16, 32}, :range neighbourhood, random paths, 8
 realizations each, FP32 solves with refinement. One step is one batched plan
 plus the batched sweep.
 
+Workload "kriging-mean-256x256x128": the Gaussian posterior mean (simple
+kriging, `mean(process, grid; data)`) on a 256 x 256 x 128 unit grid
+(8,388,608 nodes) from 100,000 data uniform in the domain, exponential
+variogram sill 1, range 20, nugget 0.01, the 10 nearest data of every node
+(GeoStatsModels' fitpredict defaults), FP64 solves. One step is one
+data-only plan plus its zero-normal sweep (the mean at every node).
+
 Prints one JSON line. `cpu_baseline` times the CPU restatement of the
 reference's cost structure (search + solve per node per realization) on a
 bounded sample of the same workload.
@@ -61,6 +68,10 @@ WORKLOADS = {
         kmax=24, radius=20.0, R=16, seed=1004, cpu_dims=(64, 64, 32),
         learned=dict(train_dims=(128, 128, 64), train_plans=3, samples=2_000_000, epochs=20,
                      batch=1024, lr=2e-3, tolerance=0.05)),
+    "kriging-mean-256x256x128": dict(
+        dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01,
+        n_data=100_000, kmax=10, radius=0.0, R=1, seed=1006, mode="data_only", precision="fp64",
+        cpu_nodes=1_500_000),
 }
 
 
@@ -90,7 +101,9 @@ def make_inputs(spec):
     dims = spec["dims"]
     N = int(np.prod(dims))
     rng = np.random.default_rng(spec["seed"])
-    path = rng.permutation(N).astype(np.int64)
+    # the posterior mean visits the nodes in order (its result does not depend on it)
+    path = np.arange(N, dtype=np.int64) if spec.get("mode") == "data_only" \
+        else rng.permutation(N).astype(np.int64)
     data = np.random.default_rng(spec["seed"] + 1).uniform(0, 1, (spec["n_data"], len(dims))) \
         * np.asarray(dims, dtype=np.float64)
     vals = np.random.default_rng(spec["seed"] + 2).standard_normal(spec["n_data"])
@@ -120,7 +133,7 @@ def problem(spec, path_ptr, data_ptr, precision, N):
     p.minneighbors = 1
     p.maxneighbors = spec["kmax"]
     p.radius = spec["radius"]
-    p.search_mode = _lib.KNN_THEN_MASK
+    p.search_mode = _lib.DATA_ONLY if spec.get("mode") == "data_only" else _lib.KNN_THEN_MASK
     p.precision = precision
     p.device_pointers = 1
     return p
@@ -157,7 +170,7 @@ def profiled_traffic(workload, kernel):
     return int(total / launches)
 
 
-def cpu_baseline(reqs, precision, R_batch=None, node_budget=None):
+def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
     """The oracle's reference-cost restatement (KD-free cell-hash search +
     Cholesky per node per realization), OpenMP over realizations.
 
@@ -176,19 +189,23 @@ def cpu_baseline(reqs, precision, R_batch=None, node_budget=None):
         case = O.SeqsimCase(dims=dims, family=spec["family"], sill=spec["sill"],
                             ranges=spec["ranges"][:len(dims)], nugget=spec["nugget"],
                             kmax=spec["kmax"], radius=spec["radius"], data_coords=spec["data"],
-                            data_vals=spec["vals"], path=spec["path"], precision=precision)
+                            data_vals=spec["vals"], path=spec["path"], precision=precision,
+                            mode=O.DATA_ONLY if spec.get("mode") == "data_only" else O.KNN_THEN_MASK)
         N = case.n_nodes
         R = R_batch if R_batch is not None else threads
         z = np.random.default_rng(7).standard_normal((R, N))
         t0 = time.perf_counter()
-        _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=N)
+        _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=node_limit or N)
         dt_total += time.perf_counter() - t0
         n_total += n
         used += 1
         nodes += N
         if node_budget is None or nodes >= node_budget:
             break
-    if R_batch is None:
+    if node_limit:
+        sample = (f"{threads} passes (one per thread) over the first {node_limit} nodes of the "
+                  f"path ({dt_total:.1f} s; search + kriging solve per node, {threads} threads)")
+    elif R_batch is None:
         sample = (f"{threads} full realizations ({nodes} path nodes each) of the same workload "
                   f"({dt_total:.1f} s; search + kriging solve per node per realization, "
                   f"OpenMP over realizations, {threads} threads)")
@@ -351,7 +368,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="seqsim-learned-256x256x128", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp32r", choices=["fp32r", "fp64"])
+    ap.add_argument("--precision", default=None, choices=["fp32r", "fp64"],
+                    help="kriging solves (default: fp64 for the posterior mean, fp32r otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the learned-vs-exact accuracy pass")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -374,8 +392,10 @@ def main():
         return
 
     spec = WORKLOADS[args.workload]
+    args.precision = args.precision or spec.get("precision", "fp32r")
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
     R = spec["R"]
+    mean_only = spec.get("mode") == "data_only"
     if "batch" in spec:
         reqs = batch_specs(spec)
     else:
@@ -393,7 +413,10 @@ def main():
     ddata = _lib.DeviceBuffer.from_array(ctx, coords_all)
     dvals = _lib.DeviceBuffer.from_array(ctx, vals_all)
     del path_all
-    z = upload_normals(ctx, _lib, R, N, spec["seed"])
+    if mean_only:  # the posterior mean: one realization of zero normals
+        z = _lib.DeviceBuffer.from_array(ctx, np.zeros((R, N)))
+    else:
+        z = upload_normals(ctx, _lib, R, N, spec["seed"])
     out = _lib.DeviceBuffer(ctx, (R, N))
 
     def problems(prec):
@@ -455,6 +478,9 @@ def main():
                        path=np.random.default_rng(7).permutation(int(np.prod(spec["cpu_dims"]))))
             cpu = cpu_baseline([sub], precision)
             cpu["sample"] = f"a {'x'.join(map(str, spec['cpu_dims']))} grid of the same variogram: " + cpu["sample"]
+        elif mean_only:
+            cpu = cpu_baseline(reqs, precision, node_limit=spec["cpu_nodes"])
+            cpu["unit"] = "nodes/s"
         else:
             cpu = cpu_baseline(reqs, precision)
 
@@ -465,13 +491,18 @@ def main():
     else:
         config = {"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
                   "maxneighbors": spec["kmax"], "radius": spec["radius"], "n_data": spec["n_data"]}
-    config |= {"search": "k nearest then mask", "precision": args.precision,
-               "depth": info.depth, "mean_neighbours": round(kbar, 2)}
+    config |= {"search": "k nearest data" if mean_only else "k nearest then mask",
+               "precision": args.precision, "depth": info.depth, "mean_neighbours": round(kbar, 2)}
+    if S is not None:
+        metric = "SEQSIM nodes*realizations/s (learned kriging weights, exact fallback)"
+    elif mean_only:
+        metric = "kriging posterior mean nodes/s"
+    else:
+        metric = "SEQSIM nodes*realizations/s (exact kriging)"
     line = {
-        "metric": ("SEQSIM nodes*realizations/s (learned kriging weights, exact fallback)"
-                   if S is not None else "SEQSIM nodes*realizations/s (exact kriging)"),
+        "metric": metric,
         "value": round(value, 1),
-        "unit": "nodes*realizations/s",
+        "unit": "nodes/s" if mean_only else "nodes*realizations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

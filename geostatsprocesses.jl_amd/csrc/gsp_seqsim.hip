@@ -6,7 +6,8 @@
 // and variances depend only on the path and the geometry, so this file splits
 // the work into
 //   k_search    one wavefront per node: neighbour search (offset table over the
-//               implicit grid + a cell hash for the data);
+//               implicit grid + a cell hash for the data; rings of hash cells
+//               when no distance bound is known, as in the data-only search);
 //   k_krige*    covariance assembly and a register-resident Cholesky solve of
 //               the kriging system (four systems per wavefront up to 16
 //               neighbours, one above);
@@ -284,6 +285,11 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
 // kmax usable grid offsets of the request's sorted offset table, the data of
 // the node's hash list within the k-th grid distance, their merge by
 // (d2, index) and the mask. Writes nbr[i][0..K) and nfound[i].
+// RING: searches without a distance bound (data only, or masked with no
+// ball) visit rings of hash cells; the other searches never lack a bound
+// (the offset table always yields kmax grid candidates), and keep the simpler
+// scan of every cell for that case.
+template <bool RING>
 __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_gidx[kWaves][64];
   __shared__ double s_gd2[kWaves][64];
@@ -351,62 +357,101 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     // ---- data candidates with d2 <= Dk, from the cell hash
     int nd = 0;
     if (R.n_data > 0) {
-      // scan one hash-cell list, appending data with d2 <= Dk to the LDS list
-      auto scan_list = [&](int s0, int e0l) {
-        for (int b0 = s0; b0 < e0l; b0 += 64) {
-          bool ok = false;
-          int di = 0;
-          double d2 = 0.0;
-          if (b0 + lane < e0l) {
-            const DataItem it = A.hitems[b0 + lane];
-            di = it.id;
-            d2 = data_d2(R, it, ti);
-            ok = d2 <= Dk && d2 <= R.r2;
-          }
-          const uint64_t b = __ballot(ok);
-          const int add = __popcll(b);
-          if (add == 0) continue;
-          if (nd + add > 128) {
-            // keep the best kmax entries found so far
-            wave_sync();
-            int keep_rank[2] = {128, 128};
-            int kid[2] = {0, 0};
-            double kd[2] = {0.0, 0.0};
+      double lim = Dk;  // data farther than this cannot enter the top kmax
+      // keep the best kmax entries of the LDS list, each at its rank
+      auto trim = [&]() {
+        wave_sync();
+        int keep_rank[2] = {128, 128};
+        int kid[2] = {0, 0};
+        double kd[2] = {0.0, 0.0};
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-              const int m = lane + 64 * h;
-              if (m < nd) {
-                kd[h] = s_dd2[w][m];
-                kid[h] = s_didx[w][m];
-                int r = 0;
-                for (int m2 = 0; m2 < nd; ++m2)
-                  r += key_less(s_dd2[w][m2], s_didx[w][m2], kd[h], kid[h]);
-                keep_rank[h] = r;
-              }
-            }
-            wave_sync();
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-              if (keep_rank[h] < kmax) {
-                s_dd2[w][keep_rank[h]] = kd[h];
-                s_didx[w][keep_rank[h]] = kid[h];
-              }
-            nd = nd < kmax ? nd : kmax;
-            wave_sync();
+        for (int h = 0; h < 2; ++h) {
+          const int m = lane + 64 * h;
+          if (m < nd) {
+            kd[h] = s_dd2[w][m];
+            kid[h] = s_didx[w][m];
+            int r = 0;
+            for (int m2 = 0; m2 < nd; ++m2)
+              r += key_less(s_dd2[w][m2], s_didx[w][m2], kd[h], kid[h]);
+            keep_rank[h] = r;
           }
-          if (ok) {
-            const int r = nd + mbcnt(b);
-            s_didx[w][r] = A.NT + di;
-            s_dd2[w][r] = d2;
-          }
-          nd += add;
         }
+        wave_sync();
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (keep_rank[h] < kmax) {
+            s_dd2[w][keep_rank[h]] = kd[h];
+            s_didx[w][keep_rank[h]] = kid[h];
+          }
+        nd = nd < kmax ? nd : kmax;
+        wave_sync();
+        if (nd == kmax) lim = s_dd2[w][kmax - 1];
+      };
+      // one 64-lane pass: item `idx` of the hash lists on each valid lane;
+      // appends data with d2 <= lim to the LDS list
+      auto consider = [&](bool valid, int idx) {
+        bool ok = false;
+        int di = 0;
+        double d2 = 0.0;
+        if (valid) {
+          const DataItem it = A.hitems[idx];
+          di = it.id;
+          d2 = data_d2(R, it, ti);
+          ok = d2 <= lim && d2 <= R.r2;
+        }
+        const uint64_t b = __ballot(ok);
+        const int add = __popcll(b);
+        if (add == 0) return;
+        if (nd + add > 128) trim();  // may lower lim
+        const bool put = ok && d2 <= lim;
+        const uint64_t pb = __ballot(put);
+        if (put) {
+          const int r = nd + mbcnt(pb);
+          s_didx[w][r] = A.NT + di;
+          s_dd2[w][r] = d2;
+        }
+        nd += __popcll(pb);
+      };
+      auto scan_list = [&](int s0, int e0l) {
+        for (int b0 = s0; b0 < e0l; b0 += 64) consider(b0 + lane < e0l, b0 + lane);
       };
       const int32_t* hs = A.hstart + R.cell_off;
+      const auto hsv = const_view(hs);  // wave-uniform list bounds: scalar loads
+      // Searches without a bound gather item ranges (runs of adjacent cells
+      // along x) in lanes, up to 64 at a time, and scan them as one flat list:
+      // few, full 64-lane passes instead of one pass per cell.
+      int rs = 0, rn = 0, nr = 0;  // this lane's range (first item, count); ranges held
+      auto flush = [&]() {
+        if (nr == 0) return;
+        int pre = 0, total = 0;
+        for (int j = 0; j < nr; ++j) {
+          const int nj = rli(rn, j);
+          if (lane == j) pre = total;
+          total += nj;
+        }
+        for (int b0 = 0; b0 < total; b0 += 64) {
+          const int t = b0 + lane;
+          int jj = 0;
+          for (int j = 1; j < nr; ++j) jj = t >= rli(pre, j) ? j : jj;
+          consider(t < total, __shfl(rs, jj) + (t - __shfl(pre, jj)));
+        }
+        nr = 0;
+      };
+      auto add_range = [&](int a, int b) {
+        if (b > a) {
+          if (lane == nr) {
+            rs = a;
+            rn = b - a;
+          }
+          if (++nr == 64) flush();
+        }
+      };
       if (R.hexp) {
         const int cell = ti[0] / R.hB + R.hd[0] * (ti[1] / R.hB + R.hd[1] * (ti[2] / R.hB));
         scan_list(hs[cell], hs[cell + 1]);
-      } else {
+      } else if (!RING || Dk < __builtin_inf()) {
+        // every cell overlapping the ball of radius sqrt(Dk) (all cells
+        // without a bound)
         int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         const bool all = !(Dk < __builtin_inf());
         const double rho = all ? 0.0 : __builtin_sqrt(Dk) * (1.0 + 0x1p-20) + 0x1p-40;
@@ -416,22 +461,80 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
           if (all) {
             lo[d] = 0;
             hi[d] = R.hd[d] - 1;
-          } else {
-            const double c = R.org[d] + ((double)ti[d] + 0.5) * R.sp[d];
-            double fl = __builtin_floor((c - rho - R.org[d]) / R.hcs[d]);
-            double fh = __builtin_floor((c + rho - R.org[d]) / R.hcs[d]);
-            fl = fl < 0.0 ? 0.0 : (fl > R.hd[d] - 1 ? R.hd[d] - 1 : fl);
-            fh = fh < 0.0 ? 0.0 : (fh > R.hd[d] - 1 ? R.hd[d] - 1 : fh);
-            lo[d] = (int)fl;
-            hi[d] = (int)fh;
+            continue;
           }
+          const double c = R.org[d] + ((double)ti[d] + 0.5) * R.sp[d];
+          double fl = __builtin_floor((c - rho - R.org[d]) / R.hcs[d]);
+          double fh = __builtin_floor((c + rho - R.org[d]) / R.hcs[d]);
+          fl = fl < 0.0 ? 0.0 : (fl > R.hd[d] - 1 ? R.hd[d] - 1 : fl);
+          fh = fh < 0.0 ? 0.0 : (fh > R.hd[d] - 1 ? R.hd[d] - 1 : fh);
+          lo[d] = (int)fl;
+          hi[d] = (int)fh;
         }
         for (int cz = lo[2]; cz <= hi[2]; ++cz)
-          for (int cy = lo[1]; cy <= hi[1]; ++cy)
-            for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-              const int cell = cx + R.hd[0] * (cy + R.hd[1] * cz);
-              scan_list(hs[cell], hs[cell + 1]);
+          for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            const int row = R.hd[0] * (cy + R.hd[1] * cz);
+            if constexpr (RING) {
+              add_range(hsv[row + lo[0]], hsv[row + hi[0] + 1]);
+            } else {
+              for (int cx = lo[0]; cx <= hi[0]; ++cx) scan_list(hs[row + cx], hs[row + cx + 1]);
             }
+          }
+        if constexpr (RING) flush();
+      } else {
+        // no bound yet: rings of cells at Chebyshev distance r = 0, 1, ... from
+        // the node's cell, until the kmax-th candidate is closer than every
+        // cell not yet visited (data outside the grid sit in the edge cells,
+        // beyond their cell's inner face, so the bound holds for them too)
+        int cc[3];
+        double c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          cc[d] = ti[d] / R.hB;
+          c[d] = R.org[d] + ((double)ti[d] + 0.5) * R.sp[d];
+        }
+        for (int r = 0;; ++r) {
+          int lo[3], hi[3];
+          bool whole = true;
+          double gap = __builtin_inf();
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = cc[d] - r > 0 ? cc[d] - r : 0;
+            hi[d] = cc[d] + r < R.hd[d] - 1 ? cc[d] + r : R.hd[d] - 1;
+            if (lo[d] > 0) {
+              whole = false;
+              const double g = c[d] - (R.org[d] + (double)lo[d] * R.hcs[d]) - 0x1p-30 * R.hcs[d];
+              gap = g < gap ? g : gap;
+            }
+            if (hi[d] < R.hd[d] - 1) {
+              whole = false;
+              const double g =
+                  R.org[d] + (double)(hi[d] + 1) * R.hcs[d] - c[d] - 0x1p-30 * R.hcs[d];
+              gap = g < gap ? g : gap;
+            }
+          }
+          for (int cz = lo[2]; cz <= hi[2]; ++cz)
+            for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+              const int row = R.hd[0] * (cy + R.hd[1] * cz);
+              if (cz - cc[2] == r || cc[2] - cz == r || cy - cc[1] == r || cc[1] - cy == r) {
+                add_range(hsv[row + lo[0]], hsv[row + hi[0] + 1]);
+              } else {
+                if (cc[0] - r >= 0) add_range(hsv[row + cc[0] - r], hsv[row + cc[0] - r + 1]);
+                if (r > 0 && cc[0] + r <= R.hd[0] - 1)
+                  add_range(hsv[row + cc[0] + r], hsv[row + cc[0] + r + 1]);
+              }
+            }
+          if (whole) {
+            flush();
+            break;
+          }
+          if (r == 0) continue;  // ring 0 joins ring 1's pass
+          flush();
+          if (nd >= kmax) {
+            trim();
+            if (gap > 0.0 && lim < gap * gap) break;
+          }
+        }
       }
       wave_sync();
     }
@@ -1314,7 +1417,10 @@ void build_hash(const gsp_grid& g, int mode, HostReq& h) {
     }
   }
   if (!h.hexp) {
-    int64_t B = (int64_t)std::llround(std::pow((double)h.N / (double)nD, 1.0 / nd));
+    // about one datum per cell; kmax / 2 per cell for the data-only search,
+    // whose rings then rarely pass the first ring of neighbours
+    const double per = mode == GSP_DATA_ONLY ? std::max(1.0, 0.5 * h.kmax) : 1.0;
+    int64_t B = (int64_t)std::llround(std::pow((double)h.N * per / (double)nD, 1.0 / nd));
     if (B < 1) B = 1;
     h.hB = B;
     int64_t ncell = 1;
@@ -1498,7 +1604,10 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   const int32_t devp = problems[0].device_pointers ? 1 : 0;
   int64_t NT = 0, nDT = 0, nC = 0;
   int32_t K = 1;
+  bool ring = false;  // some request searches without a distance bound
   for (int32_t q = 0; q < n_req; ++q) {
+    const int32_t mode = problems[q].search_mode;
+    ring = ring || mode == GSP_DATA_ONLY || (mode == GSP_MASKED_KNN && !(problems[q].radius > 0));
     for_request(n_req, q, [&] {
       const gsp_seqsim_problem* p = &problems[q];
       validate_problem(p, H[(size_t)q]);
@@ -1771,7 +1880,10 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
 
   {
     TimedLaunch tl(ctx, GSP_K_SEARCH);
-    hipLaunchKernelGGL(k_search, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
+    if (ring)
+      hipLaunchKernelGGL(k_search<true>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
+    else
+      hipLaunchKernelGGL(k_search<false>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
     GSP_HIP(hipGetLastError());
   }
   tr.mark("search kernel");

@@ -16,9 +16,9 @@ JULIA_RTOL = float(np.sqrt(np.finfo(float).eps))  # Base.isapprox default
 
 
 def data_case(seed, dims=(30, 24, 10), n_data=300, kmax=10, radius=0.0, kmin=1,
-              precision=O.FP64, family=O.SPHERICAL):
+              precision=O.FP64, family=O.SPHERICAL, spread=(-0.05, 1.05)):
     rng = np.random.default_rng(seed)
-    data = rng.uniform(-0.05, 1.05, (n_data, len(dims))) * np.asarray(dims, float)
+    data = rng.uniform(*spread, (n_data, len(dims))) * np.asarray(dims, float)
     return O.SeqsimCase(dims=dims, family=family, sill=1.3, nugget=0.05,
                         ranges=(9.0, 7.0, 4.0)[:len(dims)], mean=0.25, data_coords=data,
                         data_vals=rng.standard_normal(n_data), kmin=kmin, kmax=kmax,
@@ -34,6 +34,12 @@ def data_case(seed, dims=(30, 24, 10), n_data=300, kmax=10, radius=0.0, kmin=1,
     dict(seed=5, dims=(64, 48), n_data=500, kmax=16),
     dict(seed=6, kmax=10, precision=O.FP32_REFINE),
     dict(seed=7, kmax=1, n_data=30),
+    # rings of hash cells: data clustered in one corner, data far outside the
+    # grid, a 1D grid, more data than nodes
+    dict(seed=8, n_data=200, spread=(0.0, 0.2)),
+    dict(seed=9, n_data=200, kmax=20, spread=(-2.0, 3.0)),
+    dict(seed=10, dims=(300,), n_data=17, kmax=6),
+    dict(seed=11, dims=(20, 16), n_data=2000, kmax=12),
 ])
 def test_data_only_plan_matches_oracle(ctx, spec):
     case = data_case(**spec)
@@ -113,3 +119,18 @@ def test_posterior_mean_missing_values_rejected(ctx):
     data = gsp.georef({"Z": [1.0, np.nan]}, [(1.0, 1.0), (3.0, 4.0)])
     with pytest.raises(_lib.UnsupportedError):
         gsp.mean(proc, grid, data=data, ctx=ctx)
+
+
+def test_masked_search_without_ball_matches_oracle(ctx):
+    """Masked k nearest with no search ball: the first nodes of the path find
+    fewer than kmax simulated nodes and search the data without a bound."""
+    rng = np.random.default_rng(4)
+    dims = (18, 14, 6)
+    data = rng.uniform(0.0, 0.3, (60, 3)) * np.asarray(dims, float)
+    case = O.SeqsimCase(dims=dims, family=O.EXPONENTIAL, ranges=(6.0,), data_coords=data,
+                        data_vals=rng.standard_normal(60), kmax=12, mode=O.MASKED_KNN,
+                        path=rng.permutation(int(np.prod(dims))))
+    g = _lib.SeqsimPlan(ctx, gpu_problem(case), keep=(case,)).export()
+    o = O.seqsim_plan(case)
+    for k in ("nfound", "nbr", "lam", "sigma", "flags", "level"):
+        np.testing.assert_array_equal(g[k], getattr(o, k), err_msg=k)
