@@ -304,7 +304,21 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
 
   int cur = -1;
   Offset e0 = {0.0, 0, 0, 0, 0};
-  for (int64_t q = (int64_t)blockIdx.x * kWaves + w; q < A.NT; q += wstride) {
+  // RING: every wave takes a contiguous run of path positions, and carries
+  // the k-th data distance of its previous node: if that node lies delta
+  // away, the k nearest data of this one lie within (D_prev + delta), a bound
+  // that replaces the ring search
+  int64_t q_begin = (int64_t)blockIdx.x * kWaves + w, q_end = A.NT, q_step = wstride;
+  if constexpr (RING) {
+    const int64_t chunk = (A.NT + wstride - 1) / wstride;
+    q_begin = q_begin * chunk;
+    q_end = q_begin + chunk < A.NT ? q_begin + chunk : A.NT;
+    q_step = 1;
+  }
+  int prev_req = -1;
+  double prev_k = -1.0;
+  int pti[3] = {0, 0, 0};
+  for (int64_t q = q_begin; q < q_end; q += q_step) {
     const int64_t pv = A.path[q];
     const int gi = __builtin_amdgcn_readfirstlane((int)(pv & 0xFFFFFFFF));
     const int p = __builtin_amdgcn_readfirstlane((int)(pv >> 32));
@@ -352,7 +366,19 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     }
     const int ng = cnt < kmax ? cnt : kmax;
     wave_sync();
-    const double Dk = ng == kmax ? s_gd2[w][ng - 1] : R.r2;
+    double Dk = ng == kmax ? s_gd2[w][ng - 1] : R.r2;
+    if constexpr (RING) {
+      if (!(Dk < __builtin_inf()) && p == prev_req && prev_k >= 0.0) {
+        double dl = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double u = (double)(ti[d] - pti[d]) * R.sp[d];
+          dl = dl + u * u;
+        }
+        const double b = (__builtin_sqrt(prev_k) + __builtin_sqrt(dl)) * (1.0 + 0x1p-30);
+        Dk = b * b;
+      }
+    }
 
     // ---- data candidates with d2 <= Dk, from the cell hash
     int nd = 0;
@@ -536,7 +562,21 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
           }
         }
       }
+      if constexpr (RING) {
+        // the k-th data distance of this node, for the next one
+        prev_k = -1.0;
+        if (nd >= kmax) {
+          trim();
+          prev_k = lim;
+        }
+      }
       wave_sync();
+    }
+    if constexpr (RING) {
+      prev_req = p;
+      pti[0] = ti[0];
+      pti[1] = ti[1];
+      pti[2] = ti[2];
     }
 
     // ---- merge: top-kmax of the grid list (sorted) and the data list
