@@ -39,6 +39,13 @@ variogram sill 1, range 20, nugget 0.01, the 10 nearest data of every node
 (GeoStatsModels' fitpredict defaults), FP64 solves. One step is one
 data-only plan plus its zero-normal sweep (the mean at every node).
 
+Workload "fftsim-256x256x128": FFT moving-average simulation (FFTSIM, the
+default method of unconditional grids) on a 256 x 256 x 128 unit grid
+(8,388,608 cells), Gaussian variogram sill 1, range 20, 16 realizations from
+uniform white noise resident in HBM. One step is one plan (covariance,
+amplitude spectrum) plus the 16 realizations (forward and inverse transforms,
+phase, variance scaling).
+
 Prints one JSON line. `cpu_baseline` times the CPU restatement of the
 reference's cost structure (search + solve per node per realization) on a
 bounded sample of the same workload.
@@ -72,6 +79,8 @@ WORKLOADS = {
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01,
         n_data=100_000, kmax=10, radius=0.0, R=1, seed=1006, mode="data_only", precision="fp64",
         cpu_nodes=1_500_000),
+    "fftsim-256x256x128": dict(dims=(256, 256, 128), family=2, sill=1.0, ranges=(20.0, 20.0, 20.0),
+                               nugget=0.0, R=16, seed=1007, method="fftsim", cpu_reals=2),
 }
 
 
@@ -362,6 +371,90 @@ def train_surrogate(ctx, _lib, spec, precision):
                          "same variogram, random paths"}
 
 
+def fftsim_problem(_lib, spec):
+    dims = spec["dims"]
+    nd = len(dims)
+    p = _lib.GspFftsimProblem()
+    p.grid.ndim = nd
+    for d in range(3):
+        p.grid.dims[d] = dims[d] if d < nd else 1
+        p.grid.spacing[d] = 1.0
+        p.model.ranges[d] = spec["ranges"][d] if d < nd else 1.0
+    p.model.family = spec["family"]
+    p.model.kind = _lib.VARIOGRAM
+    p.model.sill = spec["sill"]
+    p.model.nugget = spec["nugget"]
+    p.minneighbors, p.maxneighbors = 1, 26
+    return p
+
+
+def bench_fftsim(args, spec):
+    """FFTSIM: plan + R realizations per step; the transforms are rocFFT's."""
+    from geostatsprocesses import _lib
+    dims, R = spec["dims"], spec["R"]
+    N = int(np.prod(dims))
+    ctx = _lib.default_context(0)
+    w = _lib.DeviceBuffer(ctx, (R, N))
+    for r in range(R):
+        row = np.random.default_rng(spec["seed"] + r).random(N)
+        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, w.ptr + r * N * 8, row.ctypes.data, N * 8))
+    out = _lib.DeviceBuffer(ctx, (R, N))
+    prob = fftsim_problem(_lib, spec)
+
+    class Step:
+        def __init__(self):
+            self.plan = _lib.FftsimPlan(ctx, prob)
+            self.plan.sample(w, out, device_pointers=True)
+            ctx.synchronize()
+
+        def close(self):
+            self.plan.close()
+
+    ms_per_step, tim = timed(ctx, Step, args.steps, args.warmup)
+    Nh = (dims[0] // 2 + 1) * int(np.prod(dims[1:]))
+    kern = {}
+    for slot, name in ((_lib.K_FFT, "fft"), (_lib.K_SPECTRAL, "spectral")):
+        ms, launches = tim[slot]
+        if launches:
+            kern[name] = {"avg_ms": round(ms / launches, 4), "launches_per_step": launches // args.steps,
+                          "ms_per_step": round(ms / args.steps, 3)}
+    # transforms: a real-to-complex pass reads N doubles and writes Nh complex
+    # values, the inverse the reverse (one plan transform + 2 per realization)
+    fft_bytes = (N * 8 + Nh * 16) * (2 * R + 1)
+    fft_launches = kern["fft"]["launches_per_step"]
+    per_launch = fft_bytes / fft_launches
+    achieved = per_launch / (kern["fft"]["avg_ms"] * 1e-3) / 1e9
+    kern["fft"] |= {"algorithmic_bytes_per_launch": int(per_launch), "achieved_GBs": round(achieved, 1)}
+    roofline = {"bound": "hbm", "kernel": "fft (rocFFT)", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        F = O.fftsim_amplitude(dims, spec["family"], spec["sill"], spec["nugget"], spec["ranges"])
+        nr = spec["cpu_reals"]
+        O.fftsim_unconditional(F, np.random.default_rng(0).random((nr, N)), spec["sill"], 0.0)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(N * nr / dt, 1), "unit": "cells*realizations/s", "cores": 1,
+               "kind": "port",
+               "sample": f"one plan and {nr} realizations of the same grid with numpy's FFT "
+                         f"({dt:.1f} s, one thread)"}
+    line = {
+        "metric": "FFTSIM cells*realizations/s",
+        "value": round(N * R / (ms_per_step / 1e3), 1),
+        "unit": "cells*realizations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(dims), "realizations": R,
+                   "variogram": "gaussian", "range": spec["ranges"][0]},
+        "roofline": roofline, "kernels": kern, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,6 +485,8 @@ def main():
         return
 
     spec = WORKLOADS[args.workload]
+    if spec.get("method") == "fftsim":
+        return bench_fftsim(args, spec)
     args.precision = args.precision or spec.get("precision", "fp32r")
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
     R = spec["R"]

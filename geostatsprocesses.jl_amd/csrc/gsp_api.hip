@@ -136,6 +136,7 @@ int32_t gsp_ctx_destroy(gsp_ctx* ctx) {
     ctx->Zt.release();
     ctx->counters.release();
     gsp::release_timing_events(ctx);
+    ctx->fft_plans.reset();
     ctx->pool->trim();
     (void)hipStreamDestroy(ctx->stream);
   }

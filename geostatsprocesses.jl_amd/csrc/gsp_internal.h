@@ -162,6 +162,8 @@ struct gsp_ctx {
   gsp::DevBuf<double> stage_out;
   gsp::DevBuf<double> Zt;       // normals of one realization chunk, node-major
   gsp::DevBuf<int32_t> counters;  // [0] ticket, [1] error flag, [2..] scratch counters
+  // FFTSIM transform plans by shape, reused across plans (gsp_fftsim.hip)
+  std::shared_ptr<void> fft_plans;
   // kernel timing
   bool timing = false;
   struct Pending {
@@ -325,6 +327,9 @@ __host__ __device__ inline double cov_points(const DevModel& m, double a0, doubl
 }
 
 DevModel make_dev_model(const gsp_model& m, int ndim, double alpha);
+// Argument checks shared by the plan builders (number of cells of a valid grid).
+int64_t validate_grid(const gsp_grid& g);
+void validate_model(const gsp_model& m, int ndim);
 
 // ------------------------------------------------------ seqsim internals ----
 struct Offset {

@@ -334,9 +334,6 @@ void mfma_probe(gsp_ctx* ctx, int fp32, double* host_out) {
   GSP_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-int64_t validate_grid(const gsp_grid& g);
-void validate_model(const gsp_model& m, int ndim);
-
 void lusim_factor_build(gsp_ctx* ctx, const gsp_lusim_problem* p, gsp_lusim_plan* plan) {
   GSP_REQUIRE(p, GSP_ERR_ARGUMENT, "null problem");
   const int64_t N = validate_grid(p->grid);

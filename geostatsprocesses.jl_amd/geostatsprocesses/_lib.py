@@ -71,6 +71,15 @@ class GspLusimProblem(ctypes.Structure):
                 ("device_pointers", ctypes.c_int32)]
 
 
+class GspFftsimProblem(ctypes.Structure):
+    _fields_ = [("grid", GspGrid), ("model", GspModel), ("mean", ctypes.c_double),
+                ("n_data", ctypes.c_int64), ("data_coords", ctypes.c_void_p),
+                ("data_vals", ctypes.c_void_p), ("n_cells", ctypes.c_int64),
+                ("cells", ctypes.c_void_p), ("minneighbors", ctypes.c_int32),
+                ("maxneighbors", ctypes.c_int32), ("radius", ctypes.c_double),
+                ("device_pointers", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
 class GspSeqsimInfo(ctypes.Structure):
     _fields_ = [("n_nodes", ctypes.c_int64), ("n_data", ctypes.c_int64),
                 ("minneighbors", ctypes.c_int32), ("maxneighbors", ctypes.c_int32),
@@ -79,8 +88,8 @@ class GspSeqsimInfo(ctypes.Structure):
                 ("n_fallback", ctypes.c_int64), ("n_learned", ctypes.c_int64)]
 
 
-K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_LEARNED, K_COUNT = \
-    0, 1, 2, 3, 4, 5, 6, 7, 8
+(K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_LEARNED, K_FFT,
+ K_SPECTRAL, K_COUNT) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
 
 
 class GspTrainOptions(ctypes.Structure):
@@ -134,6 +143,10 @@ SIGNATURES = {
     "gsp_lusim_dims": (_I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "gsp_lusim_export": (_I32, [_VP, _VP, _VP, _VP]),
     "gsp_lusim_plan_free": (_I32, [_VP]),
+    "gsp_fftsim_plan_create": (_I32, [_VP, ctypes.POINTER(GspFftsimProblem), ctypes.POINTER(_VP)]),
+    "gsp_fftsim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
+    "gsp_fftsim_export": (_I32, [_VP, _VP, _VP, _VP]),
+    "gsp_fftsim_plan_free": (_I32, [_VP]),
 }
 
 # test hooks (include/gsp_testing.h)
@@ -471,6 +484,58 @@ class LusimPlan:
     def close(self):
         if self._h and self._h.value:
             load().gsp_lusim_plan_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FftsimPlan:
+    """gsp_fftsim_plan: amplitude spectrum and, with data, the kriging mean and
+    the kriging weights of the data cells."""
+
+    def __init__(self, ctx: Context, problem: GspFftsimProblem, keep=()):
+        self.ctx = ctx
+        self._keep = keep
+        self._h = ctypes.c_void_p()
+        check(load().gsp_fftsim_plan_create(ctx.handle, ctypes.byref(problem), ctypes.byref(self._h)))
+        g = problem.grid
+        self.dims = tuple(int(g.dims[d]) for d in range(g.ndim))
+        self.n_nodes = int(np.prod(self.dims))
+        self.conditional = problem.n_data > 0
+
+    def export(self):
+        """(amplitude on the half spectrum, shape (nz, ny, nx // 2 + 1) in C
+        order for 3D, zbar or None)."""
+        half = list(self.dims[::-1])
+        half[-1] = half[-1] // 2 + 1
+        F = np.zeros(half)
+        zbar = np.zeros(self.n_nodes) if self.conditional else None
+        check(load().gsp_fftsim_export(self.ctx.handle, self._h, ptr(F), ptr(zbar)))
+        return F, zbar
+
+    def sample(self, w, out=None, device_pointers=False):
+        """w: (R, N) uniform [0, 1) noise, column-major grid order. Returns (R, N)."""
+        if device_pointers:
+            R = int(w.shape[0])
+            check(load().gsp_fftsim_sample(self.ctx.handle, self._h, R, ptr(w), ptr(out), 1))
+            return out
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if w.ndim == 1:
+            w = w[None, :]
+        if w.shape[1] != self.n_nodes:
+            raise ArgumentError("w must hold one value per grid cell")
+        R = w.shape[0]
+        res = np.empty((R, self.n_nodes)) if out is None else out
+        check(load().gsp_fftsim_sample(self.ctx.handle, self._h, R, ptr(w), ptr(res), 0))
+        return res
+
+    def close(self):
+        if self._h and self._h.value:
+            load().gsp_fftsim_plan_free(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

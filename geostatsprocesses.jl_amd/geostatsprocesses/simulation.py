@@ -4,11 +4,11 @@
   preprocess(rng, process, method, init, domain, data)    called once
   randsingle(rng, process, method, domain, data, preproc) called per realization
 
-for (GaussianProcess, SEQSIM) and (GaussianProcess, LUSIM). All arithmetic
-runs in libgsp_hip.so on the GPU; this module only shapes arguments. The
-ensemble form of `rand` draws every realization's normals in the reference's
-order (realization after realization, one per path node / simulated cell) and
-runs them in one device call.
+for (GaussianProcess, SEQSIM), (GaussianProcess, LUSIM) and (GaussianProcess,
+FFTSIM). All arithmetic runs in libgsp_hip.so on the GPU; this module only
+shapes arguments. The ensemble form of `rand` draws every realization's random
+numbers in the reference's order (realization after realization, one per path
+node / simulated cell / grid cell) and runs them in one device call.
 """
 from __future__ import annotations
 
@@ -62,10 +62,17 @@ class LUSIM(FieldSimulationMethod):
 
 
 class FFTSIM(FieldSimulationMethod):
-    """Marker of the reference's FFT method (not provided by this backend)."""
+    """FFTSIM(minneighbors=1, maxneighbors=26, neighborhood=None,
+    distance='euclidean') — fftsim.jl:49-53. The options are those of the
+    kriging that conditions the realizations on data."""
 
-    def __init__(self, **kwargs):
-        self.options = kwargs
+    def __init__(self, minneighbors=1, maxneighbors=26, neighborhood=None, distance="euclidean"):
+        self.minneighbors = int(minneighbors)
+        self.maxneighbors = int(maxneighbors)
+        self.neighborhood = neighborhood
+        if distance not in ("euclidean", None):
+            raise _lib.UnsupportedError("only the Euclidean distance is supported")
+        self.distance = distance
 
 
 def _grid_struct(domain) -> _lib.GspGrid:
@@ -186,6 +193,57 @@ def _lusim_problem(process, method, init, domain, data, ctx):
     return LusimPreproc(plan, names, float(process.mean))
 
 
+# ---------------------------------------------------------------- FFTSIM ----
+class FftsimPreproc:
+    def __init__(self, plan, names):
+        self.plan = plan
+        self.names = names
+
+
+def _fftsim_problem(process, method, init, domain, data, ctx):
+    """fftsim.jl:54-107: the amplitude spectrum and, with data, the kriging
+    mean and the cells initialize() placed the data on."""
+    f = process.func
+    if not f.isstationary():
+        raise AssertionError("geostatistical function must be stationary")
+    grid = _grid_struct(domain)
+    real, mask = initialize(process, domain, data, init)
+    names = tuple(real)
+    if len(names) != 1:
+        raise AssertionError("FFTSIM does not support multivariate simulation")
+    var = names[0]
+    nd = domain.ndim
+    p = _lib.GspFftsimProblem()
+    p.grid = grid
+    p.model = f.struct(nd)
+    p.mean = float(process.mean)
+    keep = ()
+    if data is not None:
+        coords = np.ascontiguousarray(data.domain.centroids(), dtype=np.float64).reshape(-1, nd)
+        vals = np.ascontiguousarray(data[var], dtype=np.float64)
+        if np.isnan(vals).any():
+            raise _lib.UnsupportedError("missing data values are not supported by this backend")
+        cells = np.ascontiguousarray(np.flatnonzero(mask[var]), dtype=np.int64)
+        p.n_data = len(coords)
+        p.data_coords = coords.ctypes.data
+        p.data_vals = vals.ctypes.data
+        p.n_cells = len(cells)
+        p.cells = cells.ctypes.data
+        keep = (coords, vals, cells)
+    p.minneighbors = method.minneighbors
+    p.maxneighbors = method.maxneighbors
+    neigh = method.neighborhood
+    if neigh is None:
+        p.radius = 0.0
+    elif isinstance(neigh, MetricBall):
+        p.radius = neigh.radius
+    else:
+        p.radius = float(neigh)
+    p.device_pointers = 0
+    plan = _lib.FftsimPlan(ctx, p, keep=keep)
+    return FftsimPreproc(plan, names)
+
+
 # -------------------------------------------------------- plugin contract ----
 def preprocess(rng, process, method, init, domain, data, ctx=None):
     ctx = _lib.default_context() if ctx is None else ctx
@@ -195,6 +253,8 @@ def preprocess(rng, process, method, init, domain, data, ctx=None):
         return _seqsim_problem(process, method, domain, data, ctx)
     if isinstance(method, LUSIM):
         return _lusim_problem(process, method, init, domain, data, ctx)
+    if isinstance(method, FFTSIM):
+        return _fftsim_problem(process, method, init, domain, data, ctx)
     raise _lib.UnsupportedError(f"{type(method).__name__} is not provided by this backend")
 
 
@@ -202,6 +262,10 @@ def _draw(rng, preproc, R):
     if isinstance(preproc, SeqsimPreproc):
         z = rng.standard_normal((R, preproc.plan.n_nodes))
         return preproc.plan.sweep(z, preproc.data_vals)
+    if isinstance(preproc, FftsimPreproc):
+        # uniform white noise, one value per cell (fftsim.jl:124)
+        w = rng.random((R, preproc.plan.n_nodes))
+        return preproc.plan.sample(w)
     w = rng.standard_normal((R, preproc.plan.n_sim))
     return preproc.plan.sample(w)
 

@@ -1,13 +1,14 @@
 /*
  * gsp.h — C ABI of libgsp_hip.so, the MI355X (gfx950) backend for
- * GeoStatsProcesses' GaussianProcess simulation methods SEQSIM and LUSIM.
+ * GeoStatsProcesses' GaussianProcess simulation methods SEQSIM and LUSIM
+ * (and FFTSIM, the default method of unconditional grids, field.jl:159).
  *
  * The Julia host keeps `rand(process, domain, nreals; method=...)` unchanged
  * (src/simulation/field.jl:73-124). A package extension (the pattern of
  * ext/GeoStatsProcessesTuringPatternsExt.jl:15, `import GeoStatsProcesses:
  * preprocess, randsingle`) overrides the two plugin functions for
- * (GaussianProcess, SEQSIM) and (GaussianProcess, LUSIM) and `ccall`s the
- * entry points below. Plain C types only; every function returns a gsp_status.
+ * (GaussianProcess, SEQSIM), (GaussianProcess, LUSIM) and (GaussianProcess,
+ * FFTSIM) and `ccall`s the entry points below. Plain C types only; every function returns a gsp_status.
  *
  * Conventions
  *  - Indices are 0-based; grids are column-major (x fastest), like Meshes'
@@ -156,7 +157,9 @@ enum {
   GSP_K_LU_SAMPLE = 5,   /* LUSIM d2 + L22 W and scatter (whole call) */
   GSP_K_SEARCH = 6,      /* neighbour search */
   GSP_K_LEARNED = 7,     /* surrogate inference (learned plans) */
-  GSP_K_COUNT = 8
+  GSP_K_FFT = 8,         /* FFTSIM transforms (rocFFT through hipFFT) */
+  GSP_K_SPECTRAL = 9,    /* FFTSIM phase/amplitude, variance, scaling, conditioning */
+  GSP_K_COUNT = 10
 };
 typedef struct {
   double ms[GSP_K_COUNT];       /* accumulated milliseconds */
@@ -187,6 +190,26 @@ typedef struct gsp_ctx gsp_ctx;
 typedef struct gsp_seqsim_plan gsp_seqsim_plan;
 typedef struct gsp_lusim_plan gsp_lusim_plan;
 typedef struct gsp_surrogate gsp_surrogate;
+typedef struct gsp_fftsim_plan gsp_fftsim_plan;
+
+/* Everything `preprocess(rng, process, ::FFTSIM, init, domain, data)` needs
+ * after `initialize` has placed the data (fftsim.jl:54-107). */
+typedef struct {
+  gsp_grid grid;                /* every side >= 2 (fftsim.jl:79-80) */
+  gsp_model model;              /* stationary, one variable (fftsim.jl:60,69) */
+  double mean;
+  /* conditioning (n_data == 0: unconditional) */
+  int64_t n_data;
+  const double* data_coords;    /* n_data x ndim: the kriging mean (fftsim.jl:94-100) */
+  const double* data_vals;      /* n_data */
+  int64_t n_cells;
+  const int64_t* cells;         /* findall(mask): ascending grid ids (fftsim.jl:104) */
+  int32_t minneighbors;         /* FFTSIM options (fftsim.jl:49-53) */
+  int32_t maxneighbors;
+  double radius;                /* neighborhood MetricBall radius; <= 0: nothing */
+  int32_t device_pointers;      /* data_coords, data_vals, cells on the device */
+  int32_t pad_;
+} gsp_fftsim_problem;
 
 /* ---- context -------------------------------------------------------------
  * One per (process, device): owns a HIP stream and reusable device workspace.
@@ -330,6 +353,31 @@ int32_t gsp_lusim_dims(const gsp_lusim_plan* plan, int64_t* n_sim, int64_t* n_da
 /* Copy L22 (n_s x n_s, row-major lower, upper zeroed) and d2 (n_s); NULL skips. */
 int32_t gsp_lusim_export(gsp_ctx* ctx, const gsp_lusim_plan* plan, double* L22, double* d2);
 int32_t gsp_lusim_plan_free(gsp_lusim_plan* plan);
+
+/* ---- FFTSIM --------------------------------------------------------------
+ * gsp_fftsim_plan_create replaces preprocess(rng, process, ::FFTSIM, ...)
+ * (fftsim.jl:54-107): C = covariances between the centroid of cell dims .÷ 2
+ * (1-based, column-major) and every centroid, F = sqrt.(abs.(fft(fftshift(C)))),
+ * F[1] = 0; with data, the simple-kriging mean zbar of the data over the grid
+ * (GeoStatsModels fitpredict with the FFTSIM options; missing -> NaN) and the
+ * kriging weights of the data cells. */
+int32_t gsp_fftsim_plan_create(gsp_ctx* ctx, const gsp_fftsim_problem* problem,
+                               gsp_fftsim_plan** out);
+/* gsp_fftsim_sample replaces R calls of randsingle(..., ::FFTSIM, ...)
+ * (fftsim.jl:109-156): P = F .* exp.(im .* angle.(fft(w))), Z = real(ifft(P)),
+ * Z = sqrt(sill / var(Z, mean=0)) Z + mean; with data
+ * z = zbar + (Z - krige(Z[cells])).
+ *   w:   R x N uniform [0, 1) white noise, realization-major, column-major grid
+ *        order (fftsim.jl:124)
+ *   out: R x N, realization-major. */
+int32_t gsp_fftsim_sample(gsp_ctx* ctx, const gsp_fftsim_plan* plan, int32_t R,
+                          const double* w, double* out, int32_t device_pointers);
+/* Copy F on the half spectrum ((nx/2 + 1) x ny x nz, first axis fastest; the
+ * other half is its mirror image) and zbar (N; conditional plans only). NULL
+ * skips. */
+int32_t gsp_fftsim_export(gsp_ctx* ctx, const gsp_fftsim_plan* plan, double* amplitude,
+                          double* zbar);
+int32_t gsp_fftsim_plan_free(gsp_fftsim_plan* plan);
 
 #ifdef __cplusplus
 }

@@ -293,23 +293,25 @@ def seqsim_sweep(case: SeqsimCase, plan: Plan, z: np.ndarray) -> np.ndarray:
     return out
 
 
-def simple_kriging_np(case: SeqsimCase, kmax=None):
-    """Simple kriging of the data over the grid centroids, restated in numpy
-    independently of the C oracle: GeoStatsModels 0.13 `fitpredict(Kriging(f,
-    mu), data, domain)` (the posterior mean of expectation/field/gaussian.jl:
-    21-25; GeoStatsModels is not vendored, its published algorithm is: the k
-    nearest data of each centroid, within the search ball when there is one,
-    then lambda = C^-1 c, mean = mu + lambda . (z - mu), variance = C(0) -
-    lambda . c). Neighbours are ordered by (distance, data index). Nodes with
-    fewer than kmin data are NaN. Returns (mean, variance, neighbour ids)."""
+def simple_kriging_weights(case: SeqsimCase, kmax=None):
+    """Simple kriging weights of every grid centroid over the data, restated in
+    numpy independently of the C oracle: GeoStatsModels 0.13 `fitpredict(
+    Kriging(f, mu), data, domain)` (the posterior mean of expectation/field/
+    gaussian.jl:21-25 and the conditioning of fftsim.jl:99,148; GeoStatsModels
+    is not vendored, its published algorithm is: the k nearest data of each
+    centroid, within the search ball when there is one, then lambda = C^-1 c,
+    variance = C(0) - lambda . c). Neighbours are ordered by (distance, data
+    index). Returns (ids (N, k) with -1 padding, lambda (N, k), variance,
+    ok), ok False where fewer than kmin data were found."""
     kmax = case.kmax if kmax is None else kmax
     X = grid_centroids(case.dims, case.origin, case.spacing)
     D = case.data_coords
     k = min(kmax, len(D))
     r2 = case.radius ** 2 if case.radius > 0 else np.inf
-    mean = np.full(len(X), np.nan)
     var = np.full(len(X), np.nan)
     ids = np.full((len(X), k), -1, np.int64)
+    lam = np.zeros((len(X), k))
+    ok = np.zeros(len(X), bool)
     for i, x in enumerate(X):
         d2 = np.sum((D - x) ** 2, axis=1)
         order = np.lexsort((np.arange(len(D)), d2))[:k]
@@ -321,10 +323,65 @@ def simple_kriging_np(case: SeqsimCase, kmax=None):
         C = covariance_np(case.family, case.sill, case.nugget, case.ranges, xd, xd, case.rotation)
         c = covariance_np(case.family, case.sill, case.nugget, case.ranges, xd, x[None, :],
                           case.rotation)[:, 0]
-        lam = np.linalg.solve(C, c)
-        mean[i] = case.mean + lam @ (case.data_vals[order] - case.mean)
-        var[i] = case.sill - lam @ c
+        lam[i, :len(order)] = np.linalg.solve(C, c)
+        var[i] = case.sill - lam[i, :len(order)] @ c
+        ok[i] = True
+    return ids, lam, var, ok
+
+
+def simple_kriging_apply(ids, lam, ok, vals, mean):
+    """mean + sum_j lambda_j (z_j - mean) for every row of `vals` (R, n_data);
+    NaN where the node has no estimate (the reference's `missing`)."""
+    vals = np.atleast_2d(np.asarray(vals, dtype=np.float64))
+    used = ids >= 0
+    g = vals[:, np.where(used, ids, 0)] - mean        # (R, N, k)
+    out = mean + np.sum(np.where(used, lam, 0.0) * g, axis=2)
+    out[:, ~ok] = np.nan
+    return out
+
+
+def simple_kriging_np(case: SeqsimCase, kmax=None):
+    """Posterior mean and variance of the data values (simple_kriging_weights).
+    Returns (mean, variance, neighbour ids)."""
+    ids, lam, var, ok = simple_kriging_weights(case, kmax)
+    mean = simple_kriging_apply(ids, lam, ok, case.data_vals, case.mean)[0]
     return mean, var, ids
+
+
+# ---------------------------------------------------------------- FFTSIM ----
+def fftsim_amplitude(dims, family, sill, nugget, ranges, origin=None, spacing=None, rotation=None):
+    """fftsim.jl:79-91: C between the centroid of cell dims .÷ 2 (1-based) and
+    every centroid, F = sqrt.(abs.(fft(fftshift(C)))), F[1] = 0. The grid is
+    column-major, so the numpy array has axes (z, y, x)."""
+    nd = len(dims)
+    origin = np.zeros(nd) if origin is None else np.asarray(origin, dtype=np.float64)
+    spacing = np.ones(nd) if spacing is None else np.asarray(spacing, dtype=np.float64)
+    cents = grid_centroids(dims, origin, spacing)
+    centre = origin + (np.array([d // 2 - 1 for d in dims]) + 0.5) * spacing
+    C = covariance_np(family, sill, nugget, ranges, centre[None, :], cents, rotation)[0]
+    F = np.sqrt(np.abs(np.fft.fftn(np.fft.fftshift(C.reshape(tuple(dims)[::-1])))))
+    F.flat[0] = 0.0
+    return F
+
+
+def fftsim_unconditional(F, w, sill, mean):
+    """fftsim.jl:123-134 for every row of w (R, N): P = F exp(i angle(fft(w))),
+    Z = real(ifft(P)), Z = sqrt(sill / var(Z, mean=0)) Z + mean."""
+    out = []
+    for wr in np.atleast_2d(np.asarray(w, dtype=np.float64)):
+        P = F * np.exp(1j * np.angle(np.fft.fftn(wr.reshape(F.shape))))
+        Z = np.real(np.fft.ifftn(P)).ravel()
+        s2 = np.sum(Z * Z) / (Z.size - 1)  # Statistics.var(Z, mean=0), corrected
+        out.append(np.sqrt(sill / s2) * Z + mean)
+    return np.array(out)
+
+
+def fftsim_conditional(zu, zbar, cells, kweights, mean):
+    """fftsim.jl:139-153: z = zbar + (zu - krige(zu[cells])), `kweights` the
+    simple_kriging_weights of the cell centroids."""
+    ids, lam, _, ok = kweights
+    zu = np.atleast_2d(zu)
+    return zbar + zu - simple_kriging_apply(ids, lam, ok, zu[:, cells], mean)
 
 
 def seqsim_reference_cost(case: SeqsimCase, z: np.ndarray, threads=1, n_nodes_limit=0):

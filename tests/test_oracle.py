@@ -396,3 +396,71 @@ def test_posterior_mean_oracle_matches_numpy(fam, kmax, radius):
     np.testing.assert_allclose(m[~few], ref[~few], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(plan.sigma[~few] ** 2, var[~few], rtol=1e-9, atol=1e-12)
     assert np.all(plan.level == 0)
+
+
+# ---------------------------------------------------------------- FFTSIM ----
+def circular_autocorrelation(x, shape):
+    X = np.fft.fftn(x.reshape(shape))
+    return np.real(np.fft.ifftn(X * np.conj(X)))
+
+
+def test_fftsim_oracle_reproduces_covariance():
+    """FFT moving average: every realization has |fft(Z)| = F exactly, so its
+    circular autocorrelation is ifft(F^2) = ifft(|fft(C_shifted)|); for a
+    covariance whose sampled spectrum is non-negative that is C itself, less
+    its mean over the periodic window (F[1] = 0 removes the field's mean):
+    A(h) / A(0) = (C(h) - Cbar) / (C(0) - Cbar) in every realization."""
+    dims = (100, 80)
+    F = O.fftsim_amplitude(dims, O.GAUSSIAN, 1.0, 0.0, (10.0, 10.0))
+    w = np.random.default_rng(0).random((2, 8000))
+    out = O.fftsim_unconditional(F, w, 1.0, 0.3)
+    # the periodic window holds every lag once: -(n/2 - 1) .. n/2 per axis
+    hx = np.arange(-49, 51)[None, :]
+    hy = np.arange(-39, 41)[:, None]
+    cbar = np.mean(np.exp(-3.0 * (hx ** 2 + hy ** 2) / 100.0))
+    for r in range(2):
+        A = circular_autocorrelation(out[r] - 0.3, F.shape)
+        for (dy, dx) in [(0, 1), (1, 0), (2, 3), (5, 0), (0, 12), (7, 7)]:
+            c = np.exp(-3.0 * (dx * dx + dy * dy) / 100.0)
+            expect = (c - cbar) / (1.0 - cbar)
+            assert A[dy, dx] / A[0, 0] == pytest.approx(expect, abs=1e-9)
+    # a dropped square root would give the covariance convolved with itself
+    A2 = np.real(np.fft.ifftn(F ** 4))
+    assert abs(A2[0, 5] / A2[0, 0] - np.exp(-3.0 * 0.25)) > 0.05
+
+
+def test_fftsim_oracle_invariants():
+    """var(Z - mean) = sill exactly; the result depends on the noise only
+    through its phases: invariant to scaling the noise and, as F[1] = 0, to
+    adding a constant to it."""
+    dims = (24, 18, 10)
+    F = O.fftsim_amplitude(dims, O.EXPONENTIAL, 2.0, 0.1, (6.0, 5.0, 3.0))
+    assert F.flat[0] == 0.0 and F.shape == (10, 18, 24)
+    w = np.random.default_rng(1).random((1, F.size))
+    out = O.fftsim_unconditional(F, w, 2.0, -1.0)[0]
+    assert np.sum((out + 1.0) ** 2) / (out.size - 1) == pytest.approx(2.0, rel=1e-12)
+    np.testing.assert_allclose(O.fftsim_unconditional(F, 3.7 * w, 2.0, -1.0)[0], out, atol=1e-12)
+    np.testing.assert_allclose(O.fftsim_unconditional(F, w + 0.5, 2.0, -1.0)[0], out, atol=1e-12)
+
+
+def test_fftsim_oracle_conditional_honours_centred_data():
+    """test/field.jl:134-141 setting: data on cell centroids are reproduced
+    (zbar honours the data and the kriged unconditional values equal the
+    unconditional values at the data cells)."""
+    dims = (100, 100)
+    coords = np.array([[25.0, 25.0], [50.0, 75.0], [75.0, 50.0]])
+    vals = np.array([1.0, 0.0, 1.0])
+    origin = (0.5, 0.5)
+    F = O.fftsim_amplitude(dims, O.GAUSSIAN, 1.0, 0.0, (35.0, 35.0), origin=origin)
+    zu = O.fftsim_unconditional(F, np.random.default_rng(2).random((2, 10000)), 1.0, 0.0)
+    dcase = O.SeqsimCase(dims=dims, origin=origin, family=O.GAUSSIAN, ranges=(35.0,),
+                         data_coords=coords, data_vals=vals, kmax=3, mode=O.DATA_ONLY)
+    zbar, _, _ = O.simple_kriging_np(dcase)
+    cells = np.sort(O.nearest_init(O.grid_centroids(dims, origin), coords))
+    ccase = O.SeqsimCase(dims=dims, origin=origin, family=O.GAUSSIAN, ranges=(35.0,),
+                         data_coords=O.grid_centroids(dims, origin)[cells], kmax=3,
+                         mode=O.DATA_ONLY)
+    z = O.fftsim_conditional(zu, zbar, cells, O.simple_kriging_weights(ccase), 0.0)
+    for c, v in zip(O.nearest_init(O.grid_centroids(dims, origin), coords), vals):
+        np.testing.assert_allclose(z[:, c], v, atol=1e-9)
+    assert not np.allclose(z[0], z[1])

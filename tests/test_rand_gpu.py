@@ -91,5 +91,3 @@ def test_seqsim_unsupported_paths(ctx):
     with pytest.raises(gsp.UnsupportedError):
         gsp.rand(proc, gsp.PointSet(np.random.default_rng(0).random((20, 2))),
                  method=gsp.SEQSIM())
-    with pytest.raises(gsp.UnsupportedError):
-        gsp.rand(proc, gsp.CartesianGrid(10, 10), method=gsp.FFTSIM())
