@@ -89,7 +89,8 @@ class GspSeqsimInfo(ctypes.Structure):
 
 
 (K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_LEARNED, K_FFT,
- K_SPECTRAL, K_COUNT) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
+ K_SPECTRAL, K_STATS, K_COUNT) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
+STAT_MEAN, STAT_VAR, STAT_CDF, STAT_CCDF, STAT_QUANTILE = 0, 1, 2, 3, 4
 
 
 class GspTrainOptions(ctypes.Structure):
@@ -147,6 +148,7 @@ SIGNATURES = {
     "gsp_fftsim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
     "gsp_fftsim_export": (_I32, [_VP, _VP, _VP, _VP]),
     "gsp_fftsim_plan_free": (_I32, [_VP]),
+    "gsp_ensemble_stats": (_I32, [_VP, _I32, _I64, _VP, _I32, _VP, _VP, _VP, _I32]),
 }
 
 # test hooks (include/gsp_testing.h)
@@ -543,3 +545,24 @@ class FftsimPlan:
             self.close()
         except Exception:
             pass
+
+
+def ensemble_stats(ctx: Context, reals, stats, out=None, device_pointers=False, shape=None):
+    """gsp_ensemble_stats: `stats` is a list of (kind, param) pairs; `reals` an
+    (R, N) array (or a device buffer with `shape` = (R, N)). Returns
+    (len(stats), N)."""
+    kinds = np.ascontiguousarray([k for k, _ in stats], dtype=np.int32)
+    params = np.ascontiguousarray([float(p) for _, p in stats], dtype=np.float64)
+    if device_pointers:
+        R, N = shape
+        check(load().gsp_ensemble_stats(ctx.handle, R, N, ptr(reals), len(kinds), ptr(kinds),
+                                        ptr(params), ptr(out), 1))
+        return out
+    reals = np.ascontiguousarray(reals, dtype=np.float64)
+    if reals.ndim != 2:
+        raise ArgumentError("reals must be (realizations, cells)")
+    R, N = reals.shape
+    res = np.empty((len(kinds), N)) if out is None else out
+    check(load().gsp_ensemble_stats(ctx.handle, R, N, ptr(reals), len(kinds), ptr(kinds),
+                                    ptr(params), ptr(res), 0))
+    return res

@@ -289,4 +289,4 @@ def rand(process, domain, nreals=None, *, rng=None, data=None, method=None, init
     if nreals is None:
         return GeoTable(randsingle(rng, process, smethod, domain, data, preproc), domain)
     out = _draw(rng, preproc, int(nreals))
-    return Ensemble(domain, {preproc.names[0]: out})
+    return Ensemble(domain, {preproc.names[0]: out}, ctx=ctx)

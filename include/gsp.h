@@ -159,7 +159,8 @@ enum {
   GSP_K_LEARNED = 7,     /* surrogate inference (learned plans) */
   GSP_K_FFT = 8,         /* FFTSIM transforms (rocFFT through hipFFT) */
   GSP_K_SPECTRAL = 9,    /* FFTSIM phase/amplitude, variance, scaling, conditioning */
-  GSP_K_COUNT = 10
+  GSP_K_STATS = 10,      /* ensemble statistics */
+  GSP_K_COUNT = 11
 };
 typedef struct {
   double ms[GSP_K_COUNT];       /* accumulated milliseconds */
@@ -378,6 +379,31 @@ int32_t gsp_fftsim_sample(gsp_ctx* ctx, const gsp_fftsim_plan* plan, int32_t R,
 int32_t gsp_fftsim_export(gsp_ctx* ctx, const gsp_fftsim_plan* plan, double* amplitude,
                           double* zbar);
 int32_t gsp_fftsim_plan_free(gsp_fftsim_plan* plan);
+
+/* ---- ensemble statistics -------------------------------------------------
+ * Per-cell statistics across the R realizations of an ensemble
+ * (ensembles.jl:42-52 through ereduce, :76-85), computed where the
+ * realizations already are (the R x N realization-major output of the
+ * samplers), several in one pass:
+ *   GSP_STAT_MEAN      Statistics.mean: left-to-right sum / R
+ *   GSP_STAT_VAR       Statistics.var (corrected, Welford's recurrence)
+ *   GSP_STAT_CDF       count(<= param) / R
+ *   GSP_STAT_CCDF      count(> param) / R
+ *   GSP_STAT_QUANTILE  Statistics.quantile(vals, param), type 7: h = fma(R, p,
+ *                      1 - p), j = clamp(trunc(h), 1, R - 1), g = clamp(h - j,
+ *                      0, 1), a + g (b - a) of the j-th and (j+1)-th smallest;
+ *                      NaN where a value is NaN (the reference throws)
+ * kinds, params: n_stats entries (1..32, host memory); out: n_stats x N. */
+typedef enum {
+  GSP_STAT_MEAN = 0,
+  GSP_STAT_VAR = 1,
+  GSP_STAT_CDF = 2,
+  GSP_STAT_CCDF = 3,
+  GSP_STAT_QUANTILE = 4
+} gsp_stat;
+int32_t gsp_ensemble_stats(gsp_ctx* ctx, int32_t R, int64_t N, const double* reals,
+                           int32_t n_stats, const int32_t* kinds, const double* params,
+                           double* out, int32_t device_pointers);
 
 #ifdef __cplusplus
 }

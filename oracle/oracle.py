@@ -384,6 +384,62 @@ def fftsim_conditional(zu, zbar, cells, kweights, mean):
     return zbar + zu - simple_kriging_apply(ids, lam, ok, zu[:, cells], mean)
 
 
+# ------------------------------------------------------ ensemble statistics ----
+# ensembles.jl:42-52 reduce with Julia's Statistics (stdlib of Julia 1.10, not
+# vendored): mean(itr) sums left to right and divides by the count; var(itr)
+# runs Welford's recurrence M_k = M + (x - M) / k, S += (x - M)(x - M_k) and
+# returns S / (n - 1); quantile(itr, p) sorts and, with alpha = beta = 1,
+# takes h = fma(n, p, 1 - p), j = clamp(trunc(h), 1, n - 1),
+# g = clamp(h - j, 0, 1), a + g (b - a) of the j-th and (j+1)-th values
+# (1-based), or (1 - g) a + g b when one is not finite.
+def _fma_exact(a, b, c):
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def ensemble_stat(reals, kind, param=0.0):
+    """One statistic of every column of `reals` (R, N), in the operation order
+    above (vectorised across cells, sequential across realizations)."""
+    v = np.asarray(reals, dtype=np.float64)
+    R = v.shape[0]
+    if kind == "mean":
+        acc = v[0].copy()
+        for r in range(1, R):
+            acc = acc + v[r]
+        return acc / float(R)
+    if kind == "var":
+        M = v[0].copy()
+        S = np.zeros_like(M)
+        for r in range(1, R):
+            nm = M + (v[r] - M) / float(r + 1)
+            S = S + (v[r] - M) * (v[r] - nm)
+            M = nm
+        with np.errstate(invalid="ignore", divide="ignore"):
+            return S / float(R - 1)
+    if kind == "cdf":
+        return np.sum(v <= param, axis=0) / float(R)
+    if kind == "ccdf":
+        return np.sum(v > param, axis=0) / float(R)
+    if kind == "quantile":
+        p = float(param)
+        m = 1.0 + p * (1.0 - 1.0 - 1.0)
+        h = _fma_exact(float(R), p, m)
+        srt = np.sort(v, axis=0)
+        if R == 1:
+            a = b = srt[0]
+            g = 0.0
+        else:
+            j = min(max(int(np.trunc(h)), 1), R - 1)
+            g = min(max(h - j, 0.0), 1.0)
+            a, b = srt[j - 1], srt[j]
+        fin = np.isfinite(a) & np.isfinite(b)
+        with np.errstate(invalid="ignore"):
+            out = np.where(fin, a + g * (b - a), (1.0 - g) * a + g * b)
+        out[np.isnan(v).any(axis=0)] = np.nan
+        return out
+    raise ValueError(kind)
+
+
 def seqsim_reference_cost(case: SeqsimCase, z: np.ndarray, threads=1, n_nodes_limit=0):
     """Search + solve per node per realization (seqsim.jl:98-138 cost structure)."""
     st = case.struct()

@@ -464,3 +464,43 @@ def test_fftsim_oracle_conditional_honours_centred_data():
     for c, v in zip(O.nearest_init(O.grid_centroids(dims, origin), coords), vals):
         np.testing.assert_allclose(z[:, c], v, atol=1e-9)
     assert not np.allclose(z[0], z[1])
+
+
+# ---------------------------------------------------- ensemble statistics ----
+def test_ensemble_stat_reference_testset():
+    """test/ensembles.jl:24-59: three constant realizations 1, 2, 3."""
+    reals = np.stack([i * np.ones(9) for i in (1.0, 2.0, 3.0)])
+    assert np.all(O.ensemble_stat(reals, "mean") == 2.0)
+    assert np.all(O.ensemble_stat(reals, "var") == 1.0)
+    for i in (1, 2, 3):
+        assert np.all(O.ensemble_stat(reals, "cdf", i) == i / 3)
+        np.testing.assert_allclose(O.ensemble_stat(reals, "ccdf", i), 1 - i / 3)
+    assert np.all(O.ensemble_stat(reals, "quantile", 0.5) == 2.0)
+    assert np.all(O.ensemble_stat(reals, "quantile", 0.0) == 1.0)
+    assert np.all(O.ensemble_stat(reals, "quantile", 1.0) == 3.0)
+
+
+@pytest.mark.parametrize("R", [1, 2, 7, 16, 33])
+def test_ensemble_stat_matches_numpy(R):
+    """Independent routes: numpy's pairwise mean, two-pass var, and the
+    'linear' (type 7) quantile agree to rounding; p = 0 gives the minimum
+    exactly (g = 0), p = 1 the second largest plus the gap to the largest
+    (g = 1), which is the maximum up to rounding."""
+    v = np.random.default_rng(R).standard_normal((R, 500)) * 3.0 + 1.0
+    np.testing.assert_allclose(O.ensemble_stat(v, "mean"), v.mean(axis=0), rtol=1e-13, atol=1e-14)
+    if R > 1:
+        np.testing.assert_allclose(O.ensemble_stat(v, "var"), v.var(axis=0, ddof=1), rtol=1e-12)
+    else:
+        assert np.all(np.isnan(O.ensemble_stat(v, "var")))
+    for p in (0.0, 0.1, 0.25, 0.5, 0.9, 1.0):
+        np.testing.assert_allclose(O.ensemble_stat(v, "quantile", p), np.quantile(v, p, axis=0),
+                                   rtol=1e-13, atol=1e-14)
+    assert np.array_equal(O.ensemble_stat(v, "quantile", 0.0), v.min(axis=0))
+    if R > 1:
+        srt = np.sort(v, axis=0)
+        assert np.array_equal(O.ensemble_stat(v, "quantile", 1.0), srt[-2] + (srt[-1] - srt[-2]))
+    np.testing.assert_allclose(O.ensemble_stat(v, "quantile", 1.0), v.max(axis=0), rtol=0,
+                               atol=1e-15 * np.abs(v).max())
+    np.testing.assert_array_equal(O.ensemble_stat(v, "cdf", 1.0), (v <= 1.0).mean(axis=0))
+    np.testing.assert_array_equal(O.ensemble_stat(v, "ccdf", 1.0) + O.ensemble_stat(v, "cdf", 1.0),
+                                  np.ones(500))
