@@ -46,6 +46,12 @@ uniform white noise resident in HBM. One step is one plan (covariance,
 amplitude spectrum) plus the 16 realizations (forward and inverse transforms,
 phase, variance scaling).
 
+Workload "ensemble-stats-32x256x256x128": per-cell statistics of an ensemble
+of 32 realizations on a 256 x 256 x 128 grid (2.1 GB, realization-major in
+HBM): mean, variance, cdf(0), ccdf(1) and the 5%, 50% and 95% quantiles in
+one pass. The same statistics without the quantiles (a pure streaming pass)
+are timed beside it.
+
 Prints one JSON line. `cpu_baseline` times the CPU restatement of the
 reference's cost structure (search + solve per node per realization) on a
 bounded sample of the same workload.
@@ -79,6 +85,8 @@ WORKLOADS = {
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01,
         n_data=100_000, kmax=10, radius=0.0, R=1, seed=1006, mode="data_only", precision="fp64",
         cpu_nodes=1_500_000),
+    "ensemble-stats-32x256x256x128": dict(dims=(256, 256, 128), R=32, seed=1008, method="stats",
+                                          cpu_cells=1_000_000),
     "fftsim-256x256x128": dict(dims=(256, 256, 128), family=2, sill=1.0, ranges=(20.0, 20.0, 20.0),
                                nugget=0.0, R=16, seed=1007, method="fftsim", cpu_reals=2),
 }
@@ -455,6 +463,75 @@ def bench_fftsim(args, spec):
     print(json.dumps(line), flush=True)
 
 
+def bench_stats(args, spec):
+    """Ensemble statistics: one gsp_ensemble_stats call per step."""
+    from geostatsprocesses import _lib
+    dims, R = spec["dims"], spec["R"]
+    N = int(np.prod(dims))
+    ctx = _lib.default_context(0)
+    reals = _lib.DeviceBuffer(ctx, (R, N))
+    for r in range(R):
+        row = np.random.default_rng(spec["seed"] + r).standard_normal(N)
+        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, reals.ptr + r * N * 8, row.ctypes.data, N * 8))
+    full = [(_lib.STAT_MEAN, 0.0), (_lib.STAT_VAR, 0.0), (_lib.STAT_CDF, 0.0), (_lib.STAT_CCDF, 1.0),
+            (_lib.STAT_QUANTILE, 0.05), (_lib.STAT_QUANTILE, 0.5), (_lib.STAT_QUANTILE, 0.95)]
+    moments = full[:4]
+    out = _lib.DeviceBuffer(ctx, (len(full), N))
+
+    def run(stats):
+        class Step:
+            def __init__(self):
+                _lib.ensemble_stats(ctx, reals, stats, out=out, device_pointers=True, shape=(R, N))
+                ctx.synchronize()
+
+            def close(self):
+                pass
+        return timed(ctx, Step, args.steps, args.warmup)
+
+    ms_full, tim_full = run(full)
+    ms_mom, tim_mom = run(moments)
+
+    def kern(tim, n_stats):
+        ms, launches = tim[_lib.K_STATS]
+        avg = ms / launches
+        nbytes = R * N * 8 + n_stats * N * 8
+        return {"avg_ms": round(avg, 4), "launches_per_step": launches // args.steps,
+                "algorithmic_bytes_per_launch": nbytes,
+                "achieved_GBs": round(nbytes / (avg * 1e-3) / 1e9, 1)}
+    kf, km = kern(tim_full, len(full)), kern(tim_mom, len(moments))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        n = spec["cpu_cells"]
+        sample = np.random.default_rng(0).standard_normal((R, n))
+        t0 = time.perf_counter()
+        for kind, p in (("mean", 0), ("var", 0), ("cdf", 0.0), ("ccdf", 1.0), ("quantile", 0.05),
+                        ("quantile", 0.5), ("quantile", 0.95)):
+            O.ensemble_stat(sample, kind, p)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(n * R / dt, 1), "unit": "cells*realizations/s", "cores": 1,
+               "kind": "port", "sample": f"the same 7 statistics over {n} cells x {R} realizations "
+                                         f"with numpy ({dt:.1f} s, one thread)"}
+    line = {
+        "metric": "ensemble statistics cells*realizations/s",
+        "value": round(N * R / (ms_full / 1e3), 1),
+        "unit": "cells*realizations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_full, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(dims), "realizations": R,
+                   "statistics": "mean, var, cdf(0), ccdf(1), quantile(0.05, 0.5, 0.95)"},
+        "roofline": {"bound": "hbm", "kernel": "k_ens_stats (moments only)",
+                     "achieved": km["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(km["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernels": {"stats_with_quantiles": kf, "stats_moments_only": km},
+        "moments_only_ms_per_step": round(ms_mom, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,6 +564,8 @@ def main():
     spec = WORKLOADS[args.workload]
     if spec.get("method") == "fftsim":
         return bench_fftsim(args, spec)
+    if spec.get("method") == "stats":
+        return bench_stats(args, spec)
     args.precision = args.precision or spec.get("precision", "fp32r")
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
     R = spec["R"]
