@@ -157,19 +157,21 @@ def problem(spec, path_ptr, data_ptr, precision, N):
 
 
 # kernels of each bench name in the committed traffic profiles
-PROFILE_KERNELS = {"search": ["k_search"], "krige": ["k_krige16<1>", "k_krige<24, 1>"],
-                   "sweep": ["k_sweep_level"]}
+PROFILE_KERNELS = {"search": ["k_search<false>", "k_search<true>"],
+                   "krige": ["k_krige16<1>", "k_krige<24, 1>", "k_krige16<0>", "k_krige<24, 0>"],
+                   "sweep": ["k_sweep_level"], "learned": ["k_mlp_infer"],
+                   "stats": ["k_ens_stats<false>"]}
 
 
 def profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` (read + written, counter-calibrated) from
+    """HBM bytes per launch of the kernel family `kernel` (read + written,
+    counter-calibrated, averaged over its dispatches) from
     profiles/<workload>_traffic.md, which rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
     runs of this script produced; None when no profile covers it."""
-    path = os.path.join(ROOT, "profiles", workload.replace("seqsim-", "seqsim_").replace("-", "_")
-                        + "_traffic.md")
+    path = os.path.join(ROOT, "profiles", workload.replace("-", "_") + "_traffic.md")
     if not os.path.exists(path):
         return None
-    total, found = 0.0, False
+    total, n = 0.0, 0
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) >= 4 and cells[0] in PROFILE_KERNELS.get(kernel, []):
@@ -178,13 +180,9 @@ def profiled_traffic(workload, kernel):
                 per = (float(cells[2]) + float(cells[3])) * 1e6
             except ValueError:
                 continue
-            # the profiled run made two plans and two sweeps (export + timed step)
-            total += per * dispatches / 2
-            found = True
-    if not found:
-        return None
-    launches = {"search": 1, "krige": 1, "sweep": 1}[kernel]
-    return int(total / launches)
+            total += per * dispatches
+            n += dispatches
+    return int(total / n) if n else None
 
 
 def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
@@ -296,7 +294,8 @@ def roofline_of(kern, workload):
     if dominant == "learned":
         return {"bound": "mfma", "kernel": dominant, "achieved": d["achieved_TFLOPs"],
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(d["achieved_TFLOPs"] / BF16_PEAK_TFLOPS, 4), "traffic": None}
+                "frac": round(d["achieved_TFLOPs"] / BF16_PEAK_TFLOPS, 4),
+                "traffic": profiled_traffic(workload, dominant)}
     return {"bound": "hbm", "kernel": dominant, "achieved": d["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(d["achieved_GBs"] / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic(workload, dominant)}
@@ -524,7 +523,8 @@ def bench_stats(args, spec):
                    "statistics": "mean, var, cdf(0), ccdf(1), quantile(0.05, 0.5, 0.95)"},
         "roofline": {"bound": "hbm", "kernel": "k_ens_stats (moments only)",
                      "achieved": km["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(km["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(km["achieved_GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(args.workload, "stats")},
         "kernels": {"stats_with_quantiles": kf, "stats_moments_only": km},
         "moments_only_ms_per_step": round(ms_mom, 3),
         "cpu_baseline": cpu,
@@ -542,6 +542,8 @@ def main():
                     help="kriging solves (default: fp64 for the posterior mean, fp32r otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the learned-vs-exact accuracy pass")
+    ap.add_argument("--epochs", type=int, default=None,
+                    help="surrogate training epochs (default: the workload's; fewer for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
@@ -605,6 +607,8 @@ def main():
     plan_arg = problems(precision)
     S, training, tol = None, None, None
     if "learned" in spec:
+        if args.epochs is not None:
+            spec = dict(spec, learned=dict(spec["learned"], epochs=args.epochs))
         S, training = train_surrogate(ctx, _lib, spec, precision)
         tol = spec["learned"]["tolerance"]
 

@@ -289,6 +289,10 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
 // ball) visit rings of hash cells; the other searches never lack a bound
 // (the offset table always yields kmax grid candidates), and keep the simpler
 // scan of every cell for that case.
+// MI355X dispatches consecutive workgroups round-robin to its 8 XCDs, each
+// with its own L2.
+constexpr int kXcds = 8;
+
 template <bool RING>
 __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   __shared__ int s_gidx[kWaves][64];
@@ -304,11 +308,15 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
 
   int cur = -1;
   Offset e0 = {0.0, 0, 0, 0, 0};
-  // RING: every wave takes a contiguous run of path positions, and carries
+  // RING: every wave takes a contiguous run of nodes, and carries
   // the k-th data distance of its previous node: if that node lies delta
   // away, the k nearest data of this one lie within (D_prev + delta), a bound
   // that replaces the ring search
-  int64_t q_begin = (int64_t)blockIdx.x * kWaves + w, q_end = A.NT, q_step = wstride;
+  // blocks of one XCD take consecutive runs of nodes (when the grid divides)
+  const int G = (int)gridDim.x;
+  const int64_t lb = G % kXcds == 0 ? (int64_t)(blockIdx.x % kXcds) * (G / kXcds) + blockIdx.x / kXcds
+                                    : (int64_t)blockIdx.x;
+  int64_t q_begin = lb * kWaves + w, q_end = A.NT, q_step = wstride;
   if constexpr (RING) {
     const int64_t chunk = (A.NT + wstride - 1) / wstride;
     q_begin = q_begin * chunk;
@@ -319,9 +327,11 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   double prev_k = -1.0;
   int pti[3] = {0, 0, 0};
   for (int64_t q = q_begin; q < q_end; q += q_step) {
-    const int64_t pv = A.path[q];
-    const int gi = __builtin_amdgcn_readfirstlane((int)(pv & 0xFFFFFFFF));
-    const int p = __builtin_amdgcn_readfirstlane((int)(pv >> 32));
+    // nodes in grid order (every node's search is independent of the others):
+    // consecutive wavefronts then test the positions of overlapping
+    // neighbourhoods, which share cache lines
+    const int gi = __builtin_amdgcn_readfirstlane((int)q);
+    const int p = __builtin_amdgcn_readfirstlane(const_view(A.node_req)[gi]);
     // the request, read through the scalar cache (the stores below cannot alias it)
     const auto& R = const_view(A.reqs)[p];
     if (p != cur) {  // the first 64 offsets serve every node of the request
@@ -329,7 +339,7 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
       e0 = Offset{0.0, 0, 0, 0, 0};
       if (lane < R.T) e0 = A.table[R.table_off + lane];
     }
-    const int pi = (int)q;
+    const int pi = __builtin_amdgcn_readfirstlane(const_view(A.pos)[gi]);  // its path position
     const int nx = R.dims[0], ny = R.dims[1], nz = R.dims[2];
     const int kmax = R.kmax;
     const int i = gi - R.node_off;
@@ -1180,7 +1190,6 @@ struct LevelArgs {
 // block b works on chunk (b % 8) * per + b / 8 of the level, so every XCD
 // sweeps one contiguous run of the level's (id-ordered) nodes and the
 // neighbour rows those nodes share stay in one L2. The grid is 8 * per blocks.
-constexpr int kXcds = 8;
 
 __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
   const int per = gridDim.x / kXcds;
