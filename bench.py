@@ -78,7 +78,7 @@ WORKLOADS = {
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
     "seqsim-learned-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,
-        kmax=24, radius=20.0, R=16, seed=1004, cpu_dims=(64, 64, 32),
+        kmax=24, radius=20.0, R=16, seed=1004, cpu_dims=(128, 128, 128),
         learned=dict(train_dims=(128, 128, 64), train_plans=3, samples=2_000_000, epochs=20,
                      batch=1024, lr=2e-3, tolerance=0.05)),
     "kriging-mean-256x256x128": dict(
@@ -189,7 +189,8 @@ def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
     """The oracle's reference-cost restatement (KD-free cell-hash search +
     Cholesky per node per realization), OpenMP over realizations.
 
-    Single workloads: one full realization per thread. Batches: the first
+    Single workloads: one full realization per thread, repeated until about
+    10 s of CPU time. Batches: the first
     requests up to `node_budget` nodes, with the workload's R realizations
     each (every path position costs differently, so samples are whole
     realizations)."""
@@ -209,10 +210,16 @@ def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
         N = case.n_nodes
         R = R_batch if R_batch is not None else threads
         z = np.random.default_rng(7).standard_normal((R, N))
-        t0 = time.perf_counter()
-        _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=node_limit or N)
-        dt_total += time.perf_counter() - t0
-        n_total += n
+        # single workloads repeat their realizations up to ~10 s of CPU time
+        reps = 0
+        while True:
+            t0 = time.perf_counter()
+            _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=node_limit or N)
+            dt_total += time.perf_counter() - t0
+            n_total += n
+            reps += 1
+            if R_batch is not None or node_limit or dt_total >= 10.0 or reps >= 4096:
+                break
         used += 1
         nodes += N
         if node_budget is None or nodes >= node_budget:
@@ -221,7 +228,7 @@ def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
         sample = (f"{threads} passes (one per thread) over the first {node_limit} nodes of the "
                   f"path ({dt_total:.1f} s; search + kriging solve per node, {threads} threads)")
     elif R_batch is None:
-        sample = (f"{threads} full realizations ({nodes} path nodes each) of the same workload "
+        sample = (f"{threads * reps} full realizations ({nodes} path nodes each) of the same workload "
                   f"({dt_total:.1f} s; search + kriging solve per node per realization, "
                   f"OpenMP over realizations, {threads} threads)")
     else:
