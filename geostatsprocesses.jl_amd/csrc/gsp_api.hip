@@ -20,7 +20,7 @@ void set_error(const char* fmt, ...) {
   g_last_error = buf;
 }
 
-static hipEvent_t take_event(gsp_ctx* ctx) {
+hipEvent_t take_event(gsp_ctx* ctx) {
   if (!ctx->event_pool.empty()) {
     hipEvent_t e = ctx->event_pool.back();
     ctx->event_pool.pop_back();
@@ -46,6 +46,10 @@ TimedLaunch::~TimedLaunch() {
     ctx->event_pool.push_back(a);
     ctx->event_pool.push_back(b);
   }
+}
+
+void record_timing(gsp_ctx* ctx, hipEvent_t a, hipEvent_t b, int slot) {
+  ctx->pending.push_back({a, b, slot});
 }
 
 void flush_timings(gsp_ctx* ctx) {
@@ -114,6 +118,8 @@ int32_t gsp_ctx_create(int32_t device, gsp_ctx** out) {
   ctx->pool->device = device;
   e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
   if (e != hipSuccess) {
     gsp::set_error("context creation failed: %s", hipGetErrorString(e));
     delete ctx;
@@ -137,7 +143,10 @@ int32_t gsp_ctx_destroy(gsp_ctx* ctx) {
     ctx->counters.release();
     gsp::release_timing_events(ctx);
     ctx->fft_plans.reset();
+    (void)hipStreamSynchronize(ctx->side);
     ctx->pool->trim();
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
   }
   delete ctx;

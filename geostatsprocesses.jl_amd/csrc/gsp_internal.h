@@ -153,6 +153,10 @@ struct DevBuf {
 struct gsp_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for work that runs beside the main
+  // stream inside one call, forked from it with `fork`
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr;
   std::mutex mu;
   std::shared_ptr<gsp::Pool> pool;  // first to be built, last to go
   // reusable workspace
@@ -186,6 +190,10 @@ struct TimedLaunch {
   ~TimedLaunch();
 };
 void flush_timings(gsp_ctx* ctx);
+// An event from the context's pool, and a measured interval [a, b] of work
+// recorded on another stream (adds to the totals at the next flush).
+hipEvent_t take_event(gsp_ctx* ctx);
+void record_timing(gsp_ctx* ctx, hipEvent_t a, hipEvent_t b, int slot);
 void release_timing_events(gsp_ctx* ctx);
 }  // namespace gsp
 

@@ -53,11 +53,16 @@ struct NodeOut {
   double* sigma;
   uint8_t* flags;
   int32_t* counters;
-  int32_t* outdeg;
-  int32_t* indeg;
 };
 
-enum { CTR_PRIOR = 2, CTR_FALLBACK = 3, CTR_POS_ERR = 5, CTR_LEARNED = 6, CTR_EXACT = 7 };
+enum {
+  CTR_PRIOR = 2,
+  CTR_FALLBACK = 3,
+  CTR_POS_ERR = 5,
+  CTR_LEARNED = 6,
+  CTR_EXACT = 7,
+  CTR_FAIL = 8  // kriging systems that were not positive definite
+};
 
 // A table the kernels only read (request descriptors), viewed through the
 // constant address space so that wave-uniform reads become scalar loads.
@@ -94,7 +99,7 @@ namespace gsp {
 
 // The learned path of the plan builder (gsp_surrogate.hip): runs the
 // surrogate on every node of the eligible requests, writes the accepted
-// nodes' weights, sigmas, flags and dependency degrees, and lists the other
+// nodes' weights, sigmas and flags, and lists the other
 // nodes in `exact_list` (count in counters[CTR_EXACT]).
 void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, const NodeOut& out, double tolerance,
                      int32_t* exact_list);

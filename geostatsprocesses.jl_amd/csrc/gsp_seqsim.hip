@@ -67,8 +67,6 @@ struct PlanArgs {
   double* sigma;
   uint8_t* flags;
   int32_t* counters;
-  int32_t* outdeg;  // dependents per node (atomic counts)
-  int32_t* indeg;   // grid neighbours per node
 };
 
 // Squared physical distance between a datum and the centre of node t.
@@ -687,15 +685,11 @@ __global__ __launch_bounds__(64 * plan_waves(KB)) void k_krige(const PlanArgs A,
       }
     }
     if (lane < A.K) A.lam[(int64_t)gi * A.K + lane] = lane < n ? lam : 0.0;
-    // dependency degrees for the level computation (grid neighbours only)
-    const bool dep = !(flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) && lane < n && nb < A.NT;
-    if (dep) atomicAdd(&A.outdeg[nb], 1);
-    const int ndep = __popcll(__ballot(dep));
     if (lane == 0) {
       A.sigma[gi] = sig;
       A.flags[gi] = (uint8_t)flag;
-      A.indeg[gi] = ndep;
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
+      if (flag & GSP_FLAG_FAIL) atomicAdd(&A.counters[CTR_FAIL], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
   }
@@ -942,14 +936,11 @@ __global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t
     if (act) {
       for (int col = r; col < A.K; col += 16) A.lam[(int64_t)gi * A.K + col] = col < n ? lam : 0.0;
     }
-    const bool dep = solve && !(flag & GSP_FLAG_FAIL) && r < n && nb < A.NT;
-    if (dep) atomicAdd(&A.outdeg[nb], 1);
-    const uint64_t dm = __ballot(dep);
     if (act && r == 0) {
       A.sigma[gi] = sig;
       A.flags[gi] = (uint8_t)flag;
-      A.indeg[gi] = __popcll((dm >> (16 * g)) & 0xFFFFull);
       if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
+      if (flag & GSP_FLAG_FAIL) atomicAdd(&A.counters[CTR_FAIL], 1);
       if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
     }
   }
@@ -1056,32 +1047,81 @@ __global__ void k_positions(const int64_t* path, int64_t N, int32_t* pos, int32_
 // grid neighbours, else 1 + the largest level of its grid neighbours; the
 // nodes of level l are exactly the frontier after l rounds of removing
 // finished nodes. Reverse edges (neighbour -> dependent) live in CSR form.
-__device__ inline int deps_of(const uint8_t* flags, const int32_t* nfound, int i) {
-  return (flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : nfound[i];
+//
+// A node draws from the prior, and so depends on nothing, when it has fewer
+// than kmin neighbours (known after the search) or its kriging system is not
+// positive definite (known after the solve). The levels are built from the
+// search alone (FLAGS = false), beside the solves; a plan with failed systems
+// builds them again from the flags (FLAGS = true).
+struct DepArgs {
+  int32_t N, K;
+  const Req* reqs;
+  const int32_t* node_req;
+  const int32_t* nfound;
+  const uint8_t* flags;
+  const int32_t* nbr;
+};
+
+template <bool FLAGS>
+__device__ inline int deps_of(const DepArgs& D, int i) {
+  if constexpr (FLAGS) return (D.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) ? 0 : D.nfound[i];
+  const int n = D.nfound[i];
+  return n < D.reqs[D.node_req[i]].kmin ? 0 : n;
+}
+
+// Grid neighbours per node (indeg) and dependents per node (outdeg, atomic
+// counts; data neighbours have ids >= N and are not edges).
+template <bool FLAGS>
+__global__ void k_deg_count(const DepArgs D, int32_t* outdeg, int32_t* indeg,
+                            unsigned long long* edges) {
+  uint32_t total = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D.N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = deps_of<FLAGS>(D, (int)i);
+    const int32_t* nb = D.nbr + i * D.K;
+    int c = 0;
+    for (int j0 = 0; j0 < k; j0 += 8) {
+      int a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = j0 + u < k ? nb[j0 + u] : D.N;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (a[u] < D.N) {
+          atomicAdd(&outdeg[a[u]], 1);
+          ++c;
+        }
+    }
+    indeg[i] = c;
+    total += (uint32_t)c;
+  }
+  // the number of edges: one atomic per wavefront
+  unsigned long long w = total;
+  for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+  if ((threadIdx.x & 63) == 0 && w) atomicAdd(edges, w);
 }
 
 // Reverse edges in CSR form; each thread takes the edges of one node eight
 // at a time so that their atomics are in flight together.
-__global__ void k_dep_fill(int32_t N, int32_t K, const int32_t* nfound, const uint8_t* flags,
-                           const int32_t* nbr, const int32_t* rowptr, int32_t* cursor,
+template <bool FLAGS>
+__global__ void k_dep_fill(const DepArgs D, const int32_t* rowptr, int32_t* cursor,
                            int32_t* deps) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D.N;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int k = deps_of(flags, nfound, (int)i);
-    const int32_t* nb = nbr + i * K;
+    const int k = deps_of<FLAGS>(D, (int)i);
+    const int32_t* nb = D.nbr + i * D.K;
     for (int j0 = 0; j0 < k; j0 += 8) {
       int a[8], slot[8], base[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = j0 + u < k ? nb[j0 + u] : N;
+      for (int u = 0; u < 8; ++u) a[u] = j0 + u < k ? nb[j0 + u] : D.N;
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (a[u] < N) {
+        if (a[u] < D.N) {
           slot[u] = atomicAdd(&cursor[a[u]], 1);
           base[u] = rowptr[a[u]];
         }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
-        if (a[u] < N) deps[base[u] + slot[u]] = (int)i;
+        if (a[u] < D.N) deps[base[u] + slot[u]] = (int)i;
     }
   }
 }
@@ -1641,6 +1681,121 @@ void parallel_requests(int32_t n, F&& f) {
 
 }  // namespace
 
+namespace {
+
+// Buffers of the level computation, all made before the worker starts (the
+// context's allocator hands out blocks in main-stream order only).
+struct LevelBufs {
+  DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
+  DevBuf<char> tmp;
+  size_t tmp_bytes = 0;
+};
+
+void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* outdeg,
+                      hipStream_t s) {
+  B.rowptr.alloc((size_t)NT + 1);
+  B.cursor.alloc((size_t)NT);
+  B.cnt.alloc((size_t)NT + 16);
+  B.off.alloc((size_t)NT + 16);
+  B.deps.alloc((size_t)std::max<int64_t>(n_edges, 1));
+  B.tmp_bytes = 0;
+  GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, B.tmp_bytes, outdeg, B.rowptr.p,
+                                           (int)NT + 1, s));
+  B.tmp.alloc(B.tmp_bytes);
+}
+
+// Kahn's algorithm on stream `st`: reverse CSR, level 0, then two launches
+// per level; the frontier sizes are checked every kBatch levels. Sets
+// plan->level, order, depth and level_off.
+template <bool FLAGS>
+void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg,
+                  const int32_t* indeg_in, LevelBufs& B, hipStream_t st) {
+  int32_t* indeg = const_cast<int32_t*>(indeg_in);
+  const int64_t NT = D.N;
+  const int nb = grid_for(NT, 256, 4096);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 16, 256, 4096)), dim3(256), 0, st, B.cnt.p,
+                     0, NT + 16);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, B.off.p, 0, (int64_t)1);
+  size_t tmp_bytes = B.tmp_bytes;
+  GSP_HIP(hipcub::DeviceScan::ExclusiveSum(B.tmp.p, tmp_bytes, outdeg, B.rowptr.p, (int)NT + 1,
+                                           st));
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, st, B.cursor.p, 0, NT);
+  hipLaunchKernelGGL(k_dep_fill<FLAGS>, dim3(nb), dim3(256), 0, st, D, B.rowptr.p, B.cursor.p,
+                     B.deps.p);
+  const int cb = grid_for(NT, 2048, 1024);
+  hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, -1, indeg,
+                     plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+  constexpr int kBatch = 16;
+  const int fb = grid_for(NT / 16, 256, 1024);
+  int32_t depth = -1;
+  std::vector<int32_t> hc(kBatch + 1);
+  for (int32_t L = 0; depth < 0; L += kBatch) {
+    GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
+    for (int b = 0; b < kBatch; ++b) {
+      hipLaunchKernelGGL(k_level_push, dim3(fb), dim3(256), 0, st, L + b, B.rowptr.p, B.deps.p,
+                         indeg, plan->order.p, B.off.p, B.cnt.p);
+      hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, L + b, indeg,
+                         plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+    }
+    GSP_HIP(hipMemcpyAsync(hc.data(), B.cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, st));
+    GSP_HIP(hipStreamSynchronize(st));
+    for (int b = 0; b <= kBatch; ++b)
+      if (hc[(size_t)b] == 0) {
+        depth = L + b;
+        break;
+      }
+  }
+  GSP_REQUIRE(depth >= 1, GSP_ERR_INTERNAL, "empty dependency levels");
+  std::vector<int32_t> hcnt((size_t)depth);
+  GSP_HIP(hipMemcpyAsync(hcnt.data(), B.cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, st));
+  GSP_HIP(hipStreamSynchronize(st));
+  plan->depth = depth;
+  plan->level_off.assign((size_t)depth + 1, 0);
+  for (int32_t l = 0; l < depth; ++l)
+    plan->level_off[(size_t)l + 1] = plan->level_off[(size_t)l] + hcnt[(size_t)l];
+  GSP_REQUIRE(plan->level_off[(size_t)depth] == NT, GSP_ERR_INTERNAL,
+              "dependency levels cover %lld of %lld nodes",
+              (long long)plan->level_off[(size_t)depth], (long long)NT);
+}
+
+struct LevelJob {
+  int32_t code = GSP_OK;
+  std::string msg;
+  hipEvent_t a = nullptr, b = nullptr;  // timing (when enabled)
+};
+
+// The worker: the levels from the search on the side stream, after the fork.
+void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D,
+                    const int32_t* outdeg, const int32_t* indeg, LevelBufs& B, LevelJob& job) {
+  try {
+    GSP_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->side;
+    GSP_HIP(hipStreamWaitEvent(st, ctx->fork, 0));
+    if (job.a) GSP_HIP(hipEventRecord(job.a, st));
+    build_levels<false>(plan, D, outdeg, indeg, B, st);
+    if (job.b) GSP_HIP(hipEventRecord(job.b, st));
+    GSP_HIP(hipStreamSynchronize(st));
+  } catch (const Fail& f) {
+    job.code = f.code;
+    job.msg = gsp_last_error();
+  } catch (...) {
+    job.code = GSP_ERR_INTERNAL;
+    job.msg = "dependency level worker failed";
+  }
+}
+
+// A thread joined when it goes out of scope (also when the caller throws).
+struct WorkerThread {
+  std::thread t;
+  template <class F>
+  explicit WorkerThread(F&& f) : t(std::forward<F>(f)) {}
+  ~WorkerThread() {
+    if (t.joinable()) t.join();
+  }
+};
+
+}  // namespace
+
 void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
                        gsp_seqsim_plan* plan, const gsp_surrogate* surrogate, double tolerance) {
   GSP_REQUIRE(problems, GSP_ERR_ARGUMENT, "null problem");
@@ -1924,8 +2079,6 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   A.sigma = plan->sigma.p;
   A.flags = plan->flags.p;
   A.counters = ctx->counters.p;
-  A.outdeg = outdeg.p;
-  A.indeg = indeg.p;
 
   {
     TimedLaunch tl(ctx, GSP_K_SEARCH);
@@ -1936,6 +2089,38 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     GSP_HIP(hipGetLastError());
   }
   tr.mark("search kernel");
+
+  // dependency levels from the search, on the side stream in a worker thread
+  // while this thread runs the solves on the main stream (the level kernels
+  // are latency-bound, the solves compute-bound; they touch disjoint arrays)
+  DepArgs D;
+  D.N = (int32_t)NT;
+  D.K = K;
+  D.reqs = dreqs.p;
+  D.node_req = plan->node_req.p;
+  D.nfound = plan->nfound.p;
+  D.flags = plan->flags.p;
+  D.nbr = plan->nbr.p;
+  DevBuf<unsigned long long> edges;
+  edges.alloc(1);
+  GSP_HIP(hipMemsetAsync(edges.p, 0, 8, s));
+  hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
+                     outdeg.p, indeg.p, edges.p);
+  unsigned long long n_edges = 0;
+  GSP_HIP(hipMemcpyAsync(&n_edges, edges.p, 8, hipMemcpyDeviceToHost, s));
+  GSP_HIP(hipStreamSynchronize(s));
+  GSP_REQUIRE(n_edges < (1ull << 31), GSP_ERR_UNSUPPORTED,
+              "too many dependencies (%llu) for the level computation", n_edges);
+  LevelBufs LB;
+  level_bufs_alloc(LB, NT, (int64_t)n_edges, outdeg.p, s);  // every buffer the worker uses
+  GSP_HIP(hipEventRecord(ctx->fork, s));
+  LevelJob job;
+  if (ctx->timing) {
+    job.a = take_event(ctx);
+    job.b = take_event(ctx);
+  }
+  {
+    WorkerThread worker([&] { levels_on_side(ctx, plan, D, outdeg.p, indeg.p, LB, job); });
   {
     // learned plans: the surrogate takes the nodes it can, the rest are listed
     // for the exact solver
@@ -1956,8 +2141,6 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       O.sigma = plan->sigma.p;
       O.flags = plan->flags.p;
       O.counters = ctx->counters.p;
-      O.outdeg = outdeg.p;
-      O.indeg = indeg.p;
       {
         TimedLaunch tl(ctx, GSP_K_LEARNED);
         surrogate_infer(ctx, surrogate, O, tolerance, exact_ids.p);
@@ -2012,69 +2195,25 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     GSP_HIP(hipGetLastError());
   }
   tr.mark("kriging kernel");
-  {
+  }  // joins the worker
+  if (job.a) record_timing(ctx, job.a, job.b, GSP_K_LEVELS);
+  GSP_REQUIRE(job.code == GSP_OK, job.code, "%s", job.msg.c_str());
+  int32_t n_fail = 0;
+  GSP_HIP(hipMemcpyAsync(&n_fail, ctx->counters.p + CTR_FAIL, 4, hipMemcpyDeviceToHost, s));
+  GSP_HIP(hipStreamSynchronize(s));
+  if (n_fail > 0) {
+    // failed systems draw from the prior: their grid neighbours are no
+    // dependencies, so the levels are built again from the flags
     TimedLaunch tl(ctx, GSP_K_LEVELS);
-    DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
-    rowptr.alloc((size_t)NT + 1);
-    cnt.alloc((size_t)NT + 16);
-    off.alloc((size_t)NT + 16);
-    const int nb = grid_for(NT, 256, 4096);
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 16, 256, 4096)), dim3(256), 0, s, cnt.p, 0,
-                       NT + 16);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, off.p, 0, (int64_t)1);
-    size_t tmp_bytes = 0;
-    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, outdeg.p, rowptr.p, (int)NT + 1,
-                                             s));
-    DevBuf<char> tmp;
-    tmp.alloc(tmp_bytes);
-    GSP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, tmp_bytes, outdeg.p, rowptr.p, (int)NT + 1, s));
-    int32_t n_edges = 0;
-    GSP_HIP(hipMemcpyAsync(&n_edges, rowptr.p + NT, 4, hipMemcpyDeviceToHost, s));
-    GSP_HIP(hipStreamSynchronize(s));
-    tr.mark("levels: count+scan");
-    deps.alloc((size_t)std::max(n_edges, 1));
-    cursor.alloc((size_t)NT);
-    hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(256), 0, s, cursor.p, 0, NT);
-    hipLaunchKernelGGL(k_dep_fill, dim3(nb), dim3(256), 0, s, (int32_t)NT, K, plan->nfound.p,
-                       plan->flags.p, plan->nbr.p, rowptr.p, cursor.p, deps.p);
-    const int cb = grid_for(NT, 2048, 1024);
-    hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, s, (int32_t)NT, -1, indeg.p,
-                       plan->level.p, plan->order.p, off.p, cnt.p);
-    tr.mark("levels: fill");
-    // two launches per level; the frontier sizes are checked every kBatch levels
-    constexpr int kBatch = 16;
-    const int fb = grid_for(NT / 16, 256, 1024);
-    int32_t depth = -1;
-    std::vector<int32_t> hc(kBatch + 1);
-    for (int32_t L = 0; depth < 0; L += kBatch) {
-      GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
-      for (int b = 0; b < kBatch; ++b) {
-        hipLaunchKernelGGL(k_level_push, dim3(fb), dim3(256), 0, s, L + b, rowptr.p, deps.p,
-                           indeg.p, plan->order.p, off.p, cnt.p);
-        hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, s, (int32_t)NT, L + b, indeg.p,
-                           plan->level.p, plan->order.p, off.p, cnt.p);
-      }
-      GSP_HIP(hipMemcpyAsync(hc.data(), cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, s));
-      GSP_HIP(hipStreamSynchronize(s));
-      for (int b = 0; b <= kBatch; ++b)
-        if (hc[(size_t)b] == 0) {
-          depth = L + b;
-          break;
-        }
-    }
-    tr.mark("levels: frontiers");
-    GSP_REQUIRE(depth >= 1, GSP_ERR_INTERNAL, "empty dependency levels");
-    std::vector<int32_t> hcnt((size_t)depth);
-    GSP_HIP(hipMemcpyAsync(hcnt.data(), cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, s));
-    GSP_HIP(hipStreamSynchronize(s));
-    plan->depth = depth;
-    plan->level_off.assign((size_t)depth + 1, 0);
-    for (int32_t l = 0; l < depth; ++l)
-      plan->level_off[(size_t)l + 1] = plan->level_off[(size_t)l] + hcnt[(size_t)l];
-    GSP_REQUIRE(plan->level_off[(size_t)depth] == NT, GSP_ERR_INTERNAL,
-                "dependency levels cover %lld of %lld nodes",
-                (long long)plan->level_off[(size_t)depth], (long long)NT);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 1, 256, 4096)), dim3(256), 0, s, outdeg.p,
+                       0, NT + 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
+                       plan->level.p, -1, NT);
+    hipLaunchKernelGGL(k_deg_count<true>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
+                       outdeg.p, indeg.p, edges.p);
+    build_levels<true>(plan, D, outdeg.p, indeg.p, LB, s);
   }
+  tr.mark("levels");
   int32_t ctr[16];
   GSP_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
   GSP_HIP(hipStreamSynchronize(s));

@@ -455,9 +455,9 @@ struct InferArgs {
   int32_t* exact_list;
 };
 
-constexpr int kInferWaves = 4;
+constexpr int kInferWaves = 8;  // 256 nodes per pass of the weights through LDS
 
-// Weight fragments of output tiles, staged in LDS for the block's four
+// Weight fragments of output tiles, staged in LDS for the block's
 // wavefronts: three buffers, so that the copy of tile rt + 2 runs while tile
 // rt is multiplied and tile rt - 1 goes through its activation.
 struct FragLds {
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(const InferArgs A) {
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
   __shared__ FragLds s_frag;
   __shared__ uint32_t s_mask[kInferWaves][32];
   __shared__ int8_t s_slot[kInferWaves][32][kSlots];
@@ -656,8 +656,6 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const InferArgs A) {
           O.lam[i * O.K + j] = j < n ? (double)out[s_slot[w][lane][j]] : 0.0;
         O.sigma[i] = sqrt(R.M.sill * (double)pred);
         O.flags[i] = (uint8_t)GSP_FLAG_LEARNED;
-        O.indeg[i] = n;
-        for (int j = 0; j < n; ++j) atomicAdd(&O.outdeg[O.nbr[i * O.K + j]], 1);
         atomicAdd(&O.counters[CTR_LEARNED], 1);
         state = 2;
       } else {
@@ -727,7 +725,7 @@ void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, const NodeOut& out, d
   A.exact_list = exact_list;
   const int64_t blocks = (out.NT + kInferWaves * 32 - 1) / (kInferWaves * 32);
   GSP_REQUIRE(blocks < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "too many nodes");
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, A);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0, ctx->stream, A);
   GSP_HIP(hipGetLastError());
 }
 
@@ -746,8 +744,6 @@ NodeOut node_view(const gsp_seqsim_plan* p) {
   O.sigma = p->sigma.p;
   O.flags = p->flags.p;
   O.counters = nullptr;
-  O.outdeg = nullptr;
-  O.indeg = nullptr;
   return O;
 }
 
