@@ -1,4 +1,5 @@
 // gsp_api.cpp — the C ABI of libgsp_hip.so (declared in include/gsp.h).
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <new>
@@ -413,6 +414,49 @@ int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, c
       GSP_HIP(hipMemcpyAsync(out, ctx->stage_out.p, nz * 8, hipMemcpyDeviceToHost, s));
     }
     if (!device_pointers) {
+      GSP_HIP(hipStreamSynchronize(s));
+      gsp::flush_timings(ctx);
+    }
+  });
+}
+
+int32_t gsp_seqsim_sweep_indicator(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                   int32_t n_cat, const double* prob, const double* u,
+                                   const int32_t* data_cat, int32_t* out,
+                                   int32_t device_pointers) {
+  if (!plan) {
+    gsp::set_error("null plan");
+    return GSP_ERR_ARGUMENT;
+  }
+  return gsp_guard(ctx, [&] {
+    GSP_REQUIRE(R >= 1, GSP_ERR_ARGUMENT, "number of realizations must be positive");
+    GSP_REQUIRE(u && out && prob, GSP_ERR_ARGUMENT, "null prob, u or output buffer");
+    GSP_REQUIRE(plan->n_data == 0 || data_cat, GSP_ERR_ARGUMENT, "null data categories");
+    double sum = 0.0;
+    for (int c = 0; c < n_cat && c < 64; ++c) {
+      GSP_REQUIRE(prob[c] >= 0.0 && prob[c] <= 1.0, GSP_ERR_ASSERTION,
+                  "probabilities must be in [0, 1]");
+      sum += prob[c];
+    }
+    GSP_REQUIRE(std::fabs(sum - 1.0) <= 1e-8, GSP_ERR_ASSERTION,
+                "probabilities must sum up to one");
+    const size_t n = (size_t)R * (size_t)plan->N;
+    hipStream_t s = ctx->stream;
+    if (device_pointers) {
+      gsp::seqsim_sweep_indicator_run(ctx, plan, R, n_cat, prob, u, data_cat, out);
+    } else {
+      ctx->stage_in.reserve(n);
+      ctx->stage_out.reserve((n + 1) / 2);  // int32 output
+      gsp::DevBuf<int32_t> dcat;
+      if (plan->n_data > 0) {
+        dcat.alloc((size_t)plan->n_data);
+        GSP_HIP(hipMemcpyAsync(dcat.p, data_cat, (size_t)plan->n_data * 4, hipMemcpyHostToDevice,
+                               s));
+      }
+      GSP_HIP(hipMemcpyAsync(ctx->stage_in.p, u, n * 8, hipMemcpyHostToDevice, s));
+      int32_t* dout = (int32_t*)ctx->stage_out.p;
+      gsp::seqsim_sweep_indicator_run(ctx, plan, R, n_cat, prob, ctx->stage_in.p, dcat.p, dout);
+      GSP_HIP(hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, s));
       GSP_HIP(hipStreamSynchronize(s));
       gsp::flush_timings(ctx);
     }

@@ -369,6 +369,9 @@ void surrogate_export(gsp_ctx* ctx, const gsp_surrogate* s, float* params, int64
                       double* slots);
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out);
+void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                int32_t n_cat, const double* prob, const double* u,
+                                const int32_t* data_cat, int32_t* out);
 void lusim_factor_build(gsp_ctx* ctx, const gsp_lusim_problem* p, gsp_lusim_plan* plan);
 void lusim_sample_run(gsp_ctx* ctx, const gsp_lusim_plan* plan, int32_t R, const double* w,
                       double* out);

@@ -1264,6 +1264,106 @@ __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
   A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * A.Zt[(int64_t)A.pos[i] * A.RC + r];
 }
 
+// Indicator sweep (an IndicatorProcess with an intrinsic coregionalization
+// A * c(h): the plan's weights lambda_j serve every category). X holds the
+// 0-based categories as doubles, Zt the uniforms of the categorical draws.
+constexpr int kMaxCat = 8;
+
+struct IndicatorArgs {
+  LevelArgs L;
+  int32_t n_cat;
+  int32_t pad_;
+  double prob[kMaxCat];
+};
+
+// The first category c whose running sum q_0 + ... + q_c exceeds u (the last
+// one at most): the categorical draw of a probability vector.
+__device__ inline int draw_category(const double (&q)[kMaxCat], int n_cat, double u) {
+  int cat = 0;
+  double cp = q[0];
+  bool more = cp <= u;
+#pragma unroll
+  for (int c = 1; c < kMaxCat; ++c)
+    if (c < n_cat && more) {
+      cat = c;
+      cp = cp + q[c];
+      more = cp <= u;
+    }
+  return cat;
+}
+
+// Per node and realization: p_c + sum_j lambda_j ([x_j == c] - p_c) for every
+// category (neighbours in order), clamped to [0, 1], multiplied by the
+// inverse of their sum (normalize(., 1)), then the categorical draw; nodes
+// that draw from the prior use the process probabilities.
+__global__ __launch_bounds__(256) void k_sweep_level_ind(const IndicatorArgs I) {
+  const LevelArgs& A = I.L;
+  const int per = gridDim.x / kXcds;
+  const int64_t blk = (int64_t)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  const int64_t t = blk * blockDim.x + threadIdx.x;
+  const int64_t slot = t >> A.shift;
+  const int r = (int)(t & ((1 << A.shift) - 1));
+  if (slot >= A.count || r >= A.RC) return;
+  const int i = A.order[slot];
+  const bool prior = (A.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) != 0;
+  const int k = prior ? 0 : A.nfound[i];
+  const int32_t* nb = A.nbr + (int64_t)i * A.K;
+  const double* lm = A.lam + (int64_t)i * A.K;
+  const int n_cat = I.n_cat;
+  double q[kMaxCat];
+#pragma unroll
+  for (int c = 0; c < kMaxCat; ++c) q[c] = I.prob[c];
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    int nj[8];
+    double lj[8], xj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      nj[u] = j0 + u < k ? nb[j0 + u] : 0;
+      lj[u] = j0 + u < k ? lm[j0 + u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xj[u] = j0 + u < k ? A.X[(int64_t)nj[u] * A.RC + r] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j0 + u < k) {
+#pragma unroll
+        for (int c = 0; c < kMaxCat; ++c)
+          if (c < n_cat) q[c] = q[c] + lj[u] * ((xj[u] == (double)c ? 1.0 : 0.0) - I.prob[c]);
+      }
+  }
+  if (!prior) {
+    double sum = 0.0;
+#pragma unroll
+    for (int c = 0; c < kMaxCat; ++c)
+      if (c < n_cat) {
+        const double x = q[c];
+        q[c] = x > 1.0 ? 1.0 : (x < 0.0 ? 0.0 : x);  // clamp(x, 0, 1)
+        sum = c == 0 ? fabs(q[c]) : sum + fabs(q[c]);
+      }
+    // normalize(q, 1): multiply by the inverse norm, through a scaling when
+    // the norm is below 1 / floatmax
+    const double delta = 1.0 / 0x1.fffffffffffffp+1023;
+    if (sum >= delta) {
+      const double inv = 1.0 / sum;
+#pragma unroll
+      for (int c = 0; c < kMaxCat; ++c) q[c] = q[c] * inv;
+    } else {
+      const double ed = 0x1p-52 / delta;
+      const double inv = 1.0 / (sum * ed);
+#pragma unroll
+      for (int c = 0; c < kMaxCat; ++c) q[c] = (q[c] * ed) * inv;
+    }
+  }
+  const double u = A.Zt[(int64_t)A.pos[i] * A.RC + r];
+  A.X[(int64_t)i * A.RC + r] = (double)draw_category(q, n_cat, u);
+}
+
+__global__ void k_i32_to_f64(const int32_t* in, double* out, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = (double)in[t];
+}
+
 // X rows NT..NT+n_data-1 hold the conditioning data for every realization.
 __global__ void k_data_rows(double* X, int32_t N, int32_t n_data, int32_t RC,
                             const double* vals) {
@@ -1273,9 +1373,11 @@ __global__ void k_data_rows(double* X, int32_t N, int32_t n_data, int32_t RC,
     X[(int64_t)N * RC + t] = vals[t / RC];
 }
 
-// out[(r0 + r) * N + i] = X[i * RC + r] through a 64-node LDS tile.
+// out[(r0 + r) * N + i] = X[i * RC + r] through a 64-node LDS tile (int32
+// outputs: the indicator sweep's categories, held as exact doubles in X).
+template <class OUT>
 __global__ __launch_bounds__(256) void k_transpose(const double* X, int32_t N, int32_t RC,
-                                                   int32_t r0, double* out) {
+                                                   int32_t r0, OUT* out) {
   __shared__ double tile[64][65];
   const int64_t i0 = (int64_t)blockIdx.x * 64;
   const int cnt = (int)((N - i0) < 64 ? (N - i0) : 64);
@@ -1286,7 +1388,7 @@ __global__ __launch_bounds__(256) void k_transpose(const double* X, int32_t N, i
   __syncthreads();
   for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
     const int rr = t / cnt, ii = t - rr * cnt;
-    out[(int64_t)(r0 + rr) * N + i0 + ii] = tile[rr][ii];
+    out[(int64_t)(r0 + rr) * N + i0 + ii] = (OUT)tile[rr][ii];
   }
 }
 
@@ -2290,8 +2392,78 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
     }
     {
       TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, ctx->X.p,
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, ctx->X.p,
                          (int32_t)N, RC, r0, out);
+    }
+    GSP_HIP(hipGetLastError());
+  }
+}
+
+
+void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                int32_t n_cat, const double* prob, const double* u,
+                                const int32_t* data_cat, int32_t* out) {
+  GSP_REQUIRE(n_cat >= 2 && n_cat <= kMaxCat, GSP_ERR_UNSUPPORTED,
+              "indicator simulation supports 2 to %d categories", kMaxCat);
+  hipStream_t s = ctx->stream;
+  const int64_t N = plan->N, nD = plan->n_data;
+  const int RCmax = sweep_chunk(N + nD, R);
+  const int RC0 = R < RCmax ? R : RCmax;
+  ctx->X.reserve((size_t)((N + nD) * RC0));
+  ctx->Zt.reserve((size_t)(N * RC0));
+  DevBuf<double> dcat;
+  if (nD > 0) {
+    dcat.alloc((size_t)nD);
+    hipLaunchKernelGGL(k_i32_to_f64, dim3(grid_for(nD, 256, 1024)), dim3(256), 0, s, data_cat,
+                       dcat.p, nD);
+  }
+  IndicatorArgs I;
+  std::memset(&I, 0, sizeof(I));
+  I.n_cat = n_cat;
+  for (int c = 0; c < n_cat; ++c) I.prob[c] = prob[c];
+  for (int r0 = 0; r0 < R; r0 += RCmax) {
+    const int RC = (R - r0) < RCmax ? (R - r0) : RCmax;
+    int shift = 0;
+    while ((1 << shift) < RC) ++shift;
+    if (nD > 0)
+      hipLaunchKernelGGL(k_data_rows, dim3(grid_for(nD * RC, 256, 1024)), dim3(256), 0, s,
+                         ctx->X.p, (int32_t)N, (int32_t)nD, RC, dcat.p);
+    {
+      TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
+      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, u,
+                         (int32_t)N, RC, r0, ctx->Zt.p);
+    }
+    LevelArgs& A = I.L;
+    A.K = plan->kmax;
+    A.RC = RC;
+    A.shift = shift;
+    A.pos = plan->pos.p;
+    A.nfound = plan->nfound.p;
+    A.flags = plan->flags.p;
+    A.nbr = plan->nbr.p;
+    A.lam = plan->lam.p;
+    A.sigma = plan->sigma.p;
+    A.node_req = plan->node_req.p;
+    A.mean = plan->req_mean.p;
+    A.Zt = ctx->Zt.p;
+    A.X = ctx->X.p;
+    {
+      TimedLaunch tl(ctx, GSP_K_SWEEP);
+      for (int32_t l = 0; l < plan->depth; ++l) {
+        const int64_t off = plan->level_off[(size_t)l];
+        const int64_t cnt = plan->level_off[(size_t)l + 1] - off;
+        if (cnt == 0) continue;
+        A.order = plan->order.p + off;
+        A.count = (int32_t)cnt;
+        const int64_t threads = cnt << shift;
+        const int64_t per = ((threads + 255) / 256 + kXcds - 1) / kXcds;
+        hipLaunchKernelGGL(k_sweep_level_ind, dim3((unsigned)(per * kXcds)), dim3(256), 0, s, I);
+      }
+    }
+    {
+      TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
+      hipLaunchKernelGGL(k_transpose<int32_t>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s,
+                         ctx->X.p, (int32_t)N, RC, r0, out);
     }
     GSP_HIP(hipGetLastError());
   }

@@ -13,11 +13,13 @@ from ._lib import (ArgumentError, Context, GspError, PosDefException, Unsupporte
                    default_context)
 from .ensembles import Ensemble
 from .expectation import KrigingPlan, expectedvalue, mean, posteriormean, priormean
-from .functions import (ExponentialCovariance, ExponentialVariogram, GaussianCovariance,
-                        GaussianVariogram, SphericalCovariance, SphericalVariogram)
+from .functions import (CoregionalizedFunction, ExponentialCovariance, ExponentialVariogram,
+                        GaussianCovariance, GaussianVariogram, SphericalCovariance,
+                        SphericalVariogram)
 from .geometry import CartesianGrid, GeoTable, PointSet, georef
 from .initialization import ExplicitInit, NearestInit, initialize
-from .processes import FixedPath, GaussianProcess, LinearPath, RandomPath, SourcePath
+from .processes import (FixedPath, GaussianProcess, IndicatorProcess, LinearPath, RandomPath,
+                        SourcePath)
 from .simulation import (FFTSIM, LUSIM, SEQSIM, MetricBall, defaultsimulation, preprocess, rand,
                          randsingle)
 
@@ -33,6 +35,7 @@ __all__ = [
     "ArgumentError", "Context", "GspError", "PosDefException", "UnsupportedError",
     "default_context", "Ensemble", "ExponentialCovariance", "ExponentialVariogram",
     "GaussianCovariance", "GaussianVariogram", "SphericalCovariance", "SphericalVariogram",
+    "CoregionalizedFunction", "IndicatorProcess",
     "CartesianGrid", "GeoTable", "PointSet", "georef", "ExplicitInit", "NearestInit",
     "initialize", "FixedPath", "GaussianProcess", "LinearPath", "RandomPath", "SourcePath",
     "FFTSIM", "LUSIM", "SEQSIM", "MetricBall", "defaultsimulation", "preprocess", "rand",

@@ -130,6 +130,7 @@ SIGNATURES = {
     "gsp_seqsim_plan_export": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "gsp_seqsim_sweep": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP, _I32]),
     "gsp_seqsim_plan_free": (_I32, [_VP]),
+    "gsp_seqsim_sweep_indicator": (_I32, [_VP, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _I32]),
     "gsp_seqsim_plan_create_batch": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
                                             ctypes.POINTER(_VP)]),
     "gsp_seqsim_plan_layout": (_I32, [_VP, ctypes.POINTER(_I32), _VP, _VP]),
@@ -372,6 +373,27 @@ class SeqsimPlan:
         check(load().gsp_seqsim_plan_export(self.ctx.handle, self._h, *(ptr(out.get(k)) for k in
                                             ("nfound", "nbr", "lam", "sigma", "flags", "level"))))
         return out
+
+    def sweep_indicator(self, u, prob, data_cat=None, out=None, device_pointers=False):
+        """u: (R, N) uniforms in path order; prob: a priori probabilities;
+        data_cat: 0-based data categories. Returns (R, N) int32 categories."""
+        prob = np.ascontiguousarray(prob, dtype=np.float64)
+        if device_pointers:
+            R = int(u.shape[0])
+            check(load().gsp_seqsim_sweep_indicator(self.ctx.handle, self._h, R, len(prob),
+                                                    ptr(prob), ptr(u), ptr(data_cat), ptr(out), 1))
+            return out
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.ndim == 1:
+            u = u[None, :]
+        R = u.shape[0]
+        if u.shape[1] != self.info.n_nodes:
+            raise ArgumentError("u must hold one uniform per path node")
+        dc = None if data_cat is None else np.ascontiguousarray(data_cat, dtype=np.int32)
+        res = np.empty((R, self.info.n_nodes), np.int32) if out is None else out
+        check(load().gsp_seqsim_sweep_indicator(self.ctx.handle, self._h, R, len(prob), ptr(prob),
+                                                ptr(u), ptr(dc), ptr(res), 0))
+        return res
 
     def sweep(self, z, data_vals=None, out=None, device_pointers=False):
         """z: (R, N) standard normals in path order. Returns/out: (R, N)."""

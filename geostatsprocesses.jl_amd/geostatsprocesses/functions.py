@@ -12,6 +12,7 @@ from . import _lib
 class GeoStatsFunction:
     family = None
     kind = _lib.VARIOGRAM
+    __array_ufunc__ = None  # `A * f` with a numpy matrix A goes to __rmul__
 
     def __init__(self, sill=1.0, range=1.0, nugget=0.0, ranges=None, rotation=None):
         self._sill = float(sill)
@@ -82,6 +83,47 @@ class GeoStatsFunction:
     def __repr__(self):
         return (f"{type(self).__name__}(sill={self._sill}, nugget={self._nugget}, "
                 f"ranges={self._ranges})")
+
+    def __rmul__(self, A):
+        """A * f with a symmetric matrix A: the multivariate function
+        C_ij(h) = A_ij c(h) (intrinsic coregionalization)."""
+        return CoregionalizedFunction(A, self)
+
+
+class CoregionalizedFunction:
+    """A * f: n variables sharing the spatial structure of f (GeoStatsFunctions
+    composite of a matrix and a function)."""
+
+    def __init__(self, A, base):
+        A = np.atleast_2d(np.asarray(A, dtype=np.float64))
+        if A.shape[0] != A.shape[1] or not np.allclose(A, A.T):
+            raise ValueError("the coefficient matrix must be square and symmetric")
+        self.A = A
+        self.base = base
+
+    def nvariables(self):
+        return self.A.shape[0]
+
+    def isstationary(self):
+        return self.base.isstationary()
+
+    def issymmetric(self):
+        return True
+
+    def isbanded(self):
+        return self.base.isbanded()
+
+    def range(self):
+        return self.base.range()
+
+    def ranges(self, ndim=None):
+        return self.base.ranges(ndim)
+
+    def sill(self):
+        return self.A * self.base.sill()
+
+    def __repr__(self):
+        return f"{self.A.tolist()} * {self.base!r}"
 
 
 class _Spherical:

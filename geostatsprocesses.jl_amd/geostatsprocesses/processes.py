@@ -32,6 +32,36 @@ class GaussianProcess(FieldProcess):
         return ("field",) if nf == 1 else tuple(f"field{i + 1}" for i in range(nf))
 
 
+class IndicatorProcess(FieldProcess):
+    """IndicatorProcess(func, prob) (src/processes/field/indicator.jl:29-47):
+    categorical field with multivariate function `func` (one variable per
+    category) and a priori probabilities `prob` (rescaled to sum one)."""
+
+    def __init__(self, func, prob):
+        prob = np.asarray(prob, dtype=np.float64)
+        prob = prob / prob.sum()
+        nf = func.nvariables()
+        if not nf > 1:
+            raise AssertionError("indicator process requires multivariate function")
+        if len(prob) != nf:
+            raise AssertionError(f"probabilities must have {nf} components, received {len(prob)}")
+        if not np.all((prob >= 0) & (prob <= 1)):
+            raise AssertionError("probabilities must be in [0, 1]")
+        if not np.isclose(prob.sum(), 1.0):
+            raise AssertionError("probabilities must sum up to one")
+        self.func = func
+        self.prob = prob
+
+    def iscontinuous(self):
+        return False
+
+    def isanalytical(self):
+        return False
+
+    def defaultnames(self):
+        return ("field",)
+
+
 class LinearPath:
     """Meshes LinearPath: elements in linear-index order."""
 

@@ -20,7 +20,8 @@ from . import _lib
 from .ensembles import Ensemble
 from .geometry import CartesianGrid, GeoTable, PointSet
 from .initialization import ExplicitInit, NearestInit, initialize
-from .processes import GaussianProcess, LinearPath
+from .functions import CoregionalizedFunction
+from .processes import GaussianProcess, IndicatorProcess, LinearPath, SourcePath
 
 
 class FieldSimulationMethod:
@@ -94,7 +95,13 @@ def _sides(domain):
 
 
 def defaultsimulation(process, domain, data=None):
-    """field.jl:152-166."""
+    """field.jl:152-180."""
+    if isinstance(process, IndicatorProcess):
+        # SEQSIM visiting the data's nearest cells first (field.jl:168-180)
+        if data is None:
+            return SEQSIM(path=LinearPath())
+        dest = NearestInit().destinations(domain, data)
+        return SEQSIM(path=SourcePath(_unique_in_order(dest)))
     if not isinstance(process, GaussianProcess):
         return None
     f = process.func
@@ -106,6 +113,12 @@ def defaultsimulation(process, domain, data=None):
     return SEQSIM()
 
 
+def _unique_in_order(x):
+    """unique(x) keeping first occurrences in order (Julia's `unique`)."""
+    _, first = np.unique(x, return_index=True)
+    return np.asarray(x)[np.sort(first)]
+
+
 # ---------------------------------------------------------------- SEQSIM ----
 class SeqsimPreproc:
     def __init__(self, plan, names, data_vals, path):
@@ -115,8 +128,11 @@ class SeqsimPreproc:
         self.path = path
 
 
-def _seqsim_problem(process, method, domain, data, ctx):
-    f = process.func
+def _seqsim_problem(process, method, domain, data, ctx, func=None, data_vals=None):
+    """The SEQSIM plan (seqsim.jl:50-70). `func` replaces the process function
+    for the search and the kriging weights (the spatial part of an indicator
+    process); `data_vals` replaces the data values."""
+    f = process.func if func is None else func
     if f.nvariables() != 1:
         raise _lib.UnsupportedError("multivariate SEQSIM is not supported by this backend")
     grid = _grid_struct(domain)
@@ -124,13 +140,14 @@ def _seqsim_problem(process, method, domain, data, ctx):
     p = _lib.GspSeqsimProblem()
     p.grid = grid
     p.model = f.struct(nd)
-    p.mean = float(process.mean)
+    p.mean = float(process.mean) if isinstance(process, GaussianProcess) else 0.0
     if data is not None:
         coords = np.ascontiguousarray(data.domain.centroids(), dtype=np.float64).reshape(-1, nd)
         names = data.names()
         if len(names) != 1:
             raise _lib.UnsupportedError("SEQSIM data must have exactly one variable")
-        vals = np.ascontiguousarray(data[names[0]], dtype=np.float64)
+        vals = np.ascontiguousarray(data[names[0]] if data_vals is None else data_vals,
+                                    dtype=np.float64)
         if np.isnan(vals).any():
             raise _lib.UnsupportedError("missing data values are not supported by this backend")
     else:
@@ -158,6 +175,47 @@ def _seqsim_problem(process, method, domain, data, ctx):
     p.device_pointers = 0
     plan = _lib.SeqsimPlan(ctx, p, keep=(coords, path))
     return SeqsimPreproc(plan, names, vals, path)
+
+
+class IndicatorPreproc:
+    def __init__(self, plan, names, labels, data_cat, prob):
+        self.plan = plan
+        self.names = names
+        self.labels = labels
+        self.data_cat = data_cat
+        self.prob = prob
+
+
+def _indicator_problem(process, method, domain, data, ctx):
+    """SEQSIM of an IndicatorProcess (seqsim.jl:247-251,292-310,342-347): the
+    data one-hot encoded by their sorted distinct values (OneHot), simple
+    cokriging with the a priori probabilities as mean, clamp + normalize, a
+    categorical draw. With an intrinsic coregionalization A * c(h) and every
+    category observed at every neighbour, the cokriging weights are lambda_j
+    times the identity, lambda the simple-kriging weights of c(h): the plan is
+    the Gaussian one built with c(h)."""
+    f = process.func
+    if not isinstance(f, CoregionalizedFunction):
+        raise _lib.UnsupportedError(
+            "this backend simulates indicator processes of a coregionalized covariance "
+            "A * c(h); transiograms are not supported")
+    if data is not None:
+        names = data.names()
+        if len(names) != 1:
+            raise _lib.UnsupportedError("indicator data must have exactly one variable")
+        raw = np.asarray(data[names[0]])
+        labels, data_cat = np.unique(raw, return_inverse=True)
+        if len(labels) != f.nvariables():
+            raise AssertionError(f"the data have {len(labels)} categories, the process "
+                                 f"{f.nvariables()}")
+    else:
+        names = process.defaultnames()
+        labels = np.arange(1, f.nvariables() + 1)
+        data_cat = np.zeros(0, np.int64)
+    data_cat = np.ascontiguousarray(data_cat, dtype=np.int32)
+    pre = _seqsim_problem(process, method, domain, data, ctx, func=f.base,
+                          data_vals=data_cat.astype(np.float64) if data is not None else None)
+    return IndicatorPreproc(pre.plan, names, labels, data_cat, process.prob)
 
 
 # ----------------------------------------------------------------- LUSIM ----
@@ -247,8 +305,11 @@ def _fftsim_problem(process, method, init, domain, data, ctx):
 # -------------------------------------------------------- plugin contract ----
 def preprocess(rng, process, method, init, domain, data, ctx=None):
     ctx = _lib.default_context() if ctx is None else ctx
+    if isinstance(process, IndicatorProcess) and isinstance(method, SEQSIM):
+        return _indicator_problem(process, method, domain, data, ctx)
     if not isinstance(process, GaussianProcess):
-        raise _lib.UnsupportedError("only GaussianProcess is accelerated by this backend")
+        raise _lib.UnsupportedError("only GaussianProcess and IndicatorProcess (SEQSIM) are "
+                                    "accelerated by this backend")
     if isinstance(method, SEQSIM):
         return _seqsim_problem(process, method, domain, data, ctx)
     if isinstance(method, LUSIM):
@@ -259,6 +320,11 @@ def preprocess(rng, process, method, init, domain, data, ctx=None):
 
 
 def _draw(rng, preproc, R):
+    if isinstance(preproc, IndicatorPreproc):
+        # one uniform per path node for the categorical draw (seqsim.jl:342-347)
+        u = rng.random((R, preproc.plan.n_nodes))
+        cats = preproc.plan.sweep_indicator(u, preproc.prob, preproc.data_cat)
+        return preproc.labels[cats]
     if isinstance(preproc, SeqsimPreproc):
         z = rng.standard_normal((R, preproc.plan.n_nodes))
         return preproc.plan.sweep(z, preproc.data_vals)

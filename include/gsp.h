@@ -270,6 +270,23 @@ int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                          const double* z, const double* data_vals, double* out,
                          int32_t device_pointers);
 int32_t gsp_seqsim_plan_free(gsp_seqsim_plan* plan);
+/* gsp_seqsim_sweep_indicator replaces R calls of randsingle(..., ::SEQSIM, ...)
+ * for an IndicatorProcess (seqsim.jl:247-251,292-310,342-347) whose function is
+ * an intrinsic coregionalization A * c(h): with every category observed at
+ * every neighbour, simple cokriging gives the weights lambda_j * I of the
+ * plan built with c(h), so the plan is shared with the Gaussian case. Per node
+ * and category p_c + sum_j lambda_j ([x_j == c] - p_c), clamped to [0, 1] and
+ * scaled by the inverse of its sum (normalize(., 1)), then the categorical
+ * draw: the first category whose running sum exceeds u; prior draws (fewer
+ * than minneighbors, failed systems) use `prob`.
+ *   prob:     n_cat a priori probabilities (2 <= n_cat <= 8)
+ *   u:        R x n_path uniforms in [0, 1), one per path node, path order
+ *   data_cat: n_data 0-based categories of the conditioning data
+ *   out:      R x N 0-based categories, realization-major. */
+int32_t gsp_seqsim_sweep_indicator(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                   int32_t n_cat, const double* prob, const double* u,
+                                   const int32_t* data_cat, int32_t* out,
+                                   int32_t device_pointers);
 
 /* ---- SEQSIM batches --------------------------------------------------------
  * gsp_seqsim_plan_create_batch builds one plan for n independent SEQSIM

@@ -293,6 +293,66 @@ def seqsim_sweep(case: SeqsimCase, plan: Plan, z: np.ndarray) -> np.ndarray:
     return out
 
 
+# -------------------------------------------------------- indicator SEQSIM ----
+def draw_category(q, u):
+    """Distributions 0.25 `rand(rng, Categorical(q))` (DiscreteNonParametric,
+    not vendored): cp = q[0]; i = 0; while cp <= u and i < n - 1: i += 1,
+    cp += q[i]. Vectorised over rows of q (R, n) and u (R,)."""
+    q = np.asarray(q, dtype=np.float64)
+    n = q.shape[1]
+    cat = np.zeros(len(u), np.int64)
+    cp = q[:, 0].copy()
+    more = cp <= u
+    for c in range(1, n):
+        cat = np.where(more, c, cat)
+        cp = np.where(more, cp + q[:, c], cp)
+        more = more & (cp <= u)
+    return cat
+
+
+def indicator_probabilities(q):
+    """seqsim.jl:306-310: normalize(clamp.(p, 0, 1), 1) per row, i.e. the
+    clamped vector times the inverse of its 1-norm (LinearAlgebra's
+    normalize, with its rescaling below 1/floatmax); the norm sums left to
+    right."""
+    q = np.where(q > 1.0, 1.0, np.where(q < 0.0, 0.0, q))
+    s = np.abs(q[:, 0]).copy()
+    for c in range(1, q.shape[1]):
+        s = s + np.abs(q[:, c])
+    delta = 1.0 / np.finfo(np.float64).max
+    ed = np.finfo(np.float64).eps / delta
+    big = s >= delta
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        out = np.where(big[:, None], q * (1.0 / s)[:, None], (q * ed) * (1.0 / (s * ed))[:, None])
+    return out
+
+
+def indicator_sweep(case: SeqsimCase, plan: Plan, u: np.ndarray, prob, data_cat):
+    """SEQSIM of an IndicatorProcess whose function is A * c(h), given the plan
+    of c(h) (seqsim.jl:98-138 with _probmodel/_conditional/_draw! of :292-310,
+    342-347): in path order, p_c + sum_j lambda_j ([x_j == c] - p_c) per
+    category, normalised, drawn with one uniform per node; prior draws use
+    `prob`. u: (R, n_path). Returns (R, N) 0-based categories."""
+    prob = np.asarray(prob, dtype=np.float64)
+    u = np.atleast_2d(np.asarray(u, dtype=np.float64))
+    R, N = u.shape[0], case.n_nodes
+    X = np.zeros((N + case.n_data, R), np.int64)
+    X[N:] = np.asarray(data_cat, dtype=np.int64)[:, None]
+    for q, i in enumerate(case.path):
+        prior = (plan.flags[i] & 3) != 0
+        k = 0 if prior else plan.nfound[i]
+        p = np.tile(prob, (R, 1))
+        for j in range(k):
+            x = X[plan.nbr[i, j]]
+            lam = plan.lam[i, j]
+            for c in range(len(prob)):
+                p[:, c] = p[:, c] + lam * ((x == c).astype(np.float64) - prob[c])
+        if not prior:
+            p = indicator_probabilities(p)
+        X[i] = draw_category(p, u[:, q])
+    return X[:N].T.copy()
+
+
 def simple_kriging_weights(case: SeqsimCase, kmax=None):
     """Simple kriging weights of every grid centroid over the data, restated in
     numpy independently of the C oracle: GeoStatsModels 0.13 `fitpredict(

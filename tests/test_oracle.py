@@ -504,3 +504,53 @@ def test_ensemble_stat_matches_numpy(R):
     np.testing.assert_array_equal(O.ensemble_stat(v, "cdf", 1.0), (v <= 1.0).mean(axis=0))
     np.testing.assert_array_equal(O.ensemble_stat(v, "ccdf", 1.0) + O.ensemble_stat(v, "cdf", 1.0),
                                   np.ones(500))
+
+
+# ------------------------------------------------------- indicator SEQSIM ----
+def test_cokriging_of_coregionalized_indicators_reduces_to_kriging():
+    """Simple cokriging with C_ab(h) = A_ab c(h) and every variable observed at
+    every neighbour: the full (k n) x (k n) system has the weights
+    lambda_j I, lambda the simple-kriging weights of c(h) (the reduction the
+    indicator sweep relies on)."""
+    rng = np.random.default_rng(4)
+    X = rng.uniform(0, 10, (7, 2))
+    x0 = np.array([[4.0, 5.0]])
+    A = np.array([[1.0, 0.4, 0.1], [0.4, 1.0, 0.3], [0.1, 0.3, 1.0]])
+    C = O.covariance_np(O.SPHERICAL, 1.0, 0.05, (6.0, 6.0), X, X)
+    c0 = O.covariance_np(O.SPHERICAL, 1.0, 0.05, (6.0, 6.0), X, x0)[:, 0]
+    lam = np.linalg.solve(C, c0)
+    Lam = np.linalg.solve(np.kron(C, A), np.kron(c0[:, None], A))  # (7*3) x 3
+    for j in range(7):
+        np.testing.assert_allclose(Lam[3 * j:3 * j + 3], lam[j] * np.eye(3), atol=1e-12)
+
+
+def test_categorical_draw_matches_cumulative_search():
+    """The first category whose running sum exceeds u, independently by
+    searchsorted on the cumulative sums (the last category at most)."""
+    rng = np.random.default_rng(5)
+    q = rng.dirichlet(np.ones(4), 5000)
+    u = rng.random(5000)
+    u[:10] = 0.0
+    u[10:20] = np.nextafter(1.0, 0.0)
+    ref = np.minimum([np.searchsorted(np.cumsum(qq), uu, side="right") for qq, uu in zip(q, u)], 3)
+    np.testing.assert_array_equal(O.draw_category(q, u), ref)
+    # normalisation: clamped and scaled to unit sum
+    raw = rng.normal(0.3, 0.4, (100, 3))
+    raw[:, 0] = np.abs(raw[:, 0]) + 0.01  # every row keeps some mass
+    out = O.indicator_probabilities(raw)
+    assert np.all(out >= 0) and np.allclose(out.sum(1), 1.0)
+    np.testing.assert_allclose(out, np.clip(raw, 0, 1) / np.clip(raw, 0, 1).sum(1, keepdims=True),
+                               rtol=1e-15)
+
+
+def test_indicator_sweep_prior_only_follows_probabilities():
+    """With no neighbour inside the search ball every node draws from the
+    prior: the category frequencies follow the probabilities."""
+    case = O.SeqsimCase(dims=(40, 40), family=O.SPHERICAL, ranges=(8.0,), kmax=12, radius=0.5,
+                        path=np.random.default_rng(1).permutation(1600))
+    plan = O.seqsim_plan(case)
+    assert np.all((plan.flags & 1) != 0)
+    u = np.random.default_rng(2).random((4, 1600))
+    out = O.indicator_sweep(case, plan, u, [0.6, 0.3, 0.1], [])
+    freq = np.bincount(out.ravel(), minlength=3) / out.size
+    np.testing.assert_allclose(freq, [0.6, 0.3, 0.1], atol=0.02)
