@@ -39,6 +39,13 @@ variogram sill 1, range 20, nugget 0.01, the 10 nearest data of every node
 (GeoStatsModels' fitpredict defaults), FP64 solves. One step is one
 data-only plan plus its zero-normal sweep (the mean at every node).
 
+Workload "indicator-3d-128x128x64": IndicatorProcess SEQSIM with 3 categories
+(probabilities 0.5, 0.3, 0.2, coregionalized A * exponential covariance with
+the config-3 ranges (40, 40, 8), nugget 0.05) on the 128 x 128 x 64 grid,
+2000 categorical data, maxneighbors 24 with the :range neighbourhood, random
+path, 64 realizations. One step is one plan plus the 64-realization
+categorical sweep, uniforms resident in HBM.
+
 Workload "fftsim-256x256x128": FFT moving-average simulation (FFTSIM, the
 default method of unconditional grids) on a 256 x 256 x 128 unit grid
 (8,388,608 cells), Gaussian variogram sill 1, range 20, 16 realizations from
@@ -87,6 +94,9 @@ WORKLOADS = {
         cpu_nodes=1_500_000),
     "ensemble-stats-32x256x256x128": dict(dims=(256, 256, 128), R=32, seed=1008, method="stats",
                                           cpu_cells=1_000_000),
+    "indicator-3d-128x128x64": dict(dims=(128, 128, 64), family=1, sill=1.0, ranges=(40.0, 40.0, 8.0),
+                                    nugget=0.05, n_data=2000, kmax=24, radius=40.0, R=64, seed=1009,
+                                    method="indicator", prob=(0.5, 0.3, 0.2)),
     "fftsim-256x256x128": dict(dims=(256, 256, 128), family=2, sill=1.0, ranges=(20.0, 20.0, 20.0),
                                nugget=0.0, R=16, seed=1007, method="fftsim", cpu_reals=2),
 }
@@ -539,6 +549,58 @@ def bench_stats(args, spec):
     print(json.dumps(line), flush=True)
 
 
+def bench_indicator(args, spec):
+    """IndicatorProcess SEQSIM: plan + categorical sweep per step."""
+    from geostatsprocesses import _lib
+    precision = _lib.FP32_REFINE if (args.precision or "fp32r") == "fp32r" else _lib.FP64
+    N, path, data, _ = make_inputs(spec)
+    R = spec["R"]
+    prob = np.asarray(spec["prob"])
+    cats = np.random.default_rng(spec["seed"] + 2).integers(0, len(prob), spec["n_data"]).astype(np.int32)
+    ctx = _lib.default_context(0)
+    dpath = _lib.DeviceBuffer.from_array(ctx, path)
+    ddata = _lib.DeviceBuffer.from_array(ctx, data)
+    dcats = _lib.DeviceBuffer.from_array(ctx, cats)
+    u = _lib.DeviceBuffer(ctx, (R, N))
+    for r in range(R):
+        row = np.random.default_rng(spec["seed"] + 3 + r).random(N)
+        _lib.check(_lib.load().gsp_memcpy_to_device(ctx.handle, u.ptr + r * N * 8, row.ctypes.data, N * 8))
+    out = _lib.DeviceBuffer(ctx, (R, N), np.int32)
+    prob_arg = problem(spec, dpath.ptr, ddata.ptr, precision, N)
+
+    class Step:
+        def __init__(self):
+            self.plan = _lib.SeqsimPlan(ctx, prob_arg)
+            self.plan.sweep_indicator(u, prob, dcats, out, device_pointers=True)
+            ctx.synchronize()
+
+        def close(self):
+            self.plan.close()
+
+    step = Step()
+    ex = step.plan.export(fields=("nfound", "flags"))
+    kbar = float(np.where(ex["flags"] & 3, 0, ex["nfound"]).mean())
+    K = step.plan.info.maxneighbors
+    depth = step.plan.info.depth
+    step.close()
+    ms, tim = timed(ctx, Step, args.steps, args.warmup)
+    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar)
+    line = {
+        "metric": "indicator SEQSIM nodes*realizations/s",
+        "value": round(N * R / (ms / 1e3), 1),
+        "unit": "nodes*realizations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 categories (solves fp32+fp64 refinement)",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
+                   "categories": len(prob), "maxneighbors": spec["kmax"], "radius": spec["radius"],
+                   "n_data": spec["n_data"], "depth": depth, "mean_neighbours": round(kbar, 2)},
+        "roofline": roofline_of(kern, args.workload), "kernels": kern, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -575,6 +637,8 @@ def main():
         return bench_fftsim(args, spec)
     if spec.get("method") == "stats":
         return bench_stats(args, spec)
+    if spec.get("method") == "indicator":
+        return bench_indicator(args, spec)
     args.precision = args.precision or spec.get("precision", "fp32r")
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
     R = spec["R"]

@@ -1145,6 +1145,88 @@ __global__ __launch_bounds__(256) void k_level_push(int32_t L, const int32_t* ro
   }
 }
 
+// Every level of a small plan in one workgroup of 1024 threads, without
+// launches or host round trips between levels: the dependency counters are
+// bytes in LDS (four to a word; a count never drops below zero, so the
+// packed decrements never borrow), thread t owns nodes t + 1024 m and keeps
+// their placed bits in a register. Per level: each thread counts its ready
+// nodes, a block scan gives every thread its slots in the new level, then
+// the new level's nodes release their dependents. cnt[depth] = 0 ends the
+// levels; writes depth to *depth.
+constexpr int kSmallLevels = 32768;  // nodes up to which one workgroup does all levels
+
+__global__ __launch_bounds__(1024) void k_levels_small(int32_t N, const int32_t* rowptr,
+                                                       const int32_t* deps,
+                                                       const int32_t* indeg, int32_t* level,
+                                                       int32_t* order, int32_t* off,
+                                                       int32_t* cnt, int32_t* depth) {
+  __shared__ uint32_t s_deg[kSmallLevels / 4];
+  __shared__ int32_t s_wave[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int q = tid; q < (N + 3) / 4; q += 1024) {
+    uint32_t word = 0;
+    for (int b = 0; b < 4; ++b)
+      if (4 * q + b < N) word |= (uint32_t)indeg[4 * q + b] << (8 * b);
+    s_deg[q] = word;
+  }
+  __syncthreads();
+  uint32_t placed = 0;  // bit m: node tid + 1024 m has its level
+  int32_t start = 0, size = 0;
+  for (int32_t L = -1;; ++L) {
+    uint32_t ready = 0;
+    int c = 0;
+    for (int m = 0; tid + 1024 * m < N; ++m) {
+      const int i = tid + 1024 * m;
+      const uint32_t d = (s_deg[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+      if (!((placed >> m) & 1u) && d == 0) {
+        ready |= 1u << m;
+        ++c;
+      }
+    }
+    int incl = c;  // inclusive scan over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int v = 0; v < 16; ++v) {
+      before += v < w ? s_wave[v] : 0;
+      total += s_wave[v];
+    }
+    const int32_t next = start + size;
+    int32_t slot = next + before + incl - c;
+    for (int m = 0; tid + 1024 * m < N; ++m)
+      if ((ready >> m) & 1u) {
+        const int i = tid + 1024 * m;
+        order[slot++] = i;
+        level[i] = L + 1;
+      }
+    placed |= ready;
+    if (tid == 0) {
+      off[L + 1] = next;
+      cnt[L + 1] = total;
+    }
+    if (total == 0) {
+      if (tid == 0) *depth = L + 1;
+      return;
+    }
+    start = next;
+    size = total;
+    __syncthreads();  // the new level's order[] entries visible; s_wave free again
+    for (int32_t t = tid; t < size; t += 1024) {
+      const int a = order[start + t];
+      const int e1 = rowptr[a + 1];
+      for (int e = rowptr[a]; e < e1; ++e) {
+        const int j = deps[e];
+        atomicSub(&s_deg[j >> 2], 1u << (8 * (j & 3)));
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Each block takes a contiguous chunk of node ids: it counts its ready nodes,
 // reserves their slots with one atomic, then writes them in id order (same-
 // address atomics serialise, so one per block rather than one per wavefront).
@@ -1788,7 +1870,7 @@ namespace {
 // Buffers of the level computation, all made before the worker starts (the
 // context's allocator hands out blocks in main-stream order only).
 struct LevelBufs {
-  DevBuf<int32_t> rowptr, cursor, deps, cnt, off;
+  DevBuf<int32_t> rowptr, cursor, deps, cnt, off, depth;
   DevBuf<char> tmp;
   size_t tmp_bytes = 0;
 };
@@ -1799,6 +1881,7 @@ void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* 
   B.cursor.alloc((size_t)NT);
   B.cnt.alloc((size_t)NT + 16);
   B.off.alloc((size_t)NT + 16);
+  B.depth.alloc(1);
   B.deps.alloc((size_t)std::max<int64_t>(n_edges, 1));
   B.tmp_bytes = 0;
   GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, B.tmp_bytes, outdeg, B.rowptr.p,
@@ -1825,11 +1908,21 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
   hipLaunchKernelGGL(k_dep_fill<FLAGS>, dim3(nb), dim3(256), 0, st, D, B.rowptr.p, B.cursor.p,
                      B.deps.p);
   const int cb = grid_for(NT, 2048, 1024);
+  int32_t depth = -1;
+  if (NT <= kSmallLevels) {
+    // one workgroup, one launch, one read of the depth
+    hipLaunchKernelGGL(k_levels_small, dim3(1), dim3(1024), 0, st, (int32_t)NT, B.rowptr.p,
+                       B.deps.p, indeg, plan->level.p, plan->order.p, B.off.p, B.cnt.p,
+                       B.depth.p);
+    int32_t d = 0;
+    GSP_HIP(hipMemcpyAsync(&d, B.depth.p, 4, hipMemcpyDeviceToHost, st));
+    GSP_HIP(hipStreamSynchronize(st));
+    depth = d;
+  } else {
   hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, -1, indeg,
                      plan->level.p, plan->order.p, B.off.p, B.cnt.p);
   constexpr int kBatch = 16;
   const int fb = grid_for(NT / 16, 256, 1024);
-  int32_t depth = -1;
   std::vector<int32_t> hc(kBatch + 1);
   for (int32_t L = 0; depth < 0; L += kBatch) {
     GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
@@ -1846,6 +1939,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
         depth = L + b;
         break;
       }
+  }
   }
   GSP_REQUIRE(depth >= 1, GSP_ERR_INTERNAL, "empty dependency levels");
   std::vector<int32_t> hcnt((size_t)depth);
