@@ -459,51 +459,91 @@ constexpr int kInferWaves = 8;  // 256 nodes per pass of the weights through LDS
 
 // Weight fragments of output tiles, staged in LDS for the block's
 // wavefronts: three buffers, so that the copy of tile rt + 2 runs while tile
-// rt is multiplied and tile rt - 1 goes through its activation.
+// rt is multiplied and tile rt - 1 goes through its activation. The buffers
+// are separate LDS objects, so the compiler can tell that reading one does
+// not have to wait for the asynchronous copies into the others.
+using FragBuf = u32x4[8][64];
 struct FragLds {
-  u32x4 w[3][8][64];
+  FragBuf& b0;
+  FragBuf& b1;
+  FragBuf& b2;
+  template <int B>
+  __device__ __forceinline__ FragBuf& buf() const {
+    if constexpr (B == 0) return b0;
+    else if constexpr (B == 1) return b1;
+    else return b2;
+  }
 };
 
-// Asynchronous copy of the fragments of tile `tile` into buffer `buf`: one
-// global_load_lds_dwordx4 per fragment (1 KB, 16 B per lane), fragments
-// shared out over the block's wavefronts. Completion: s_waitcnt vmcnt.
+// Asynchronous copy of the fragments of one tile into buffer `dst`
+// (global_load_lds_dwordx4: 1 KB per fragment, 16 B per lane). Every
+// wavefront issues exactly one copy per tile, so that all of them wait for
+// the same counts and no copy sits behind a branch: fragment w % STEPS. With
+// fewer fragments than wavefronts, kInferWaves / STEPS wavefronts copy the same
+// fragment and write identical bytes to the same LDS words (L2 serves the
+// repeats). Completion: s_waitcnt vmcnt.
 template <int STEPS>
-__device__ __forceinline__ void copy_tile(FragLds& F, int buf, const uint16_t* wp, int tile,
-                                          int lane, int w) {
-#pragma unroll
-  for (int s = w; s < STEPS; s += kInferWaves)
-    __builtin_amdgcn_global_load_lds(
-        (const __attribute__((address_space(1))) void*)(wp + ((int64_t)(tile * STEPS + s) * 64 + lane) * 8),
-        (__attribute__((address_space(3))) void*)&F.w[buf][s][0], 16, 0, 0);
+__device__ __forceinline__ void copy_tile(FragBuf& dst, const uint16_t* wp, int tile, int lane,
+                                          int w) {
+  static_assert(kInferWaves % STEPS == 0, "whole wavefronts per fragment");
+  const int s = w % STEPS;
+  __builtin_amdgcn_global_load_lds(
+      (const __attribute__((address_space(1))) void*)(wp + ((int64_t)(tile * STEPS + s) * 64 + lane) * 8),
+      (__attribute__((address_space(3))) void*)&dst[s][0], 16, 0, 0);
 }
+
+// s_waitcnt with only the vector-memory count bounded: vmcnt <= n (expcnt and
+// lgkmcnt at their "no wait" maxima, gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 16, "vmcnt field");
+  __builtin_amdgcn_s_waitcnt(N | (0x7 << 4) | (0xF << 8));
+}
+
+// Block barrier for the fragment buffers. __syncthreads() (and any release
+// fence) waits for every outstanding vector-memory operation, including the
+// weight copy meant to stay in flight across the barrier. Here the caller has
+// already waited (wait_vmcnt) for the copies the next step reads, and every
+// LDS read of the buffer about to be refilled has been consumed by the
+// products before the barrier, so the bare s_barrier suffices; the memory
+// clobber keeps the compiler from moving loads or stores across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 // One layer of the fused inference: out^T = act(W in^T + b) for the wave's
 // 32 nodes (2 column tiles); in/out hold the packed bf16 pairs of the
 // operand fragments, word q of step s, column tile ct at [s][ct][q]. Every
 // wavefront of the block runs it (block barriers inside). Step rt issues the
-// products of tile rt, then the activation of tile rt - 1 (the matrix cores
-// and the vector ALU overlap), while tile rt + 2 is copied into LDS.
+// bias load of tile rt + 1 and the copy of tile rt + 2 into LDS, then the
+// products of tile rt and the activation of tile rt - 1 (the matrix cores
+// and the vector ALU overlap). The step ends by waiting for the loads issued
+// one step earlier only (loads complete in order; this wavefront's copy of
+// tile rt + 2 stays in flight), so each copy has a whole step to land.
 template <int STEPS, int TILES, bool ACT>
-__device__ __forceinline__ void mlp_layer(FragLds& F, const uint32_t (&in)[8][2][4],
+__device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)[8][2][4],
                                           uint32_t (&out)[8][2][4], float (&last)[2][2][4],
                                           const uint16_t* wp, const float* bias, int lane, int w) {
   const int g = lane >> 4;
-  copy_tile<STEPS>(F, 0, wp, 0, lane, w);
-  if constexpr (TILES > 1) copy_tile<STEPS>(F, 1, wp, 1, lane, w);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+  f32x4 b_next = *(const f32x4*)(bias + 4 * g);
+  copy_tile<STEPS>(F.template buf<0>(), wp, 0, lane, w);
+  if constexpr (TILES > 1) copy_tile<STEPS>(F.template buf<1>(), wp, 1, lane, w);
+  if constexpr (TILES > 1)
+    wait_vmcnt<1>();
+  else
+    wait_vmcnt<0>();
+  lds_barrier();
   f32x4 acc[2][2];
   static_for<0, TILES + 1>([&](auto rtc) {
     constexpr int rt = decltype(rtc)::value;
-    if constexpr (rt + 2 < TILES) copy_tile<STEPS>(F, (rt + 2) % 3, wp, rt + 2, lane, w);
+    f32x4 b_cur = b_next;
+    if constexpr (rt + 1 < TILES) b_next = *(const f32x4*)(bias + 16 * (rt + 1) + 4 * g);
+    if constexpr (rt + 2 < TILES) copy_tile<STEPS>(F.template buf<(rt + 2) % 3>(), wp, rt + 2, lane, w);
     if constexpr (rt < TILES) {
       // the accumulators start at the bias of their output rows
-      const f32x4 b = *(const f32x4*)(bias + 16 * rt + 4 * g);
-      acc[rt & 1][0] = b;
-      acc[rt & 1][1] = b;
+      acc[rt & 1][0] = b_cur;
+      acc[rt & 1][1] = b_cur;
       static_for<0, STEPS>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
-        const u32x4 a = F.w[rt % 3][s][lane];
+        const u32x4 a = F.template buf<rt % 3>()[s][lane];
         acc[rt & 1][0] = mfma(a, u32x4{in[s][0][0], in[s][0][1], in[s][0][2], in[s][0][3]},
                               acc[rt & 1][0]);
         acc[rt & 1][1] = mfma(a, u32x4{in[s][1][0], in[s][1][1], in[s][1][2], in[s][1][3]},
@@ -532,8 +572,13 @@ __device__ __forceinline__ void mlp_layer(FragLds& F, const uint32_t (&in)[8][2]
       }
     }
     if constexpr (rt < TILES) {
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
+      // the copy of tile rt + 1 (and the bias of rt + 1) must have landed;
+      // this wavefront's copy of tile rt + 2, issued above, may stay in flight
+      if constexpr (rt + 2 < TILES)
+        wait_vmcnt<1>();
+      else
+        wait_vmcnt<0>();
+      lds_barrier();
     }
     __builtin_amdgcn_sched_barrier(0);
   });
@@ -572,14 +617,15 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
 }
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
-  __shared__ FragLds s_frag;
+  __shared__ FragBuf s_w0, s_w1, s_w2;
+  const FragLds s_frag{s_w0, s_w1, s_w2};
   __shared__ uint32_t s_mask[kInferWaves][32];
   __shared__ int8_t s_slot[kInferWaves][32][kSlots];
   __shared__ float s_out[kInferWaves][32][kUsedOut + 1];
   __shared__ int32_t s_any;
   const NodeOut& O = A.O;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   // one group of 32 nodes per wavefront, four groups per block (no grid-stride
   // loop: the weight addresses must not become loop invariants held in registers)
   const int64_t base = ((int64_t)blockIdx.x * kInferWaves + w) * 32;
