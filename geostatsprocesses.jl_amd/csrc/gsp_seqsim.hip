@@ -1070,9 +1070,10 @@ __device__ inline int deps_of(const DepArgs& D, int i) {
 }
 
 // Grid neighbours per node (indeg) and dependents per node (outdeg, atomic
-// counts; data neighbours have ids >= N and are not edges).
+// counts; data neighbours have ids >= N and are not edges). mark[i] = 1 for
+// the nodes without grid neighbours (ready for level 0), else 0.
 template <bool FLAGS>
-__global__ void k_deg_count(const DepArgs D, int32_t* outdeg, int32_t* indeg,
+__global__ void k_deg_count(const DepArgs D, int32_t* outdeg, int32_t* indeg, uint8_t* mark,
                             unsigned long long* edges) {
   uint32_t total = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D.N;
@@ -1092,6 +1093,7 @@ __global__ void k_deg_count(const DepArgs D, int32_t* outdeg, int32_t* indeg,
         }
     }
     indeg[i] = c;
+    mark[i] = c == 0 ? 1 : 0;
     total += (uint32_t)c;
   }
   // the number of edges: one atomic per wavefront
@@ -1127,21 +1129,32 @@ __global__ void k_dep_fill(const DepArgs D, const int32_t* rowptr, int32_t* curs
 }
 
 // Level L + 1 in two launches. Push: every node of frontier L (order[off[L]
-// .. off[L] + cnt[L])) decrements the counters of its dependents; the atomics'
-// results are not used, so they pipeline. Collect: every node not yet placed
-// whose counter reached zero gets level L + 1 and joins frontier L + 1 (one
-// atomic per wavefront; ids ascend inside each wavefront's run). L = -1
-// collects level 0 (nodes without grid neighbours).
+// .. off[L] + cnt[L])) decrements the counters of its dependents, eight at a
+// time so that their atomics are in flight together; the one that takes a
+// counter to zero marks that dependent ready (mark 1). Collect: every node
+// marked ready gets level L + 1, mark 2 (placed) and a slot in frontier
+// L + 1 (one atomic per block; ids ascend inside each block's run), reading
+// one byte per node. L = -1 collects level 0 (nodes without grid neighbours,
+// marked by the degree count).
 __global__ __launch_bounds__(256) void k_level_push(int32_t L, const int32_t* rowptr,
                                                     const int32_t* deps, int32_t* indeg,
-                                                    const int32_t* order, const int32_t* off,
-                                                    const int32_t* cnt) {
+                                                    uint8_t* mark, const int32_t* order,
+                                                    const int32_t* off, const int32_t* cnt) {
   const int32_t start = off[L], size = cnt[L];
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < size;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int a = order[start + t];
     const int e1 = rowptr[a + 1];
-    for (int e = rowptr[a]; e < e1; ++e) atomicSub(&indeg[deps[e]], 1);
+    for (int e0 = rowptr[a]; e0 < e1; e0 += 8) {
+      int d[8], old[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) d[u] = e0 + u < e1 ? deps[e0 + u] : -1;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) old[u] = d[u] >= 0 ? atomicSub(&indeg[d[u]], 1) : 0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (old[u] == 1) mark[d[u]] = 1;
+    }
   }
 }
 
@@ -1230,7 +1243,7 @@ __global__ __launch_bounds__(1024) void k_levels_small(int32_t N, const int32_t*
 // Each block takes a contiguous chunk of node ids: it counts its ready nodes,
 // reserves their slots with one atomic, then writes them in id order (same-
 // address atomics serialise, so one per block rather than one per wavefront).
-__global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, const int32_t* indeg,
+__global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, uint8_t* mark,
                                                        int32_t* level, int32_t* order,
                                                        int32_t* off, int32_t* cnt) {
   __shared__ int32_t s_red[4];
@@ -1241,7 +1254,7 @@ __global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, con
   const int64_t chunk = ((N + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = i0 + chunk < N ? i0 + chunk : N;
-  auto ready = [&](int64_t i) { return i < i1 && level[i] < 0 && indeg[i] == 0; };
+  auto ready = [&](int64_t i) { return i < i1 && mark[i] == 1; };
   int c = 0;
   for (int64_t i = i0 + tid; i < i1; i += 256) c += ready(i);
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
@@ -1252,6 +1265,7 @@ __global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, con
     s_base = total ? atomicAdd(&cnt[L + 1], total) : 0;
   }
   __syncthreads();
+  if (s_red[0] + s_red[1] + s_red[2] + s_red[3] == 0) return;  // uniform per block
   int running = s_base;
   for (int64_t b0 = i0; b0 < i1; b0 += 256) {
     const int64_t i = b0 + tid;
@@ -1264,6 +1278,7 @@ __global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, con
     for (int v = 0; v < w; ++v) before += s_red[v];
     if (r) {
       level[i] = L + 1;
+      mark[i] = 2;
       order[next + running + before + mbcnt(b)] = (int32_t)i;
     }
     running += s_red[0] + s_red[1] + s_red[2] + s_red[3];
@@ -1871,6 +1886,7 @@ namespace {
 // context's allocator hands out blocks in main-stream order only).
 struct LevelBufs {
   DevBuf<int32_t> rowptr, cursor, deps, cnt, off, depth;
+  DevBuf<uint8_t> mark;  // 0 waiting, 1 ready, 2 placed
   DevBuf<char> tmp;
   size_t tmp_bytes = 0;
 };
@@ -1882,6 +1898,7 @@ void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* 
   B.cnt.alloc((size_t)NT + 16);
   B.off.alloc((size_t)NT + 16);
   B.depth.alloc(1);
+  if (!B.mark.p) B.mark.alloc((size_t)NT);  // kept when the degree count has filled it
   B.deps.alloc((size_t)std::max<int64_t>(n_edges, 1));
   B.tmp_bytes = 0;
   GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, B.tmp_bytes, outdeg, B.rowptr.p,
@@ -1919,7 +1936,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
     GSP_HIP(hipStreamSynchronize(st));
     depth = d;
   } else {
-  hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, -1, indeg,
+  hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, -1, B.mark.p,
                      plan->level.p, plan->order.p, B.off.p, B.cnt.p);
   constexpr int kBatch = 16;
   const int fb = grid_for(NT / 16, 256, 1024);
@@ -1928,9 +1945,9 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
     GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
     for (int b = 0; b < kBatch; ++b) {
       hipLaunchKernelGGL(k_level_push, dim3(fb), dim3(256), 0, st, L + b, B.rowptr.p, B.deps.p,
-                         indeg, plan->order.p, B.off.p, B.cnt.p);
-      hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, L + b, indeg,
-                         plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+                         indeg, B.mark.p, plan->order.p, B.off.p, B.cnt.p);
+      hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, L + b,
+                         B.mark.p, plan->level.p, plan->order.p, B.off.p, B.cnt.p);
     }
     GSP_HIP(hipMemcpyAsync(hc.data(), B.cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, st));
     GSP_HIP(hipStreamSynchronize(st));
@@ -1960,14 +1977,19 @@ struct LevelJob {
   hipEvent_t a = nullptr, b = nullptr;  // timing (when enabled)
 };
 
-// The worker: the levels from the search on the side stream, after the fork.
-void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D,
-                    const int32_t* outdeg, const int32_t* indeg, LevelBufs& B, LevelJob& job) {
+// The worker: the levels from the search on the side stream, after the fork
+// (with the degree count first when `count`; outdeg is zero).
+void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D, int32_t* outdeg,
+                    int32_t* indeg, bool count, unsigned long long* edges, LevelBufs& B,
+                    LevelJob& job) {
   try {
     GSP_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->side;
     GSP_HIP(hipStreamWaitEvent(st, ctx->fork, 0));
     if (job.a) GSP_HIP(hipEventRecord(job.a, st));
+    if (count)
+      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N, 256, 4096)), dim3(256), 0, st, D,
+                         outdeg, indeg, B.mark.p, edges);
     build_levels<false>(plan, D, outdeg, indeg, B, st);
     if (job.b) GSP_HIP(hipEventRecord(job.b, st));
     GSP_HIP(hipStreamSynchronize(st));
@@ -2300,15 +2322,24 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   DevBuf<unsigned long long> edges;
   edges.alloc(1);
   GSP_HIP(hipMemsetAsync(edges.p, 0, 8, s));
-  hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
-                     outdeg.p, indeg.p, edges.p);
-  unsigned long long n_edges = 0;
-  GSP_HIP(hipMemcpyAsync(&n_edges, edges.p, 8, hipMemcpyDeviceToHost, s));
-  GSP_HIP(hipStreamSynchronize(s));
-  GSP_REQUIRE(n_edges < (1ull << 31), GSP_ERR_UNSUPPORTED,
-              "too many dependencies (%llu) for the level computation", n_edges);
   LevelBufs LB;
-  level_bufs_alloc(LB, NT, (int64_t)n_edges, outdeg.p, s);  // every buffer the worker uses
+  // The degree count runs on the side stream when the reverse edges fit the
+  // bound sum(nfound) <= NT * K in 32-bit offsets (the search's output is
+  // then all the main stream waits for); else here, for the exact count.
+  const bool side_count = (int64_t)NT * K < ((int64_t)1 << 31);
+  if (side_count) {
+    level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
+  } else {
+    LB.mark.alloc((size_t)NT);
+    hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
+                       outdeg.p, indeg.p, LB.mark.p, edges.p);
+    unsigned long long n_edges = 0;
+    GSP_HIP(hipMemcpyAsync(&n_edges, edges.p, 8, hipMemcpyDeviceToHost, s));
+    GSP_HIP(hipStreamSynchronize(s));
+    GSP_REQUIRE(n_edges < (1ull << 31), GSP_ERR_UNSUPPORTED,
+                "too many dependencies (%llu) for the level computation", n_edges);
+    level_bufs_alloc(LB, NT, (int64_t)n_edges, outdeg.p, s);
+  }
   GSP_HIP(hipEventRecord(ctx->fork, s));
   LevelJob job;
   if (ctx->timing) {
@@ -2316,7 +2347,9 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     job.b = take_event(ctx);
   }
   {
-    WorkerThread worker([&] { levels_on_side(ctx, plan, D, outdeg.p, indeg.p, LB, job); });
+    WorkerThread worker([&] {
+      levels_on_side(ctx, plan, D, outdeg.p, indeg.p, side_count, edges.p, LB, job);
+    });
   {
     // learned plans: the surrogate takes the nodes it can, the rest are listed
     // for the exact solver
@@ -2406,7 +2439,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
                        plan->level.p, -1, NT);
     hipLaunchKernelGGL(k_deg_count<true>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
-                       outdeg.p, indeg.p, edges.p);
+                       outdeg.p, indeg.p, LB.mark.p, edges.p);
     build_levels<true>(plan, D, outdeg.p, indeg.p, LB, s);
   }
   tr.mark("levels");

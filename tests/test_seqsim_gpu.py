@@ -72,6 +72,28 @@ def test_plan_parity_fp64(ctx, i, mode):
     check_plan(ctx, case, brute=case.n_nodes <= 1000)
 
 
+@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
+def test_plan_parity_large_levels(ctx, mode):
+    """34,560 nodes, more than the one-workgroup level pass takes: the levels
+    come from the launches per level, and are built again after failed systems
+    (a duplicated datum makes every system holding both copies singular)."""
+    rng = np.random.default_rng(77)
+    dims = (192, 180)
+    data = rng.uniform(0, 1, (50, 2)) * np.asarray(dims, float)
+    data[1] = [40.5, 60.5]
+    data[2] = data[1]
+    case = O.SeqsimCase(dims=dims, family=O.SPHERICAL, ranges=(8.0,), kmax=12, radius=10.0,
+                        data_coords=data, data_vals=rng.standard_normal(50),
+                        path=rng.permutation(192 * 180), mode=mode)
+    gp, o = check_plan(ctx, case)
+    assert np.count_nonzero(o.flags & O.FLAG_FAIL) > 0
+    assert np.count_nonzero(o.flags & O.FLAG_FEW) > 0
+    # the sweep visits every node once, after its neighbours
+    z = np.random.default_rng(3).standard_normal((3, case.n_nodes))
+    plan = O.Plan(case.kmax, o.nfound, o.nbr, o.lam, o.sigma, o.flags, o.level)
+    np.testing.assert_array_equal(gp.sweep(z, case.data_vals), O.seqsim_sweep(case, plan, z))
+
+
 @pytest.mark.parametrize("i", [1, 3, 4, 5])
 def test_plan_parity_fp32_refine(ctx, i):
     case = make_case(i, O.KNN_THEN_MASK, precision=O.FP32_REFINE)
