@@ -220,10 +220,37 @@ def _indicator_problem(process, method, domain, data, ctx):
 
 # ----------------------------------------------------------------- LUSIM ----
 class LusimPreproc:
-    def __init__(self, plan, names, mean):
-        self.plan = plan
+    """One factorized plan per variable (lusim.jl:64-107); `rho` couples the
+    second variable's normals to the first's (lusim.jl:118-120)."""
+
+    def __init__(self, plans, names, rho=None):
+        self.plans = plans
         self.names = names
-        self.mean = mean
+        self.rho = rho
+
+    @property
+    def plan(self):
+        return self.plans[0]
+
+
+def _marginal_struct(f, j, ndim):
+    """_marginalize (lusim.jl:132-137) of A * c: the covariance A[j, j] c(h) of
+    variable j (its nugget scaled with it)."""
+    if not isinstance(f, CoregionalizedFunction):
+        raise _lib.UnsupportedError(
+            "bivariate LUSIM needs a coregionalized function A * c (intrinsic model)")
+    m = f.base.struct(ndim)
+    m.sill = float(f.A[j, j]) * m.sill
+    m.nugget = float(f.A[j, j]) * m.nugget
+    return m
+
+
+def _rho(f):
+    """_rho (lusim.jl:139-143): C12(0) / sqrt(s1 s2), with C(0) = sill for a
+    covariance, so that rho = A[0, 1] / sqrt(A[0, 0] A[1, 1])."""
+    c12 = f.A[0, 1] * f.base.sill()
+    s1, s2 = np.diag(f.sill())
+    return float(c12 / np.sqrt(s1 * s2))
 
 
 def _lusim_problem(process, method, init, domain, data, ctx):
@@ -233,22 +260,32 @@ def _lusim_problem(process, method, init, domain, data, ctx):
     names = tuple(real)
     assert len(names) == f.nvariables(), "incompatible number of variables for geostatistical function"
     assert len(names) in (1, 2), "LUSIM only supports univariate and bivariate simulation"
-    if len(names) != 1:
-        raise _lib.UnsupportedError("bivariate LUSIM is not supported by this backend")
-    var = names[0]
-    dinds = np.ascontiguousarray(np.flatnonzero(mask[var]), dtype=np.int64)
-    z1 = np.ascontiguousarray(real[var][dinds], dtype=np.float64)
-    p = _lib.GspLusimProblem()
-    p.grid = grid
-    p.model = f.struct(domain.ndim)
-    p.mean = float(process.mean)
-    p.n_data = len(dinds)
-    p.dinds = dinds.ctypes.data if len(dinds) else None
-    p.z1 = z1.ctypes.data if len(dinds) else None
-    p.precision = method.precision
-    p.device_pointers = 0
-    plan = _lib.LusimPlan(ctx, p, keep=(dinds, z1))
-    return LusimPreproc(plan, names, float(process.mean))
+    means = np.atleast_1d(np.asarray(process.mean, dtype=np.float64))
+    plans = []
+    for j, var in enumerate(names):
+        dinds = np.ascontiguousarray(np.flatnonzero(mask[var]), dtype=np.int64)
+        z1 = np.ascontiguousarray(real[var][dinds], dtype=np.float64)
+        p = _lib.GspLusimProblem()
+        p.grid = grid
+        p.model = (_marginal_struct(f, j, domain.ndim)
+                   if len(names) == 2 or isinstance(f, CoregionalizedFunction)
+                   else f.struct(domain.ndim))
+        p.mean = float(means[j])
+        p.n_data = len(dinds)
+        p.dinds = dinds.ctypes.data if len(dinds) else None
+        p.z1 = z1.ctypes.data if len(dinds) else None
+        p.precision = method.precision
+        p.device_pointers = 0
+        plans.append(_lib.LusimPlan(ctx, p, keep=(dinds, z1)))
+    rho = None
+    if len(names) == 2:
+        rho = _rho(f)
+        if plans[0].n_sim != plans[1].n_sim:
+            # the reference mixes w1 into the second variable's draw (lusim.jl:162)
+            raise _lib.ArgumentError(
+                "bivariate LUSIM draws both variables from the same normals: the variables "
+                f"need as many simulated cells ({plans[0].n_sim} != {plans[1].n_sim})")
+    return LusimPreproc(plans, names, rho)
 
 
 # ---------------------------------------------------------------- FFTSIM ----
@@ -320,25 +357,35 @@ def preprocess(rng, process, method, init, domain, data, ctx=None):
 
 
 def _draw(rng, preproc, R):
+    """R realizations of every variable: {name: (R, N) array}."""
+    name = preproc.names[0]
     if isinstance(preproc, IndicatorPreproc):
         # one uniform per path node for the categorical draw (seqsim.jl:342-347)
         u = rng.random((R, preproc.plan.n_nodes))
         cats = preproc.plan.sweep_indicator(u, preproc.prob, preproc.data_cat)
-        return preproc.labels[cats]
+        return {name: preproc.labels[cats]}
     if isinstance(preproc, SeqsimPreproc):
         z = rng.standard_normal((R, preproc.plan.n_nodes))
-        return preproc.plan.sweep(z, preproc.data_vals)
+        return {name: preproc.plan.sweep(z, preproc.data_vals)}
     if isinstance(preproc, FftsimPreproc):
         # uniform white noise, one value per cell (fftsim.jl:124)
         w = rng.random((R, preproc.plan.n_nodes))
-        return preproc.plan.sample(w)
-    w = rng.standard_normal((R, preproc.plan.n_sim))
-    return preproc.plan.sample(w)
+        return {name: preproc.plan.sample(w)}
+    # LUSIM: per realization the first variable's normals, then the second's
+    # (lusim.jl:112-126, 160)
+    ns = [pl.n_sim for pl in preproc.plans]
+    w = rng.standard_normal((R, sum(ns)))
+    w1 = w[:, :ns[0]]
+    out = {name: preproc.plans[0].sample(w1)}
+    if len(preproc.plans) == 2:
+        w2 = w[:, ns[0]:]
+        rho = preproc.rho
+        out[preproc.names[1]] = preproc.plans[1].sample(rho * w1 + np.sqrt(1.0 - rho ** 2) * w2)
+    return out
 
 
 def randsingle(rng, process, method, domain, data, preproc):
-    out = _draw(rng, preproc, 1)
-    return {preproc.names[0]: out[0]}
+    return {k: v[0] for k, v in _draw(rng, preproc, 1).items()}
 
 
 def rand(process, domain, nreals=None, *, rng=None, data=None, method=None, init=None,
@@ -354,5 +401,4 @@ def rand(process, domain, nreals=None, *, rng=None, data=None, method=None, init
     preproc = preprocess(rng, process, smethod, init, domain, data, ctx=ctx)
     if nreals is None:
         return GeoTable(randsingle(rng, process, smethod, domain, data, preproc), domain)
-    out = _draw(rng, preproc, int(nreals))
-    return Ensemble(domain, {preproc.names[0]: out}, ctx=ctx)
+    return Ensemble(domain, _draw(rng, preproc, int(nreals)), ctx=ctx)

@@ -165,3 +165,54 @@ def test_lusim_all_cells_are_data(ctx):
     assert plan.n_sim == 0
     out = plan.sample(np.zeros((2, 0)))
     np.testing.assert_array_equal(out, np.broadcast_to(z1, (2, 12)))
+
+
+def test_lusim_bivariate_matches_oracle(ctx):
+    """Bivariate LUSIM (lusim.jl:64-126, 132-143) for A * c with data on both
+    variables: each variable is the oracle's univariate LUSIM with the
+    covariance A[j, j] c(h), the second one driven by rho w1 + sqrt(1 - rho^2) w2."""
+    A = np.array([[1.0, 0.8], [0.8, 2.0]])
+    proc = gsp.GaussianProcess(A * gsp.SphericalCovariance(range=9.0, sill=1.5, nugget=0.05),
+                               [0.1, -0.3])
+    grid = gsp.CartesianGrid(14, 11)
+    pts = [(2.5, 3.5), (7.5, 8.5), (11.5, 2.5), (4.5, 9.5)]
+    data = gsp.georef({"Cu": [0.3, -0.2, 0.5, 0.0], "Zn": [1.0, 0.4, -0.6, 0.2]}, pts)
+    R = 6
+    ens = gsp.rand(proc, grid, R, rng=np.random.default_rng(21), method=gsp.LUSIM(), data=data)
+    real, mask = gsp.initialize(proc, grid, data, gsp.NearestInit())
+    n_s = grid.nelements() - int(mask["Cu"].sum())
+    w = np.random.default_rng(21).standard_normal((R, 2 * n_s))
+    rho = 0.8 / np.sqrt(2.0)
+    for j, (var, wj) in enumerate([("Cu", w[:, :n_s]),
+                                   ("Zn", rho * w[:, :n_s] + np.sqrt(1 - rho ** 2) * w[:, n_s:])]):
+        dinds = np.flatnonzero(mask[var])
+        z1 = real[var][dinds]
+        d2, L22, sinds = O.lusim_preprocess(grid.centroids(), dinds, z1, O.SPHERICAL,
+                                            1.5 * A[j, j], 0.05 * A[j, j], (9.0, 9.0))
+        ref = O.lusim_sample(d2, L22, dinds, sinds, z1, 0.0, wj)
+        got = np.stack([ens[r][var] for r in range(R)])
+        say(f"bivariate {var}: rel err {rel_err(got, ref):.2e}")
+        assert rel_err(got, ref) <= 1e-10
+        np.testing.assert_array_equal(got[:, dinds], np.broadcast_to(z1, (R, len(dinds))))
+
+
+def test_lusim_bivariate_cross_covariance(ctx):
+    """The construction reproduces the coregionalization: over many
+    unconditional realizations Cov(z1(x), z2(y)) = A12 C(x - y) and
+    Var z2 = A22 C(0), which fixes rho and the sqrt(1 - rho^2) factor."""
+    A = np.array([[1.0, 0.7], [0.7, 2.0]])
+    base = gsp.ExponentialCovariance(range=6.0, sill=1.0)
+    proc = gsp.GaussianProcess(A * base, [0.0, 0.0])
+    grid = gsp.CartesianGrid(12, 10)
+    R = 4000
+    ens = gsp.rand(proc, grid, R, rng=np.random.default_rng(8), method=gsp.LUSIM())
+    z1 = np.stack([ens[r]["field1"] for r in range(R)])
+    z2 = np.stack([ens[r]["field2"] for r in range(R)])
+    c0 = np.mean(z1 * z2)  # lag 0, every cell
+    # lag 1 along x (cells are x-fastest: row-major (ny, nx) = (10, 12))
+    c1 = np.mean(z1.reshape(R, 10, 12)[:, :, :-1] * z2.reshape(R, 10, 12)[:, :, 1:])
+    v2 = np.mean(z2 * z2)
+    C1 = float(O.cov_u(O.EXPONENTIAL, 1.0, 0.0, 1.0 / 6.0))
+    say(f"cross cov lag0 {c0:.3f} (expect {0.7:.3f}), lag1 {c1:.3f} (expect {0.7 * C1:.3f}), "
+        f"var z2 {v2:.3f} (expect 2)")
+    assert abs(c0 - 0.7) < 0.05 and abs(c1 - 0.7 * C1) < 0.05 and abs(v2 - 2.0) < 0.1

@@ -38,6 +38,40 @@ def test_lusim_reference_cases(ctx):
         gsp.rand(proc, grid, rng=rng, method=method, data=data)
 
 
+def test_lusim_bivariate_reference_cases(ctx):
+    """test/field.jl:29-37 and 58-70 (without units): cosimulation of A * c,
+    unconditional (variables field1, field2) and with data (Cu, Zn)."""
+    method = gsp.LUSIM()
+    rng = np.random.default_rng(123)
+    grid = gsp.CartesianGrid(100)
+    # field.jl:31-37 pair a GaussianCovariance (nugget 0, range 10) with a unit
+    # grid: singular to working precision, as in the univariate cases
+    proc = gsp.GaussianProcess(np.array([[1.0, 0.95], [0.95, 1.0]])
+                               * gsp.GaussianCovariance(range=10.0), [0.0, 0.0])
+    with pytest.raises(gsp.PosDefException):
+        gsp.rand(proc, grid, rng=rng, method=method)
+    proc = gsp.GaussianProcess(np.array([[1.0, 0.95], [0.95, 1.0]])
+                               * gsp.SphericalCovariance(range=10.0), [0.0, 0.0])
+    real = gsp.rand(proc, grid, rng=rng, method=method)
+    assert real.field1.dtype == np.float64 and real.field2.dtype == np.float64
+    func = np.array([[1.0, 0.8], [0.8, 1.0]]) * gsp.SphericalCovariance(range=35.0, sill=1.0)
+    proc = gsp.GaussianProcess(func, [0.1, 0.2])
+    grid = gsp.CartesianGrid(10, 10)
+    data = gsp.georef({"Cu": [0.0, 0.1, 0.0], "Zn": [0.1, 0.0, 0.1]},
+                      [(25.0, 25.0), (50.0, 75.0), (75.0, 50.0)])
+    real = gsp.rand(proc, grid, rng=rng, method=method)
+    assert real.field1.dtype == np.float64 and real.field2.dtype == np.float64
+    real = gsp.rand(proc, grid, rng=rng, method=method, data=data)
+    # the points lie outside the 10 x 10 unit grid and snap to one corner cell,
+    # where the later datum wins
+    dest = np.asarray(gsp.NearestInit().destinations(grid, data))
+    for d in np.unique(dest):
+        last = np.flatnonzero(dest == d)[-1]
+        assert real.Cu[d] == data.Cu[last] and real.Zn[d] == data.Zn[last]
+    ens = gsp.rand(proc, grid, 3, rng=rng, method=method, data=data)
+    assert len(ens) == 3 and ens[2].Zn.dtype == np.float64
+
+
 def test_lusim_ensemble_matches_single_draws(ctx):
     """The batched ensemble equals R sequential randsingle calls with the same stream."""
     proc = gsp.GaussianProcess(gsp.ExponentialCovariance(range=6.0, nugget=0.05), 0.4)
