@@ -1128,32 +1128,28 @@ __global__ void k_dep_fill(const DepArgs D, const int32_t* rowptr, int32_t* curs
   }
 }
 
-// Level L + 1 in two launches. Push: every node of frontier L (order[off[L]
-// .. off[L] + cnt[L])) decrements the counters of its dependents, eight at a
-// time so that their atomics are in flight together; the one that takes a
-// counter to zero marks that dependent ready (mark 1). Collect: every node
-// marked ready gets level L + 1, mark 2 (placed) and a slot in frontier
-// L + 1 (one atomic per block; ids ascend inside each block's run), reading
-// one byte per node. L = -1 collects level 0 (nodes without grid neighbours,
-// marked by the degree count).
+// Level L + 1 in three launches. Push: every node of frontier L (order[off[L]
+// .. off[L] + cnt[L])) decrements the counters of its dependents, sixteen lanes
+// per node, one edge each; the atomic that takes a counter to zero marks that
+// dependent ready (mark 1). Count and place: the marks, sixteen per thread in
+// one 16-byte load, give every block's number of ready nodes, then every
+// block adds up the counts of the blocks before it and writes its ready nodes
+// in id order: level L + 1, mark 2 (placed) and a slot in frontier L + 1. No
+// same-address atomics, and each level lists its nodes by ascending id. L = -1
+// places level 0 (nodes without grid neighbours, marked by the degree count).
 __global__ __launch_bounds__(256) void k_level_push(int32_t L, const int32_t* rowptr,
                                                     const int32_t* deps, int32_t* indeg,
                                                     uint8_t* mark, const int32_t* order,
                                                     const int32_t* off, const int32_t* cnt) {
   const int32_t start = off[L], size = cnt[L];
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < size;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  const int sub = threadIdx.x & 15;
+  for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; t < size;
+       t += ((int64_t)gridDim.x * blockDim.x) >> 4) {
     const int a = order[start + t];
     const int e1 = rowptr[a + 1];
-    for (int e0 = rowptr[a]; e0 < e1; e0 += 8) {
-      int d[8], old[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) d[u] = e0 + u < e1 ? deps[e0 + u] : -1;
-#pragma unroll
-      for (int u = 0; u < 8; ++u) old[u] = d[u] >= 0 ? atomicSub(&indeg[d[u]], 1) : 0;
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (old[u] == 1) mark[d[u]] = 1;
+    for (int e = rowptr[a] + sub; e < e1; e += 16) {
+      const int d = deps[e];
+      if (atomicSub(&indeg[d], 1) == 1) mark[d] = 1;
     }
   }
 }
@@ -1240,48 +1236,92 @@ __global__ __launch_bounds__(1024) void k_levels_small(int32_t N, const int32_t*
   }
 }
 
-// Each block takes a contiguous chunk of node ids: it counts its ready nodes,
-// reserves their slots with one atomic, then writes them in id order (same-
-// address atomics serialise, so one per block rather than one per wavefront).
-__global__ __launch_bounds__(256) void k_level_collect(int32_t N, int32_t L, uint8_t* mark,
-                                                       int32_t* level, int32_t* order,
-                                                       int32_t* off, int32_t* cnt) {
+// Ready nodes among 16 marks (bytes 0, 1 or 2: ready is the only odd value).
+__device__ __forceinline__ int ready16(const uint4 m) {
+  return __popc(m.x & 0x01010101u) + __popc(m.y & 0x01010101u) + __popc(m.z & 0x01010101u) +
+         __popc(m.w & 0x01010101u);
+}
+
+// Each block takes a contiguous run of 16-node groups (the marks are padded
+// to whole groups with zeros).
+__device__ __forceinline__ void collect_chunk(int64_t n16, int64_t& g0, int64_t& g1) {
+  const int64_t chunk = ((n16 + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+  g0 = (int64_t)blockIdx.x * chunk;
+  g1 = g0 + chunk < n16 ? g0 + chunk : n16;
+}
+
+__device__ __forceinline__ int block_sum256(int v, int32_t* s_red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();  // s_red free
+  if (lane == 0) s_red[w] = v;
+  __syncthreads();
+  return s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+__global__ __launch_bounds__(256) void k_level_count(int64_t n16, const uint4* mark16,
+                                                     int32_t* blockcnt) {
   __shared__ int32_t s_red[4];
-  __shared__ int32_t s_base;
-  const int32_t next = L < 0 ? 0 : off[L] + cnt[L];
-  if (blockIdx.x == 0 && threadIdx.x == 0) off[L + 1] = next;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int64_t chunk = ((N + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
-  const int64_t i0 = (int64_t)blockIdx.x * chunk;
-  const int64_t i1 = i0 + chunk < N ? i0 + chunk : N;
-  auto ready = [&](int64_t i) { return i < i1 && mark[i] == 1; };
+  int64_t g0, g1;
+  collect_chunk(n16, g0, g1);
   int c = 0;
-  for (int64_t i = i0 + tid; i < i1; i += 256) c += ready(i);
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if (lane == 0) s_red[w] = c;
-  __syncthreads();
+  for (int64_t g = g0 + threadIdx.x; g < g1; g += 256) c += ready16(mark16[g]);
+  c = block_sum256(c, s_red);
+  if (threadIdx.x == 0) blockcnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void k_level_place(int64_t n16, int32_t L, uint4* mark16,
+                                                     const int32_t* blockcnt, int32_t* level,
+                                                     int32_t* order, int32_t* off,
+                                                     int32_t* cnt) {
+  __shared__ int32_t s_red[4];
+  __shared__ int32_t s_scan[4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int32_t next = L < 0 ? 0 : off[L] + cnt[L];
+  // ready nodes in the blocks before this one
+  int pre = 0;
+  for (int b = tid; b < (int)blockIdx.x; b += 256) pre += blockcnt[b];
+  pre = block_sum256(pre, s_red);
+  const int mine = blockcnt[blockIdx.x];
   if (tid == 0) {
-    const int total = s_red[0] + s_red[1] + s_red[2] + s_red[3];
-    s_base = total ? atomicAdd(&cnt[L + 1], total) : 0;
+    if (blockIdx.x == 0) off[L + 1] = next;
+    if (blockIdx.x == gridDim.x - 1) cnt[L + 1] = pre + mine;
   }
-  __syncthreads();
-  if (s_red[0] + s_red[1] + s_red[2] + s_red[3] == 0) return;  // uniform per block
-  int running = s_base;
-  for (int64_t b0 = i0; b0 < i1; b0 += 256) {
-    const int64_t i = b0 + tid;
-    const bool r = ready(i);
-    const uint64_t b = __ballot(r);
-    __syncthreads();  // s_red of the previous step has been read
-    if (lane == 0) s_red[w] = __popcll(b);
+  if (mine == 0) return;  // uniform per block
+  int64_t g0, g1;
+  collect_chunk(n16, g0, g1);
+  int32_t running = next + pre;
+  for (int64_t b0 = g0; b0 < g1; b0 += 256) {
+    const int64_t g = b0 + tid;
+    const uint4 m = g < g1 ? mark16[g] : make_uint4(0u, 0u, 0u, 0u);
+    const int c = ready16(m);
+    int incl = c;  // inclusive scan over the block
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    __syncthreads();  // s_scan of the previous step has been read
+    if (lane == 63) s_scan[w] = incl;
     __syncthreads();
     int before = 0;
-    for (int v = 0; v < w; ++v) before += s_red[v];
-    if (r) {
-      level[i] = L + 1;
-      mark[i] = 2;
-      order[next + running + before + mbcnt(b)] = (int32_t)i;
+    for (int v = 0; v < w; ++v) before += s_scan[v];
+    if (c) {
+      int32_t slot = running + before + incl - c;
+      const uint32_t words[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (((words[q] >> (8 * k)) & 0xFFu) == 1u) {
+            const int32_t i = (int32_t)(g * 16 + 4 * q + k);
+            order[slot++] = i;
+            level[i] = L + 1;
+          }
+      // ready (1) becomes placed (2); 0 and 2 stay
+      mark16[g] = make_uint4(m.x + (m.x & 0x01010101u), m.y + (m.y & 0x01010101u),
+                             m.z + (m.z & 0x01010101u), m.w + (m.w & 0x01010101u));
     }
-    running += s_red[0] + s_red[1] + s_red[2] + s_red[3];
+    running += s_scan[0] + s_scan[1] + s_scan[2] + s_scan[3];
   }
 }
 
@@ -1885,11 +1925,18 @@ namespace {
 // Buffers of the level computation, all made before the worker starts (the
 // context's allocator hands out blocks in main-stream order only).
 struct LevelBufs {
-  DevBuf<int32_t> rowptr, cursor, deps, cnt, off, depth;
-  DevBuf<uint8_t> mark;  // 0 waiting, 1 ready, 2 placed
+  DevBuf<int32_t> rowptr, cursor, deps, cnt, off, depth, blockcnt;
+  DevBuf<uint8_t> mark;  // 0 waiting, 1 ready, 2 placed; zero padding to 16 nodes
   DevBuf<char> tmp;
   size_t tmp_bytes = 0;
 };
+
+// The ready marks: whole 16-node groups, zero beyond the last node.
+void mark_alloc(LevelBufs& B, int64_t NT, hipStream_t s) {
+  const size_t bytes = (size_t)((NT + 15) / 16 * 16);
+  B.mark.alloc(bytes);
+  GSP_HIP(hipMemsetAsync(B.mark.p, 0, bytes, s));
+}
 
 void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* outdeg,
                       hipStream_t s) {
@@ -1898,7 +1945,8 @@ void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* 
   B.cnt.alloc((size_t)NT + 16);
   B.off.alloc((size_t)NT + 16);
   B.depth.alloc(1);
-  if (!B.mark.p) B.mark.alloc((size_t)NT);  // kept when the degree count has filled it
+  B.blockcnt.alloc(1024);
+  if (!B.mark.p) mark_alloc(B, NT, s);  // kept when the degree count has filled it
   B.deps.alloc((size_t)std::max<int64_t>(n_edges, 1));
   B.tmp_bytes = 0;
   GSP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, B.tmp_bytes, outdeg, B.rowptr.p,
@@ -1936,18 +1984,23 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
     GSP_HIP(hipStreamSynchronize(st));
     depth = d;
   } else {
-  hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, -1, B.mark.p,
-                     plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+  const int64_t n16 = (NT + 15) / 16;
+  uint4* mark16 = reinterpret_cast<uint4*>(B.mark.p);
+  auto place = [&](int32_t L) {
+    hipLaunchKernelGGL(k_level_count, dim3(cb), dim3(256), 0, st, n16, mark16, B.blockcnt.p);
+    hipLaunchKernelGGL(k_level_place, dim3(cb), dim3(256), 0, st, n16, L, mark16, B.blockcnt.p,
+                       plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+  };
+  place(-1);
   constexpr int kBatch = 16;
-  const int fb = grid_for(NT / 16, 256, 1024);
+  const int fb = grid_for(NT, 256, 2048);
   std::vector<int32_t> hc(kBatch + 1);
   for (int32_t L = 0; depth < 0; L += kBatch) {
     GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
     for (int b = 0; b < kBatch; ++b) {
       hipLaunchKernelGGL(k_level_push, dim3(fb), dim3(256), 0, st, L + b, B.rowptr.p, B.deps.p,
                          indeg, B.mark.p, plan->order.p, B.off.p, B.cnt.p);
-      hipLaunchKernelGGL(k_level_collect, dim3(cb), dim3(256), 0, st, (int32_t)NT, L + b,
-                         B.mark.p, plan->level.p, plan->order.p, B.off.p, B.cnt.p);
+      place(L + b);
     }
     GSP_HIP(hipMemcpyAsync(hc.data(), B.cnt.p + L, (kBatch + 1) * 4, hipMemcpyDeviceToHost, st));
     GSP_HIP(hipStreamSynchronize(st));
@@ -2330,7 +2383,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   if (side_count) {
     level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
   } else {
-    LB.mark.alloc((size_t)NT);
+    mark_alloc(LB, NT, s);
     hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
                        outdeg.p, indeg.p, LB.mark.p, edges.p);
     unsigned long long n_edges = 0;
