@@ -1946,6 +1946,11 @@ void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* 
   B.tmp.alloc(B.tmp_bytes);
 }
 
+// Blocks of the passes over the edges (degree count, fill, push). They are
+// bound by device atomics, which 512 blocks of 256 threads keep busy; more
+// would only hold wave slots that the solves on the main stream need.
+constexpr int kEdgeBlocks = 512;
+
 // Kahn's algorithm on stream `st`: reverse CSR, level 0, then two launches
 // per level; the frontier sizes are checked every kBatch levels. Sets
 // plan->level, order, depth and level_off.
@@ -1954,7 +1959,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
                   const int32_t* indeg_in, LevelBufs& B, hipStream_t st) {
   int32_t* indeg = const_cast<int32_t*>(indeg_in);
   const int64_t NT = D.N;
-  const int nb = grid_for(NT, 256, 4096);
+  const int nb = grid_for(NT / 16, 256, kEdgeBlocks);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 16, 256, 4096)), dim3(256), 0, st, B.cnt.p,
                      0, NT + 16);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, B.off.p, 0, (int64_t)1);
@@ -1985,7 +1990,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
   };
   place(-1);
   constexpr int kBatch = 16;
-  const int fb = grid_for(NT, 256, 2048);
+  const int fb = grid_for(NT / 16, 256, kEdgeBlocks);
   std::vector<int32_t> hc(kBatch + 1);
   for (int32_t L = 0; depth < 0; L += kBatch) {
     GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
@@ -2033,7 +2038,7 @@ void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D, int32
     GSP_HIP(hipStreamWaitEvent(st, ctx->fork, 0));
     if (job.a) GSP_HIP(hipEventRecord(job.a, st));
     if (count)
-      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N, 256, 4096)), dim3(256), 0, st, D,
+      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N / 16, 256, kEdgeBlocks)), dim3(256), 0, st, D,
                          outdeg, indeg, B.mark.p, edges);
     build_levels<false>(plan, D, outdeg, indeg, B, st);
     if (job.b) GSP_HIP(hipEventRecord(job.b, st));
