@@ -1959,7 +1959,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
                   const int32_t* indeg_in, LevelBufs& B, hipStream_t st) {
   int32_t* indeg = const_cast<int32_t*>(indeg_in);
   const int64_t NT = D.N;
-  const int nb = grid_for(NT / 16, 256, kEdgeBlocks);
+  const int nb = grid_for(NT, 16, kEdgeBlocks);  // sixteen nodes per block
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 16, 256, 4096)), dim3(256), 0, st, B.cnt.p,
                      0, NT + 16);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, B.off.p, 0, (int64_t)1);
@@ -1990,7 +1990,7 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
   };
   place(-1);
   constexpr int kBatch = 16;
-  const int fb = grid_for(NT / 16, 256, kEdgeBlocks);
+  const int fb = grid_for(NT, 16, kEdgeBlocks);
   std::vector<int32_t> hc(kBatch + 1);
   for (int32_t L = 0; depth < 0; L += kBatch) {
     GSP_REQUIRE(L + kBatch + 1 < NT + 16, GSP_ERR_INTERNAL, "dependency levels did not close");
@@ -2038,7 +2038,7 @@ void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D, int32
     GSP_HIP(hipStreamWaitEvent(st, ctx->fork, 0));
     if (job.a) GSP_HIP(hipEventRecord(job.a, st));
     if (count)
-      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N / 16, 256, kEdgeBlocks)), dim3(256), 0, st, D,
+      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N, 16, kEdgeBlocks)), dim3(256), 0, st, D,
                          outdeg, indeg, B.mark.p, edges);
     build_levels<false>(plan, D, outdeg, indeg, B, st);
     if (job.b) GSP_HIP(hipEventRecord(job.b, st));
@@ -2381,7 +2381,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
   } else {
     mark_alloc(LB, NT, s);
-    hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
+    hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 16, 4096)), dim3(256), 0, s, D,
                        outdeg.p, indeg.p, LB.mark.p, edges.p);
     unsigned long long n_edges = 0;
     GSP_HIP(hipMemcpyAsync(&n_edges, edges.p, 8, hipMemcpyDeviceToHost, s));
@@ -2488,7 +2488,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
                        0, NT + 1);
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
                        plan->level.p, -1, NT);
-    hipLaunchKernelGGL(k_deg_count<true>, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s, D,
+    hipLaunchKernelGGL(k_deg_count<true>, dim3(grid_for(NT, 16, 4096)), dim3(256), 0, s, D,
                        outdeg.p, indeg.p, LB.mark.p, edges.p);
     build_levels<true>(plan, D, outdeg.p, indeg.p, LB, s);
   }
