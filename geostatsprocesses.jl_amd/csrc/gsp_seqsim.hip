@@ -645,8 +645,10 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
   }
 }
 
-// Wavefronts per block of the kriging kernel (LDS per wave grows with KB^2).
-constexpr int plan_waves(int KB) { return KB <= 32 ? 4 : (KB <= 48 ? 2 : 1); }
+// Wavefronts per block of the kriging kernel. LDS per wavefront grows with
+// KB^2 and limits the resident wavefronts beyond KB = 24: single-wavefront
+// blocks pack the most per CU there (KB = 32: eleven instead of eight).
+constexpr int plan_waves(int KB) { return KB <= 24 ? 4 : 1; }
 
 // Kriging systems with more than 16 neighbours, one wavefront per node:
 // covariance assembly, Cholesky solve, weights, sigma and flags, plus the
@@ -770,16 +772,21 @@ __device__ inline bool packed_fp64(Krige16Lds<PREC>& L, int g, int r, int nmax, 
   return fail;
 }
 
+// One wavefront per block: its 15 KB of LDS then packs ten wavefronts per CU,
+// where blocks of four fit only twice (the kernel is latency-bound, so the
+// resident wavefronts set its rate).
+constexpr int k16Waves = 1;
+
 template <int PREC>
-__global__ __launch_bounds__(256) void k_krige16(const PlanArgs A, const int32_t* list,
-                                                 int32_t count) {
-  __shared__ Krige16Lds<PREC> s_lds[kWaves];
+__global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, const int32_t* list,
+                                                         int32_t count) {
+  __shared__ Krige16Lds<PREC> s_lds[k16Waves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int g = lane >> 4, r = lane & 15;
   Krige16Lds<PREC>& L = s_lds[w];
-  const int64_t stride = (int64_t)gridDim.x * kWaves * 4;
-  for (int64_t base = ((int64_t)blockIdx.x * kWaves + w) * 4; base < count; base += stride) {
+  const int64_t stride = (int64_t)gridDim.x * k16Waves * 4;
+  for (int64_t base = ((int64_t)blockIdx.x * k16Waves + w) * 4; base < count; base += stride) {
     const int64_t slot = base + g;
     const bool act = slot < count;
     const int gi = act ? list[slot] : list[base];
@@ -2459,13 +2466,13 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     TimedLaunch tl(ctx, GSP_K_PLAN);
     const int64_t small = upto[16];
     if (small > 0) {
-      const int blocks = grid_for((small + 3) / 4, kWaves, 16384);
+      const int blocks = grid_for((small + 3) / 4, k16Waves, 65536);
       if (precision == GSP_FP32_REFINE)
-        hipLaunchKernelGGL(k_krige16<GSP_FP32_REFINE>, dim3(blocks), dim3(256), 0, s, A, list.p,
-                           (int32_t)small);
+        hipLaunchKernelGGL(k_krige16<GSP_FP32_REFINE>, dim3(blocks), dim3(64 * k16Waves), 0, s, A,
+                           list.p, (int32_t)small);
       else
-        hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(256), 0, s, A, list.p,
-                           (int32_t)small);
+        hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(64 * k16Waves), 0, s, A,
+                           list.p, (int32_t)small);
     }
     launch_krige<24>(A, precision, list.p + upto[16], upto[24] - upto[16], s);
     launch_krige<32>(A, precision, list.p + upto[24], upto[32] - upto[24], s);
