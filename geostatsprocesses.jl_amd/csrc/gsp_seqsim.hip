@@ -995,6 +995,12 @@ __global__ __launch_bounds__(256) void k_n_scatter(const int32_t* nfound, const 
 }
 
 // ---------------------------------------------------------- small kernels --
+__global__ void k_iota_i32(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x)
@@ -2083,10 +2089,12 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   const int32_t devp = problems[0].device_pointers ? 1 : 0;
   int64_t NT = 0, nDT = 0, nC = 0;
   int32_t K = 1;
-  bool ring = false;  // some request searches without a distance bound
+  bool ring = false;     // some request searches without a distance bound
+  bool no_deps = true;   // every request searches data only: every node is level 0
   for (int32_t q = 0; q < n_req; ++q) {
     const int32_t mode = problems[q].search_mode;
     ring = ring || mode == GSP_DATA_ONLY || (mode == GSP_MASKED_KNN && !(problems[q].radius > 0));
+    no_deps = no_deps && mode == GSP_DATA_ONLY;
     for_request(n_req, q, [&] {
       const gsp_seqsim_problem* p = &problems[q];
       validate_problem(p, H[(size_t)q]);
@@ -2328,10 +2336,12 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
               "the path must be a permutation of the grid ids 0..N-1");
 
   DevBuf<int32_t> outdeg, indeg;
-  outdeg.alloc((size_t)NT + 1);
-  indeg.alloc((size_t)NT);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 1, 256, 4096)), dim3(256), 0, s, outdeg.p, 0,
-                     NT + 1);
+  if (!no_deps) {
+    outdeg.alloc((size_t)NT + 1);
+    indeg.alloc((size_t)NT);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT + 1, 256, 4096)), dim3(256), 0, s, outdeg.p,
+                       0, NT + 1);
+  }
   tr.mark("upload+positions");
   PlanArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -2377,14 +2387,18 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   D.flags = plan->flags.p;
   D.nbr = plan->nbr.p;
   DevBuf<unsigned long long> edges;
-  edges.alloc(1);
-  GSP_HIP(hipMemsetAsync(edges.p, 0, 8, s));
   LevelBufs LB;
   // The degree count runs on the side stream when the reverse edges fit the
   // bound sum(nfound) <= NT * K in 32-bit offsets (the search's output is
   // then all the main stream waits for); else here, for the exact count.
   const bool side_count = (int64_t)NT * K < ((int64_t)1 << 31);
-  if (side_count) {
+  if (!no_deps) {
+    edges.alloc(1);
+    GSP_HIP(hipMemsetAsync(edges.p, 0, 8, s));
+  }
+  if (no_deps) {
+    // no levels to build
+  } else if (side_count) {
     level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
   } else {
     mark_alloc(LB, NT, s);
@@ -2399,14 +2413,16 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   }
   GSP_HIP(hipEventRecord(ctx->fork, s));
   LevelJob job;
-  if (ctx->timing) {
+  if (ctx->timing && !no_deps) {
     job.a = take_event(ctx);
     job.b = take_event(ctx);
   }
   {
-    WorkerThread worker([&] {
-      levels_on_side(ctx, plan, D, outdeg.p, indeg.p, side_count, edges.p, LB, job);
-    });
+    std::unique_ptr<WorkerThread> worker;
+    if (!no_deps)
+      worker = std::make_unique<WorkerThread>([&] {
+        levels_on_side(ctx, plan, D, outdeg.p, indeg.p, side_count, edges.p, LB, job);
+      });
   {
     // learned plans: the surrogate takes the nodes it can, the rest are listed
     // for the exact solver
@@ -2487,7 +2503,15 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   int32_t n_fail = 0;
   GSP_HIP(hipMemcpyAsync(&n_fail, ctx->counters.p + CTR_FAIL, 4, hipMemcpyDeviceToHost, s));
   GSP_HIP(hipStreamSynchronize(s));
-  if (n_fail > 0) {
+  if (no_deps) {
+    // one level: every node, in id order
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
+                       plan->level.p, 0, NT);
+    hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
+                       plan->order.p, NT);
+    plan->depth = 1;
+    plan->level_off.assign({0, NT});
+  } else if (n_fail > 0) {
     // failed systems draw from the prior: their grid neighbours are no
     // dependencies, so the levels are built again from the flags
     TimedLaunch tl(ctx, GSP_K_LEVELS);
