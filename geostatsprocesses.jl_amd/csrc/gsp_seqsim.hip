@@ -67,6 +67,11 @@ struct PlanArgs {
   double* sigma;
   uint8_t* flags;
   int32_t* counters;
+  // optional: the dependency degrees, counted by the search as it writes each
+  // node's neighbours (as the degree count of the level build does)
+  int32_t* outdeg;
+  int32_t* indeg;
+  uint8_t* mark;
 };
 
 // Squared physical distance between a datum and the centre of node t.
@@ -642,6 +647,17 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
     if (A.nbr_slot && lane < A.K)
       A.nbr_slot[(int64_t)gi * A.K + lane] = lane < n ? s_ft[w][lane] : (uint8_t)255;
     if (lane == 0) A.nfound[gi] = n;
+    if (A.outdeg) {
+      // a node with fewer than kmin neighbours depends on nothing; data
+      // neighbours (ids >= NT) are no dependencies
+      const bool dep = n >= R.kmin && lane < n && nb < A.NT;
+      if (dep) atomicAdd(&A.outdeg[nb], 1);
+      const int c = __popcll(__ballot(dep));
+      if (lane == 0) {
+        A.indeg[gi] = c;
+        A.mark[gi] = c == 0 ? 1 : 0;
+      }
+    }
   }
 }
 
@@ -2047,19 +2063,15 @@ struct LevelJob {
   hipEvent_t a = nullptr, b = nullptr;  // timing (when enabled)
 };
 
-// The worker: the levels from the search on the side stream, after the fork
-// (with the degree count first when `count`; outdeg is zero).
+// The worker: the levels from the search's degree counts on the side stream,
+// after the fork.
 void levels_on_side(gsp_ctx* ctx, gsp_seqsim_plan* plan, const DepArgs& D, int32_t* outdeg,
-                    int32_t* indeg, bool count, unsigned long long* edges, LevelBufs& B,
-                    LevelJob& job) {
+                    int32_t* indeg, LevelBufs& B, LevelJob& job) {
   try {
     GSP_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->side;
     GSP_HIP(hipStreamWaitEvent(st, ctx->fork, 0));
     if (job.a) GSP_HIP(hipEventRecord(job.a, st));
-    if (count)
-      hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(D.N, 16, kEdgeBlocks)), dim3(256), 0, st, D,
-                         outdeg, indeg, B.mark.p, edges);
     build_levels<false>(plan, D, outdeg, indeg, B, st);
     if (job.b) GSP_HIP(hipEventRecord(job.b, st));
     GSP_HIP(hipStreamSynchronize(st));
@@ -2371,6 +2383,19 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   A.sigma = plan->sigma.p;
   A.flags = plan->flags.p;
   A.counters = ctx->counters.p;
+  // The reverse edges of the levels are sized by the bound sum(nfound) <=
+  // NT * K when that fits 32-bit offsets: then the search counts the
+  // dependency degrees itself and the main stream waits for nothing else
+  // (else the exact count runs after the search, here).
+  LevelBufs LB;
+  const bool side_count = (int64_t)NT * K < ((int64_t)1 << 31);
+  const bool search_counts = !no_deps && side_count;
+  if (search_counts) {
+    level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
+    A.outdeg = outdeg.p;
+    A.indeg = indeg.p;
+    A.mark = LB.mark.p;
+  }
 
   {
     TimedLaunch tl(ctx, GSP_K_SEARCH);
@@ -2394,19 +2419,12 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   D.flags = plan->flags.p;
   D.nbr = plan->nbr.p;
   DevBuf<unsigned long long> edges;
-  LevelBufs LB;
-  // The degree count runs on the side stream when the reverse edges fit the
-  // bound sum(nfound) <= NT * K in 32-bit offsets (the search's output is
-  // then all the main stream waits for); else here, for the exact count.
-  const bool side_count = (int64_t)NT * K < ((int64_t)1 << 31);
   if (!no_deps) {
     edges.alloc(1);
     GSP_HIP(hipMemsetAsync(edges.p, 0, 8, s));
   }
-  if (no_deps) {
-    // no levels to build
-  } else if (side_count) {
-    level_bufs_alloc(LB, NT, (int64_t)NT * K, outdeg.p, s);  // every buffer the worker uses
+  if (no_deps || search_counts) {
+    // no levels to build / degrees counted by the search
   } else {
     mark_alloc(LB, NT, s);
     hipLaunchKernelGGL(k_deg_count<false>, dim3(grid_for(NT, 16, 4096)), dim3(256), 0, s, D,
@@ -2428,7 +2446,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     std::unique_ptr<WorkerThread> worker;
     if (!no_deps)
       worker = std::make_unique<WorkerThread>([&] {
-        levels_on_side(ctx, plan, D, outdeg.p, indeg.p, side_count, edges.p, LB, job);
+        levels_on_side(ctx, plan, D, outdeg.p, indeg.p, LB, job);
       });
   {
     // learned plans: the surrogate takes the nodes it can, the rest are listed
