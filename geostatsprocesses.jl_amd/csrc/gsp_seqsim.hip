@@ -1987,8 +1987,9 @@ void level_bufs_alloc(LevelBufs& B, int64_t NT, int64_t n_edges, const int32_t* 
 // would only hold wave slots that the solves on the main stream need.
 constexpr int kEdgeBlocks = 512;
 
-// Kahn's algorithm on stream `st`: reverse CSR, level 0, then two launches
-// per level; the frontier sizes are checked every kBatch levels. Sets
+// Kahn's algorithm on stream `st`: reverse CSR, level 0, then three launches
+// per level (push, count, place); the frontier sizes are checked every kBatch
+// levels. Plans of at most kSmallLevels nodes run every level in one workgroup. Sets
 // plan->level, order, depth and level_off.
 template <bool FLAGS>
 void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg,
