@@ -2526,9 +2526,12 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   }  // joins the worker
   if (job.a) record_timing(ctx, job.a, job.b, GSP_K_LEVELS);
   GSP_REQUIRE(job.code == GSP_OK, job.code, "%s", job.msg.c_str());
-  int32_t n_fail = 0;
-  GSP_HIP(hipMemcpyAsync(&n_fail, ctx->counters.p + CTR_FAIL, 4, hipMemcpyDeviceToHost, s));
+  // one read of the counters serves the rebuild decision and the plan's
+  // statistics (the rebuild does not touch them)
+  int32_t ctr[16];
+  GSP_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
   GSP_HIP(hipStreamSynchronize(s));
+  const int32_t n_fail = ctr[CTR_FAIL];
   if (no_deps) {
     // one level: every node, in id order
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(NT, 256, 4096)), dim3(256), 0, s,
@@ -2550,10 +2553,6 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     build_levels<true>(plan, D, outdeg.p, indeg.p, LB, s);
   }
   tr.mark("levels");
-  int32_t ctr[16];
-  GSP_HIP(hipMemcpyAsync(ctr, ctx->counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
-  GSP_HIP(hipStreamSynchronize(s));
-  tr.mark("levels: offsets+free");
   flush_timings(ctx);
   plan->n_prior = ctr[CTR_PRIOR];
   plan->n_fallback = ctr[CTR_FALLBACK];
