@@ -33,7 +33,7 @@ hipEvent_t take_event(gsp_ctx* ctx) {
 }
 
 TimedLaunch::TimedLaunch(gsp_ctx* c, int s) : ctx(c), slot(s) {
-  if (!ctx->timing) return;
+  if (!ctx || !ctx->timing) return;
   a = take_event(ctx);
   b = take_event(ctx);
   GSP_HIP(hipEventRecord(a, ctx->stream));

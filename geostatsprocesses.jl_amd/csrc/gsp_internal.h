@@ -283,17 +283,27 @@ struct DevModel {
   double rot[9];  // row-major R
 };
 
+// Stabilising nugget of the gaussian model, added to its own nugget (in sill
+// units): GeoStatsFunctions' predecessor Variography.jl adds 1e-6 "for
+// numerical stability", and the reference's LUSIM testset (test/field.jl:29-56)
+// factors Gaussian covariance matrices of unit grids that are singular in
+// FP64 without it. The value is not fixed by any reference test.
+constexpr double kGaussianNugget = 1e-6;
+
 // C(u) = sill - gamma(u), gamma(0) = 0 (GeoStatsFunctions practical-range forms).
 __host__ __device__ inline double cov_u(const DevModel& m, double u) {
-  const double s = m.sill, n = m.nugget;
+  const double s = m.sill;
+  double n = m.nugget;
   if (u == 0.0) return s;
   double shape;
-  if (m.family == GSP_SPHERICAL)
+  if (m.family == GSP_SPHERICAL) {
     shape = (u < 1.0) ? (1.5 * u - 0.5 * (u * u * u)) : 1.0;
-  else if (m.family == GSP_EXPONENTIAL)
+  } else if (m.family == GSP_EXPONENTIAL) {
     shape = 1.0 - exp_cw(-3.0 * u);
-  else
+  } else {
+    n = n + kGaussianNugget;
     shape = 1.0 - exp_cw(-3.0 * (u * u));
+  }
   double gamma = (s - n) * shape + n;
   return s - gamma;
 }
@@ -369,6 +379,18 @@ void surrogate_export(gsp_ctx* ctx, const gsp_surrogate* s, float* params, int64
                       double* slots);
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out);
+// Device memory of one sweep beyond its inputs and outputs: the node-major
+// state X ((N + n_data) x RCmax doubles) and normals Zt (N x RCmax doubles);
+// the R realizations are swept in chunks of at most RCmax (1..64).
+struct SweepScratch {
+  double* X = nullptr;
+  double* Zt = nullptr;
+  int RCmax = 64;
+  bool timed = true;  // bracket the launches with timing events when enabled
+};
+// The sweep on device pointers with caller-provided scratch: allocates nothing.
+void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
+                       const double* data_vals, double* out, const SweepScratch& scratch);
 void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                                 int32_t n_cat, const double* prob, const double* u,
                                 const int32_t* data_cat, int32_t* out);

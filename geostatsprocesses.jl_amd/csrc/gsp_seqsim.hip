@@ -2209,7 +2209,11 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     h.data = coords.data() + coord_off[(size_t)q];
     for (int64_t t = 0; t < h.nD * h.nd; ++t)
       GSP_REQUIRE(std::isfinite(h.data[t]), GSP_ERR_ARGUMENT, "non-finite data coordinate");
-    h.alpha = scale_factor(p->grid, h.data, h.nD * h.nd, p->model);
+    // data-only plans stand for GeoStatsModels.fitpredict (posterior mean,
+    // FFTSIM conditioning), which krige in the domain's own coordinates
+    h.alpha = p->search_mode == GSP_DATA_ONLY ? 1.0
+                                              : scale_factor(p->grid, h.data, h.nD * h.nd,
+                                                             p->model);
     h.table = build_offsets(p->grid, h.r2, h.kmax, p->search_mode);
     build_hash(p->grid, p->search_mode, h);
   });
@@ -2578,23 +2582,38 @@ int sweep_chunk(int64_t rows, int32_t R) {
 
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out) {
-  hipStream_t s = ctx->stream;
   const int64_t N = plan->N, nD = plan->n_data;
   const int RCmax = sweep_chunk(N + nD, R);
   const int RC0 = R < RCmax ? R : RCmax;
   ctx->X.reserve((size_t)((N + nD) * RC0));
   ctx->Zt.reserve((size_t)(N * RC0));
+  SweepScratch w;
+  w.X = ctx->X.p;
+  w.Zt = ctx->Zt.p;
+  w.RCmax = RCmax;
+  w.timed = true;
+  seqsim_sweep_into(ctx, plan, R, z, data_vals, out, w);
+}
+
+void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
+                       const double* data_vals, double* out, const SweepScratch& W) {
+  hipStream_t s = ctx->stream;
+  gsp_ctx* tctx = W.timed ? ctx : nullptr;  // no event bracketing when untimed
+  const int64_t N = plan->N, nD = plan->n_data;
+  const int RCmax = W.RCmax;
+  double* const Xs = W.X;
+  double* const Zts = W.Zt;
   for (int r0 = 0; r0 < R; r0 += RCmax) {
     const int RC = (R - r0) < RCmax ? (R - r0) : RCmax;
     int shift = 0;
     while ((1 << shift) < RC) ++shift;
     if (nD > 0)
       hipLaunchKernelGGL(k_data_rows, dim3(grid_for(nD * RC, 256, 1024)), dim3(256), 0, s,
-                         ctx->X.p, (int32_t)N, (int32_t)nD, RC, data_vals);
+                         Xs, (int32_t)N, (int32_t)nD, RC, data_vals);
     {
-      TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
+      TimedLaunch tl(tctx, GSP_K_TRANSPOSE);
       hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, z,
-                         (int32_t)N, RC, r0, ctx->Zt.p);
+                         (int32_t)N, RC, r0, Zts);
     }
     LevelArgs A;
     A.K = plan->kmax;
@@ -2608,10 +2627,10 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
     A.sigma = plan->sigma.p;
     A.node_req = plan->node_req.p;
     A.mean = plan->req_mean.p;
-    A.Zt = ctx->Zt.p;
-    A.X = ctx->X.p;
+    A.Zt = Zts;
+    A.X = Xs;
     {
-      TimedLaunch tl(ctx, GSP_K_SWEEP);
+      TimedLaunch tl(tctx, GSP_K_SWEEP);
       for (int32_t l = 0; l < plan->depth; ++l) {
         const int64_t off = plan->level_off[(size_t)l];
         const int64_t cnt = plan->level_off[(size_t)l + 1] - off;
@@ -2624,8 +2643,8 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
       }
     }
     {
-      TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, ctx->X.p,
+      TimedLaunch tl(tctx, GSP_K_TRANSPOSE);
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, Xs,
                          (int32_t)N, RC, r0, out);
     }
     GSP_HIP(hipGetLastError());

@@ -735,6 +735,7 @@ struct gsp_surrogate {
   // the variogram and neighbourhood it serves
   int32_t family = 0;
   double nugget_ratio = 0.0;
+  double sill = 1.0;  // of the training variogram (gaussian models must match it)
   double canon[gsp::kSlots][3] = {};  // slot offsets in ranges
   int32_t ndim = 0;
   gsp::DevBuf<float> params, adam_m, adam_v, grad;
@@ -751,6 +752,9 @@ bool surrogate_accepts(const gsp_surrogate* s, const ReqInfo& r) {
   if (m.family != s->family || m.has_rotation || r.mode != GSP_KNN_THEN_MASK) return false;
   if (r.kmax > kSlots || r.T < kSlots || r.ndim != s->ndim || !(m.sill > 0)) return false;
   if (std::fabs(m.nugget / m.sill - s->nugget_ratio) > 1e-12) return false;
+  // the gaussian model's stabilising nugget is absolute, so its weights
+  // depend on the sill itself, not only on the nugget/sill ratio
+  if (m.family == GSP_GAUSSIAN && std::fabs(m.sill - s->sill) > 1e-12 * s->sill) return false;
   for (int t = 0; t < kSlots; ++t)
     for (int d = 0; d < r.ndim; ++d) {
       const double c = (double)r.table24[t][d] * r.spacing[d] / m.ranges[d];
@@ -815,6 +819,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
               "and an unrotated variogram");
   S->family = m0.family;
   S->nugget_ratio = m0.nugget / m0.sill;
+  S->sill = m0.sill;
   S->ndim = r0.ndim;
   for (int t = 0; t < kSlots; ++t)
     for (int d = 0; d < 3; ++d)
@@ -823,15 +828,15 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   DevModel M;
   std::memset(&M, 0, sizeof(M));
   M.family = S->family;
-  M.sill = 1.0;
-  M.nugget = S->nugget_ratio;
+  M.sill = m0.sill;
+  M.nugget = m0.nugget;
   for (int t = 0; t < kSlots; ++t) {
     double u2 = 0.0;
     for (int d = 0; d < 3; ++d) {
       canonf[(size_t)(3 * t + d)] = (float)(S->canon[t][d] * kFeatScale);
       u2 += S->canon[t][d] * S->canon[t][d];
     }
-    cslot[(size_t)t] = (float)cov_u(M, std::sqrt(u2));
+    cslot[(size_t)t] = (float)(cov_u(M, std::sqrt(u2)) / m0.sill);
   }
   S->canonf.alloc(canonf.size());
   S->cslot.alloc(cslot.size());

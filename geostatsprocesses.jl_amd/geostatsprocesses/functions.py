@@ -1,7 +1,10 @@
 """Geostatistical functions (GeoStatsFunctions 0.15 restated, un-vendored):
 spherical, exponential and gaussian variograms and covariances with nugget and
 axis ranges (optionally rotated). C(h) = sill - gamma(h), gamma(0) = 0 and
-gamma(u) = (sill - nugget) * shape(u) + nugget for u > 0 (practical ranges)."""
+gamma(u) = (sill - nugget) * shape(u) + nugget for u > 0 (practical ranges).
+Gaussian models add a stabilising nugget of 1e-6 (sill units) to their own, as
+the backend's covariance does: the reference's LUSIM testset factors Gaussian
+covariance matrices that are singular in FP64 without it (test/field.jl:29-56)."""
 from __future__ import annotations
 
 import numpy as np
@@ -12,6 +15,7 @@ from . import _lib
 class GeoStatsFunction:
     family = None
     kind = _lib.VARIOGRAM
+    stabilising_nugget = 0.0  # added to the nugget inside gamma (gaussian: 1e-6)
     __array_ufunc__ = None  # `A * f` with a numpy matrix A goes to __rmul__
 
     def __init__(self, sill=1.0, range=1.0, nugget=0.0, ranges=None, rotation=None):
@@ -59,7 +63,8 @@ class GeoStatsFunction:
     def gamma(self, h):
         """Variogram value at normalised lag(s) u = h / range."""
         u = np.asarray(h, dtype=np.float64) / self._ranges[0]
-        g = (self._sill - self._nugget) * self.shape(u) + self._nugget
+        n = self._nugget + self.stabilising_nugget
+        g = (self._sill - n) * self.shape(u) + n
         return np.where(u > 0, g, 0.0)
 
     def cov(self, h):
@@ -143,6 +148,7 @@ class _Exponential:
 
 class _Gaussian:
     family = _lib.GAUSSIAN
+    stabilising_nugget = 1e-6
 
     def shape(self, u):
         u = np.asarray(u, dtype=np.float64)

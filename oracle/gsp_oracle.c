@@ -68,10 +68,17 @@ double or_exp(double x) {
  * seqsim.jl:286 and _pairwise at utils.jl:50-62). Practical-range forms:
  *   spherical   gamma = (s-n)(1.5u - 0.5u^3) + n  for 0 < u < 1, (s-n) + n for u >= 1
  *   exponential gamma = (s-n)(1 - exp(-3u)) + n   for u > 0
- *   gaussian    gamma = (s-n)(1 - exp(-3u^2)) + n for u > 0
+ *   gaussian    gamma = (s-n')(1 - exp(-3u^2)) + n' for u > 0, n' = n + 1e-6
  * and gamma(0) = 0. Both variograms (kriging uses s - gamma) and covariances
  * (banded: the function value itself) reduce to C(u) = s - gamma(u).
- * The x3 practical-range convention is unpinned by the reference's tests. */
+ * The gaussian model carries a stabilising nugget of 1e-6 (sill units) on top
+ * of its own, as GeoStatsFunctions' predecessor Variography.jl did "for
+ * numerical stability": without it the Gaussian kernel matrix of a unit grid
+ * with range 10 is singular in FP64, and the reference's LUSIM testset
+ * (test/field.jl:29-56: GaussianCovariance(range=10) on CartesianGrid(100)
+ * and (100,100), range 35 on a 10x10 grid) expects those calls to return a
+ * field. The value 1e-6 and the x3 practical-range convention are unpinned
+ * by the reference's tests. */
 double or_cov_u(const or_model* m, double u) {
   double s = m->sill, n = m->nugget;
   if (u == 0.0) return s;
@@ -79,7 +86,10 @@ double or_cov_u(const or_model* m, double u) {
   switch (m->family) {
     case OR_SPHERICAL: shape = (u < 1.0) ? (1.5 * u - 0.5 * (u * u * u)) : 1.0; break;
     case OR_EXPONENTIAL: shape = 1.0 - or_exp(-3.0 * u); break;
-    default: shape = 1.0 - or_exp(-3.0 * (u * u)); break;
+    default:
+      n = n + OR_GAUSSIAN_NUGGET;
+      shape = 1.0 - or_exp(-3.0 * (u * u));
+      break;
   }
   double gamma = (s - n) * shape + n;
   return s - gamma;
@@ -118,8 +128,14 @@ double or_cov_points(const or_model* m, int ndim, double alpha, const double* xs
 /* ------------------------------------------------------------- scaling ----
  * seqsim.jl:162-189: alpha = 1 / max(alpha1, alpha2, alpha3) with
  * alpha1 = max |coordinate| of the grid bounding box corners,
- * alpha2 = same for the data bounding box, alpha3 = range(f) (largest range). */
+ * alpha2 = same for the data bounding box, alpha3 = range(f) (largest range).
+ * The data-only search stands for GeoStatsModels.fitpredict (the posterior
+ * mean, expectation/field/gaussian.jl:21-25, and FFTSIM's conditioning,
+ * fftsim.jl:99,148), which works in the domain's own coordinates: alpha = 1,
+ * so a datum on a cell centroid lies at lag exactly 0 (this matters where the
+ * model has a nugget, whose jump starts at any lag above 0). */
 double or_scale_factor(const or_seqsim_problem* p) {
+  if (p->search_mode == OR_DATA_ONLY) return 1.0;
   const or_grid* g = &p->grid;
   double a1 = 0.0;
   for (int d = 0; d < g->ndim; ++d) {

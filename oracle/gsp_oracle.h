@@ -27,6 +27,8 @@ extern "C" {
 #endif
 
 enum { OR_SPHERICAL = 0, OR_EXPONENTIAL = 1, OR_GAUSSIAN = 2 };
+/* stabilising nugget of the gaussian model (unpinned value, see or_cov_u) */
+#define OR_GAUSSIAN_NUGGET 1e-6
 enum { OR_KNN_THEN_MASK = 0, OR_MASKED_KNN = 1, OR_DATA_ONLY = 2 };
 enum { OR_FP64 = 0, OR_FP32_REFINE = 1 };
 enum { OR_FLAG_FEW = 1, OR_FLAG_FAIL = 2, OR_FLAG_FP64_FALLBACK = 4 };

@@ -27,6 +27,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 SPHERICAL, EXPONENTIAL, GAUSSIAN = 0, 1, 2
+# Gaussian models carry this nugget on top of their own (unpinned value; the
+# reference's LUSIM testset needs a positive one, test/field.jl:29-56)
+GAUSSIAN_NUGGET = 1e-6
 KNN_THEN_MASK, MASKED_KNN, DATA_ONLY = 0, 1, 2
 FP64, FP32_REFINE = 0, 1
 FLAG_FEW, FLAG_FAIL, FLAG_FP64_FALLBACK, FLAG_LEARNED = 1, 2, 4, 8
@@ -527,6 +530,8 @@ def covariance_np(family, sill, nugget, ranges, xa, xb, rotation=None):
     elif family == EXPONENTIAL:
         shape = 1.0 - np.exp(-3.0 * u)
     else:
+        # the gaussian model's stabilising nugget (gsp_oracle.c, or_cov_u)
+        nugget = nugget + GAUSSIAN_NUGGET
         shape = 1.0 - np.exp(-3.0 * u * u)
     gamma = (sill - nugget) * shape + nugget
     C = sill - gamma

@@ -86,6 +86,26 @@ def test_lusim_families(ctx, fam):
     assert r["L"] <= 1e-12 and r["out"] <= 1e-10
 
 
+def test_lusim_gaussian_without_nugget(ctx):
+    """GaussianCovariance with nugget 0 (the reference's LUSIM testset, test/field.jl:
+    39-56): the matrix is regular only through the stabilising nugget (1e-6), so
+    kappa is about 1e7. Unconditional: the factor is checked by its backward error
+    |L L' - C| / |C| <= 10 n eps (the bound of a stable Cholesky; the device and
+    numpy covariances differ by a few ulp). Conditional: data exact and the
+    realizations within kappa-scaled rounding (1e-6 relative) of scipy's."""
+    r = run_case(ctx, (20, 20), O.GAUSSIAN, 1.0, 0.0, (10.0, 10.0), 0, 3, 21)
+    gL = r["gL"]
+    cents = O.grid_centroids((20, 20))
+    C = O.covariance_np(O.GAUSSIAN, 1.0, 0.0, (10.0, 10.0), cents, cents)
+    back = np.linalg.norm(gL @ gL.T - C) / np.linalg.norm(C)
+    say(f"gaussian nugget 0: backward error {back:.3e}, L rel {r['L']:.3e}")
+    assert back <= 10 * 400 * np.finfo(np.float64).eps
+    assert r["out"] <= 1e-6
+    r = run_case(ctx, (20, 20), O.GAUSSIAN, 1.0, 0.0, (10.0, 10.0), 12, 3, 22)
+    say(f"gaussian nugget 0 with data: out rel {r['out']:.3e}")
+    assert r["out"] <= 1e-6
+
+
 def test_lusim_unconditional_adds_mean(ctx):
     r = run_case(ctx, (20, 20), O.SPHERICAL, 1.0, 0.0, (6.0, 6.0), 0, 3, 4, mean=0.7)
     assert r["out"] <= 1e-10
@@ -151,8 +171,9 @@ def test_lusim_rejects_variogram(ctx):
 
 
 def test_lusim_not_positive_definite(ctx):
-    # a gaussian covariance with a very long range on a fine grid is singular
-    p, keep = lusim_problem((40, 40), O.GAUSSIAN, 1.0, 0.0, (200.0, 200.0), [], [])
+    # a gaussian covariance whose sill lies below the model's stabilising nugget
+    # (1e-6): its off-diagonal covariances are negative and exceed the diagonal
+    p, keep = lusim_problem((40, 40), O.GAUSSIAN, 1e-9, 0.0, (200.0, 200.0), [], [])
     with pytest.raises(_lib.PosDefException):
         _lib.LusimPlan(ctx, p, keep=keep)
 

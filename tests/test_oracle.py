@@ -45,12 +45,65 @@ def test_covariance_closed_forms():
     assert O.cov_u(O.SPHERICAL, 1.0, 0.0, 7.0) == 0.0
     # practical range: 95% of the structured sill is reached at u = 1
     assert np.isclose(O.cov_u(O.EXPONENTIAL, 1.0, 0.0, 1.0), np.exp(-3.0), rtol=1e-14)
-    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 1.0), np.exp(-3.0), rtol=1e-14)
     assert np.isclose(O.cov_u(O.EXPONENTIAL, 1.0, 0.01, 0.5), 0.99 * np.exp(-1.5), rtol=1e-14)
-    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 0.5), np.exp(-0.75), rtol=1e-14)
+    # gaussian: the structured part is sill - nugget - 1e-6 (stabilising nugget)
+    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 1.0), (1.0 - 1e-6) * np.exp(-3.0), rtol=1e-14)
+    assert np.isclose(O.cov_u(O.GAUSSIAN, 1.0, 0.0, 0.5), (1.0 - 1e-6) * np.exp(-0.75), rtol=1e-14)
+    assert np.isclose(O.cov_u(O.GAUSSIAN, 2.0, 0.5, 0.5), (1.5 - 1e-6) * np.exp(-0.75), rtol=1e-14)
+    assert O.cov_u(O.GAUSSIAN, 2.0, 0.5, 0.0) == 2.0
+
+
+def _centroids_1d(n):
+    return (np.arange(n, dtype=np.float64) + 0.5)[:, None]
+
+
+def test_gaussian_reference_lusim_cases_are_positive_definite():
+    """The reference's LUSIM testset factors GaussianCovariance matrices of unit
+    grids (test/field.jl:29-37: range 10 on CartesianGrid(100); :46-56: range
+    35 on a 10 x 10 grid, with and without data) and expects fields back. With
+    the plain Gaussian kernel those matrices are singular in FP64 (LAPACK's
+    potrf fails); with the stabilising nugget they factor, the smallest
+    eigenvalue then being at least that nugget."""
+    import scipy.linalg as sla
+    cases = [(_centroids_1d(100), (10.0,)),
+             (O.grid_centroids((10, 10)), (35.0, 35.0))]
+    for cents, ranges in cases:
+        C = O.covariance_np(O.GAUSSIAN, 1.0, 0.0, ranges, cents, cents)
+        L = sla.cholesky(C, lower=True)
+        assert np.all(np.isfinite(L))
+        assert np.linalg.eigvalsh(C).min() > 0.5e-6
+        # the same kernel without the stabilising nugget is not positive definite
+        u = np.sqrt(np.sum(((cents[:, None, :] - cents[None, :, :]) / np.asarray(ranges)) ** 2, -1))
+        with pytest.raises(np.linalg.LinAlgError):
+            sla.cholesky(np.exp(-3.0 * u * u), lower=True)
+    # the conditional 10 x 10 case (data at (25,25), (50,75), (75,50) snap to cells)
+    cents = O.grid_centroids((10, 10))
+    dinds = np.unique(O.nearest_init(cents, np.array([[25.0, 25.0], [50.0, 75.0], [75.0, 50.0]])))
+    d2, L22, _ = O.lusim_preprocess(cents, dinds, np.zeros(len(dinds)), O.GAUSSIAN, 1.0, 0.0,
+                                    (35.0, 35.0))
+    assert np.all(np.isfinite(L22))
 
 
 # --------------------------------------------------------------- scaling ----
+def test_data_only_kriging_in_domain_coordinates():
+    """GeoStatsModels.fitpredict (posterior mean, FFTSIM conditioning) kriges in
+    the domain's own coordinates: alpha = 1 for data-only plans, so data on cell
+    centroids sit at lag exactly 0 and are reproduced exactly even where the
+    model has a nugget (a gaussian model always has its stabilising one). With
+    SEQSIM's scaling by 1/100.5 the lag would round to a tiny positive value and
+    the nugget jump would move the estimate by about 1e-6."""
+    coords = np.array([[25.0, 25.0], [50.0, 75.0], [75.0, 50.0]])
+    vals = np.array([1.0, 0.0, 1.0])
+    case = O.SeqsimCase(dims=(100, 100), origin=(0.5, 0.5), family=O.GAUSSIAN, ranges=(35.0,),
+                        data_coords=coords, data_vals=vals, kmax=3, mode=O.DATA_ONLY)
+    assert O.scale_factor(case) == 1.0
+    plan = O.seqsim_plan(case)
+    mean = O.seqsim_sweep(case, plan, np.zeros((1, case.n_nodes)))[0]
+    for (x, y), v in zip(coords, vals):
+        i = int(x - 1.0) + 100 * int(y - 1.0)
+        assert mean[i] == v
+
+
 def test_scale_factor_closed_form():
     # seqsim.jl:162-189: alpha = 1 / max(|grid bbox|, |data bbox|, range)
     c = O.SeqsimCase(dims=(100, 100), ranges=(10.0,))
@@ -220,7 +273,8 @@ def test_krige_colocated_datum_honours_data():
 
 def test_krige_singular_system_fails_to_prior():
     # two data on the same spot, nugget 0: LHS is singular -> status false
-    case = O.SeqsimCase(dims=(10, 10), family=O.GAUSSIAN, ranges=(50.0,),
+    # (a gaussian model would not do: its stabilising nugget keeps it regular)
+    case = O.SeqsimCase(dims=(10, 10), family=O.SPHERICAL, ranges=(50.0,),
                         data_coords=np.array([[2.2, 2.2], [2.2, 2.2]]))
     lam, sig, f = O.krige(case, 0, np.array([100, 101]))
     assert f & O.FLAG_FAIL
@@ -417,11 +471,16 @@ def test_fftsim_oracle_reproduces_covariance():
     # the periodic window holds every lag once: -(n/2 - 1) .. n/2 per axis
     hx = np.arange(-49, 51)[None, :]
     hy = np.arange(-39, 41)[:, None]
-    cbar = np.mean(np.exp(-3.0 * (hx ** 2 + hy ** 2) / 100.0))
+    # the model: (1 - eps) exp(-3 h^2 / r^2) for h > 0 and 1 at h = 0, eps the
+    # gaussian model's stabilising nugget (1e-6)
+    eps = 1e-6
+    Cw = (1.0 - eps) * np.exp(-3.0 * (hx ** 2 + hy ** 2) / 100.0)
+    Cw[(hy == 0) & (hx == 0)] = 1.0
+    cbar = np.mean(Cw)
     for r in range(2):
         A = circular_autocorrelation(out[r] - 0.3, F.shape)
         for (dy, dx) in [(0, 1), (1, 0), (2, 3), (5, 0), (0, 12), (7, 7)]:
-            c = np.exp(-3.0 * (dx * dx + dy * dy) / 100.0)
+            c = (1.0 - eps) * np.exp(-3.0 * (dx * dx + dy * dy) / 100.0)
             expect = (c - cbar) / (1.0 - cbar)
             assert A[dy, dx] / A[0, 0] == pytest.approx(expect, abs=1e-9)
     # a dropped square root would give the covariance convolved with itself

@@ -30,12 +30,30 @@ def test_lusim_reference_cases(ctx):
     data = gsp.georef({"Z": [1.0, 0.0, 1.0]}, [(25.0, 25.0), (50.0, 75.0), (75.0, 50.0)])
     real = gsp.rand(proc, grid, rng=rng, method=method, data=data)
     assert real.Z.dtype == np.float64
-    # test/field.jl:41-56 use GaussianCovariance (nugget 0) on unit grids with ranges
-    # of 10 and 35: that matrix is singular to working precision and LAPACK's
-    # potrf (scipy) rejects it; the backend raises PosDefException the same way.
+    # test/field.jl:46-56: GaussianCovariance(range=35) on the 10 x 10 grid, with
+    # and without data (the gaussian model's stabilising nugget keeps it regular)
     proc = gsp.GaussianProcess(gsp.GaussianCovariance(range=35.0), 0.0)
-    with pytest.raises(gsp.PosDefException):
-        gsp.rand(proc, grid, rng=rng, method=method, data=data)
+    real = gsp.rand(proc, grid, rng=rng, method=method)
+    assert real.field.dtype == np.float64 and np.isfinite(real.field).all()
+    real = gsp.rand(proc, grid, rng=rng, method=method, data=data)
+    assert real.Z.dtype == np.float64 and np.isfinite(real.Z).all()
+    dest = np.asarray(gsp.NearestInit().destinations(grid, data))
+    for d in np.unique(dest):
+        assert real.Z[d] == data.Z[np.flatnonzero(dest == d)[-1]]
+
+
+def test_lusim_reference_gaussian_2d(ctx):
+    """test/field.jl:39-44: GaussianCovariance(range=10) on CartesianGrid(100, 100),
+    a 10,000-cell covariance matrix factored whole; the result is a Float64
+    field whose variance is about the sill."""
+    rng = np.random.default_rng(123)
+    proc = gsp.GaussianProcess(gsp.GaussianCovariance(range=10.0))
+    grid = gsp.CartesianGrid(100, 100)
+    real = gsp.rand(proc, grid, rng=rng, method=gsp.LUSIM())
+    f = real.field
+    assert f.dtype == np.float64 and f.shape == (10000,) and np.isfinite(f).all()
+    say(f"gaussian 100x100 LUSIM: mean {f.mean():.3f} var {f.var():.3f}")
+    assert 0.3 < f.var() < 3.0
 
 
 def test_lusim_bivariate_reference_cases(ctx):
@@ -44,12 +62,12 @@ def test_lusim_bivariate_reference_cases(ctx):
     method = gsp.LUSIM()
     rng = np.random.default_rng(123)
     grid = gsp.CartesianGrid(100)
-    # field.jl:31-37 pair a GaussianCovariance (nugget 0, range 10) with a unit
-    # grid: singular to working precision, as in the univariate cases
+    # field.jl:31-37: cosimulation of [1 .95; .95 1] * GaussianCovariance(range=10)
     proc = gsp.GaussianProcess(np.array([[1.0, 0.95], [0.95, 1.0]])
                                * gsp.GaussianCovariance(range=10.0), [0.0, 0.0])
-    with pytest.raises(gsp.PosDefException):
-        gsp.rand(proc, grid, rng=rng, method=method)
+    real = gsp.rand(proc, grid, rng=rng, method=method)
+    assert real.field1.dtype == np.float64 and real.field2.dtype == np.float64
+    assert np.isfinite(real.field1).all() and np.isfinite(real.field2).all()
     proc = gsp.GaussianProcess(np.array([[1.0, 0.95], [0.95, 1.0]])
                                * gsp.SphericalCovariance(range=10.0), [0.0, 0.0])
     real = gsp.rand(proc, grid, rng=rng, method=method)
