@@ -126,6 +126,15 @@ int32_t gsp_ctx_create(int32_t device, gsp_ctx** out) {
     delete ctx;
     return GSP_ERR_DEVICE;
   }
+  try {
+    gsp::noalloc_init(device);  // this device's address of the allocation-free sweep's scratch
+  } catch (const gsp::Fail& f) {
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipStreamDestroy(ctx->side);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return f.code;
+  }
   *out = ctx;
   return GSP_OK;
 }

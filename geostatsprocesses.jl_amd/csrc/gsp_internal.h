@@ -388,6 +388,9 @@ struct SweepScratch {
   int RCmax = 64;
   bool timed = true;  // bracket the launches with timing events when enabled
 };
+// Device address of the allocation-free sweep's scratch on `device` (current),
+// looked up once per device (gsp_noalloc.hip).
+void noalloc_init(int device);
 // The sweep on device pointers with caller-provided scratch: allocates nothing.
 void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                        const double* data_vals, double* out, const SweepScratch& scratch);

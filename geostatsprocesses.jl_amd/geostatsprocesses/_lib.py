@@ -130,6 +130,7 @@ SIGNATURES = {
     "gsp_seqsim_plan_export": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "gsp_seqsim_sweep": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP, _I32]),
     "gsp_seqsim_plan_free": (_I32, [_VP]),
+    "gsp_seqsim_sweep_noalloc": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP]),
     "gsp_seqsim_sweep_indicator": (_I32, [_VP, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _I32]),
     "gsp_seqsim_plan_create_batch": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
                                             ctypes.POINTER(_VP)]),

@@ -270,6 +270,18 @@ int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                          const double* z, const double* data_vals, double* out,
                          int32_t device_pointers);
 int32_t gsp_seqsim_plan_free(gsp_seqsim_plan* plan);
+/* gsp_seqsim_sweep_noalloc: no counterpart in the reference. It is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): gsp_seqsim_sweep with
+ * device pointers (z, data_vals and out on the device, same layouts), whose
+ * scratch lives in a fixed 64 MiB device array per GPU instead of the
+ * context's pool. It calls no allocator. Plans with 2 N + n_data above
+ * 8,388,576 are rejected with GSP_ERR_ARGUMENT; larger R runs in chunks of
+ * realizations. Returns once the work is queued on the context stream. All
+ * calls on one GPU share the array: calls that overlap on one GPU (two
+ * contexts' streams) race. */
+int32_t gsp_seqsim_sweep_noalloc(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                 const double* z, const double* data_vals, double* out);
 /* gsp_seqsim_sweep_indicator replaces R calls of randsingle(..., ::SEQSIM, ...)
  * for an IndicatorProcess (seqsim.jl:247-251,292-310,342-347) whose function is
  * an intrinsic coregionalization A * c(h): with every category observed at
