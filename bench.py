@@ -59,6 +59,17 @@ HBM): mean, variance, cdf(0), ccdf(1) and the 5%, 50% and 95% quantiles in
 one pass. The same statistics without the quantiles (a pure streaming pass)
 are timed beside it.
 
+Workload "lusim-64x64": LUSIM on a 64 x 64 unit grid (4,096 cells),
+exponential covariance sill 1, range 20, nugget 0.01, 200 uniform conditioning
+points snapped to cells by NearestInit (later data win), 256 realizations.
+One step is one factorization (covariance assembly, L11, B12, d2 and the
+blocked Cholesky of C22 - B12' B12 on the matrix cores, FP64) plus the
+256-realization sample d2 + L22 W and scatter, with the data and the normals
+resident in HBM. The mixed form (FP32 MFMA trailing updates, FP64 diagonal
+tiles, one FP64 refinement step of the conditioning solve) is timed beside it;
+both are compared per realization with the FP64 numpy/scipy restatement,
+which is also the CPU baseline (LAPACK at 1 thread and at every core).
+
 Prints one JSON line. `cpu_baseline` times the CPU restatement of the
 reference's cost structure (search + solve per node per realization) on a
 bounded sample of the same workload.
@@ -99,7 +110,10 @@ WORKLOADS = {
                                     method="indicator", prob=(0.5, 0.3, 0.2)),
     "fftsim-256x256x128": dict(dims=(256, 256, 128), family=2, sill=1.0, ranges=(20.0, 20.0, 20.0),
                                nugget=0.0, R=16, seed=1007, method="fftsim", cpu_reals=2),
+    "lusim-64x64": dict(dims=(64, 64), family=1, sill=1.0, ranges=(20.0, 20.0), nugget=0.01,
+                        n_data=200, R=256, seed=1002, method="lusim"),
 }
+FP64_PEAK_TFLOPS = 78.6
 
 
 def batch_specs(spec):
@@ -601,6 +615,162 @@ def bench_indicator(args, spec):
     print(json.dumps(line), flush=True)
 
 
+def cpu_info():
+    """The host CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def bench_lusim(args, spec):
+    """LUSIM factorization + sampling (FP64 and mixed) on one GPU; the FP64
+    numpy/scipy restatement is the CPU baseline and the per-realization oracle."""
+    import geostatsprocesses as gsp
+    from geostatsprocesses import _lib
+    dims, R = spec["dims"], spec["R"]
+    N = int(np.prod(dims))
+    pts = np.random.default_rng(spec["seed"]).uniform(0, 1, (spec["n_data"], 2)) * np.asarray(dims)
+    vals = np.random.default_rng(spec["seed"] + 2).standard_normal(spec["n_data"])
+    grid = gsp.CartesianGrid(*dims)
+    proc = gsp.GaussianProcess(gsp.ExponentialCovariance(sill=spec["sill"], range=spec["ranges"][0],
+                                                         nugget=spec["nugget"]))
+    real, mask = gsp.initialize(proc, grid, gsp.georef({"Z": vals}, [tuple(p) for p in pts]),
+                                gsp.NearestInit())
+    dinds = np.flatnonzero(mask["Z"]).astype(np.int64)
+    z1 = np.ascontiguousarray(real["Z"][dinds])
+    nD = len(dinds)
+    nS = N - nD
+    ctx = _lib.default_context(0)
+    ddinds = _lib.DeviceBuffer.from_array(ctx, dinds)
+    dz1 = _lib.DeviceBuffer.from_array(ctx, z1)
+    w = np.random.default_rng(spec["seed"] + 3).standard_normal((R, nS))
+    dw = _lib.DeviceBuffer.from_array(ctx, w)
+    dout = _lib.DeviceBuffer(ctx, (R, N))
+
+    def problem(prec):
+        p = _lib.GspLusimProblem()
+        p.grid.ndim = 2
+        for d in range(3):
+            p.grid.dims[d] = dims[d] if d < 2 else 1
+            p.grid.spacing[d] = 1.0
+            p.model.ranges[d] = spec["ranges"][d] if d < 2 else 1.0
+        p.model.family = spec["family"]
+        p.model.kind = _lib.COVARIANCE
+        p.model.sill = spec["sill"]
+        p.model.nugget = spec["nugget"]
+        p.n_data = nD
+        p.dinds = ddinds.ptr
+        p.z1 = dz1.ptr
+        p.precision = prec
+        p.device_pointers = 1
+        return p
+
+    def stepper(prec):
+        prob = problem(prec)
+
+        def step():
+            plan = _lib.LusimPlan(ctx, prob)
+            plan.sample(dw, dout, device_pointers=True)
+            ctx.synchronize()
+            return plan
+        return step
+
+    def run(prec):
+        step = stepper(prec)
+        for _ in range(args.warmup):
+            step().close()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step().close()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        # per-kernel times from one more pass with the timing events on
+        ctx.enable_timing(True)
+        ctx.timings(reset=True)
+        step().close()
+        tim = ctx.timings(reset=True)
+        ctx.enable_timing(False)
+        return ms, tim, dout.download()
+
+    ms64, tim64, out64 = run(_lib.LU_FP64)
+    msmx, timmx, outmx = run(_lib.LU_MIXED)
+    # algorithmic flops (factorization and sampling of lusim.jl:66-107,145-175)
+    flops = nS ** 3 / 3 + 2.0 * nS * nS * nD + nD * nD * nS + R * nS * nS
+    trail_flops = sum(64.0 * r * (r + 1) for r in range(nS - 64, 0, -64))
+    t_ms, t_n = tim64[_lib.K_LU_TRAIL]
+    trail_tf = trail_flops / (t_ms * 1e-3) / 1e12 if t_ms > 0 else None
+
+    def kern(tim):
+        out = {}
+        for slot, name in ((_lib.K_LU_FACTOR, "factor (whole call)"), (_lib.K_LU_TRAIL, "k_trail"),
+                           (_lib.K_LU_SAMPLE, "sample (whole call)")):
+            ms, n = tim[slot]
+            if n:
+                out[name] = {"ms": round(ms, 4), "launches": n, "avg_ms": round(ms / n, 4)}
+        return out
+
+    cpu = accuracy = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        from threadpoolctl import threadpool_limits
+        model, ncores = cpu_info()
+        cents = O.grid_centroids(dims)
+        res = {}
+        for threads in (1, ncores):
+            with threadpool_limits(limits=threads):
+                t0 = time.perf_counter()
+                d2, L22, sinds = O.lusim_preprocess(cents, dinds, z1, spec["family"], spec["sill"],
+                                                    spec["nugget"], spec["ranges"])
+                ref = O.lusim_sample(d2, L22, dinds, sinds, z1, 0.0, w)
+                res[threads] = time.perf_counter() - t0
+        cpu = {"value": N * R / res[ncores], "unit": "nodes*realizations/s", "cores": ncores,
+               "kind": "port",
+               "sample": (f"the whole workload: numpy/scipy (LAPACK potrf/trsm, BLAS gemm) "
+                          f"restatement of lusim.jl, FP64, {res[ncores]:.2f} s at {ncores} threads"),
+               "one_thread": {"value": N * R / res[1], "cores": 1,
+                              "sample": f"the same at 1 thread, {res[1]:.2f} s"},
+               "cpu_model": model}
+
+        def per_real(out):
+            e = np.max(np.abs(out - ref), axis=1) / np.max(np.abs(ref), axis=1)
+            return {"max": float(e.max()), "median": float(np.median(e))}
+        accuracy = {"fp64_vs_oracle_per_realization": per_real(out64),
+                    "mixed_vs_oracle_per_realization": per_real(outmx),
+                    "data_exact": bool(np.array_equal(out64[:, dinds], np.broadcast_to(z1, (R, nD))))}
+    value = N * R / (ms64 / 1e3)
+    line = {
+        "metric": "LUSIM nodes*realizations/s (FP64 Cholesky)",
+        "value": round(value, 1), "unit": "nodes*realizations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms64, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": list(dims), "realizations": R,
+                   "n_data": nD, "n_sim": nS, "covariance": "exponential", "range": spec["ranges"][0],
+                   "nugget": spec["nugget"]},
+        "roofline": {"bound": "mfma", "kernel": "k_trail", "achieved": round(trail_tf, 2) if trail_tf else None,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(trail_tf / FP64_PEAK_TFLOPS, 4) if trail_tf else None,
+                     "traffic": None},
+        "step_tflops": round(flops / (ms64 * 1e-3) / 1e12, 2),
+        "step_frac_of_fp64_peak": round(flops / (ms64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "algorithmic_gflop_per_step": round(flops / 1e9, 2),
+        "kernels": kern(tim64),
+        "mixed": {"value": round(N * R / (msmx / 1e3), 1), "ms_per_step": round(msmx, 3),
+                  "kernels": kern(timmx)},
+        "accuracy": accuracy,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -639,6 +809,8 @@ def main():
         return bench_stats(args, spec)
     if spec.get("method") == "indicator":
         return bench_indicator(args, spec)
+    if spec.get("method") == "lusim":
+        return bench_lusim(args, spec)
     args.precision = args.precision or spec.get("precision", "fp32r")
     precision = _lib.FP32_REFINE if args.precision == "fp32r" else _lib.FP64
     R = spec["R"]

@@ -89,7 +89,7 @@ class GspSeqsimInfo(ctypes.Structure):
 
 
 (K_PLAN, K_LEVELS, K_SWEEP, K_TRANSPOSE, K_LU_FACTOR, K_LU_SAMPLE, K_SEARCH, K_LEARNED, K_FFT,
- K_SPECTRAL, K_STATS, K_COUNT) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
+ K_SPECTRAL, K_STATS, K_LU_TRAIL, K_COUNT) = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12
 STAT_MEAN, STAT_VAR, STAT_CDF, STAT_CCDF, STAT_QUANTILE = 0, 1, 2, 3, 4
 
 

@@ -160,7 +160,8 @@ enum {
   GSP_K_FFT = 8,         /* FFTSIM transforms (rocFFT through hipFFT) */
   GSP_K_SPECTRAL = 9,    /* FFTSIM phase/amplitude, variance, scaling, conditioning */
   GSP_K_STATS = 10,      /* ensemble statistics */
-  GSP_K_COUNT = 11
+  GSP_K_LU_TRAIL = 11,   /* LUSIM trailing updates of the Cholesky panels (per launch) */
+  GSP_K_COUNT = 12
 };
 typedef struct {
   double ms[GSP_K_COUNT];       /* accumulated milliseconds */
