@@ -92,11 +92,11 @@ WORKLOADS = {
     "seqsim-3d-128x128x64": dict(dims=(128, 128, 64), family=1, sill=1.0, ranges=(40.0, 40.0, 8.0),
                                  nugget=0.05, n_data=2000, kmax=24, radius=40.0, R=64, seed=1003),
     "seqsim-2d-100x100": dict(dims=(100, 100), family=0, sill=1.0, ranges=(10.0, 10.0), nugget=0.0,
-                              n_data=50, kmax=12, radius=30.0, R=10, seed=1001),
+                              n_data=50, kmax=12, radius=30.0, R=10, seed=1001, precision="fp64"),
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
     "seqsim-learned-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,
-        kmax=24, radius=20.0, R=16, seed=1004, cpu_dims=(128, 128, 128),
+        kmax=24, radius=20.0, R=16, seed=1004,
         learned=dict(train_dims=(128, 128, 64), train_plans=3, samples=2_000_000, epochs=20,
                      batch=1024, lr=2e-3, tolerance=0.05)),
     "kriging-mean-256x256x128": dict(
@@ -209,58 +209,75 @@ def profiled_traffic(workload, kernel):
     return int(total / n) if n else None
 
 
-def cpu_baseline(reqs, precision, R_batch=None, node_budget=None, node_limit=0):
-    """The oracle's reference-cost restatement (KD-free cell-hash search +
-    Cholesky per node per realization), OpenMP over realizations.
+def cpu_share():
+    """(CPU model, cores visible to this process, threads used): one GPU's
+    share of the box's host is 16 cores, whatever the machine has."""
+    model, ncores = cpu_info()
+    return model, ncores, max(1, min(16, ncores))
 
-    Single workloads: one full realization per thread, repeated until about
-    10 s of CPU time. Batches: the first
-    requests up to `node_budget` nodes, with the workload's R realizations
-    each (every path position costs differently, so samples are whole
-    realizations)."""
+
+def cpu_baseline(reqs, R_batch=None, node_budget=None, node_limit=0):
+    """The oracle's restatement of the reference's cost structure: search +
+    FP64 Cholesky solve per node per realization (seqsim.jl:98-138), with a
+    cell-hash neighbour search (on grids faster than the reference's KD-tree,
+    so the figure favours the CPU), OpenMP over realizations like pmap over
+    workers (field.jl:113-119). Two legs on the workload's own grid(s): one
+    thread (the reference's default of one process) and the GPU's 16-core
+    share of the host.
+
+    Single workloads: whole realizations, one per thread, repeated until the
+    leg has run for 3 s. Batches: the first requests up to `node_budget` nodes,
+    R_batch realizations each. node_limit: the first node_limit path nodes of
+    each realization (the posterior mean, whose nodes are independent)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
-    if R_batch is not None:
-        threads = min(threads, R_batch)
-    n_total, dt_total, used, nodes = 0, 0.0, 0, 0
-    for spec in reqs:
-        dims = spec["dims"]
-        case = O.SeqsimCase(dims=dims, family=spec["family"], sill=spec["sill"],
-                            ranges=spec["ranges"][:len(dims)], nugget=spec["nugget"],
-                            kmax=spec["kmax"], radius=spec["radius"], data_coords=spec["data"],
-                            data_vals=spec["vals"], path=spec["path"], precision=precision,
-                            mode=O.DATA_ONLY if spec.get("mode") == "data_only" else O.KNN_THEN_MASK)
-        N = case.n_nodes
-        R = R_batch if R_batch is not None else threads
-        z = np.random.default_rng(7).standard_normal((R, N))
-        # single workloads repeat their realizations up to ~10 s of CPU time
-        reps = 0
-        while True:
-            t0 = time.perf_counter()
-            _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=node_limit or N)
-            dt_total += time.perf_counter() - t0
-            n_total += n
-            reps += 1
-            if R_batch is not None or node_limit or dt_total >= 10.0 or reps >= 4096:
+    model, ncores, share = cpu_share()
+
+    def leg(threads):
+        n_total, dt_total, used, nodes, reps = 0, 0.0, 0, 0, 0
+        for spec in reqs:
+            dims = spec["dims"]
+            case = O.SeqsimCase(dims=dims, family=spec["family"], sill=spec["sill"],
+                                ranges=spec["ranges"][:len(dims)], nugget=spec["nugget"],
+                                kmax=spec["kmax"], radius=spec["radius"], data_coords=spec["data"],
+                                data_vals=spec["vals"], path=spec["path"], precision=O.FP64,
+                                mode=O.DATA_ONLY if spec.get("mode") == "data_only" else O.KNN_THEN_MASK)
+            N = case.n_nodes
+            R = R_batch if R_batch is not None else threads
+            z = np.random.default_rng(7).standard_normal((R, N))
+            reps = 0
+            while True:
+                t0 = time.perf_counter()
+                _, n = O.seqsim_reference_cost(case, z, threads=threads, n_nodes_limit=node_limit or N)
+                dt_total += time.perf_counter() - t0
+                n_total += n
+                reps += 1
+                if R_batch is not None or dt_total >= 3.0:
+                    break
+            used += 1
+            nodes += N
+            if node_budget is None or nodes >= node_budget:
                 break
-        used += 1
-        nodes += N
-        if node_budget is None or nodes >= node_budget:
-            break
+        return n_total / dt_total, dt_total, used, nodes, reps
+
+    v1, t1, used, nodes, reps1 = leg(1)
+    vs, ts, _, _, repss = leg(share)
     if node_limit:
-        sample = (f"{threads} passes (one per thread) over the first {node_limit} nodes of the "
-                  f"path ({dt_total:.1f} s; search + kriging solve per node, {threads} threads)")
+        what = f"the first {node_limit} nodes of the path"
+        s1, ss = f"1 pass over {what} ({t1:.1f} s)", f"{share} passes over {what} ({ts:.1f} s)"
     elif R_batch is None:
-        sample = (f"{threads * reps} full realizations ({nodes} path nodes each) of the same workload "
-                  f"({dt_total:.1f} s; search + kriging solve per node per realization, "
-                  f"OpenMP over realizations, {threads} threads)")
+        what = f"full realizations of the workload's {nodes}-node grid"
+        s1 = f"{reps1} {what} at 1 thread ({t1:.1f} s)"
+        ss = f"{share * repss} {what}, one per thread ({ts:.1f} s)"
     else:
-        sample = (f"the first {used} requests of the batch ({nodes} nodes), {R_batch} full "
-                  f"realizations each ({dt_total:.1f} s; search + kriging solve per node per "
-                  f"realization, OpenMP over realizations, {threads} threads)")
-    return {"value": n_total / dt_total, "unit": "nodes*realizations/s", "cores": threads,
-            "kind": "port", "sample": sample}
+        what = f"the first {used} requests of the batch ({nodes} nodes), {R_batch} full realizations each"
+        s1, ss = f"{what} at 1 thread ({t1:.1f} s)", f"{what} over {share} threads ({ts:.1f} s)"
+    common = ("search + FP64 kriging solve per node per realization (the reference's cost "
+              "structure; cell-hash search)")
+    return {"value": vs, "unit": "nodes*realizations/s", "cores": share, "kind": "port",
+            "sample": f"{ss}; {common}",
+            "one_thread": {"value": v1, "cores": 1, "sample": s1},
+            "cpu_model": model, "cores_visible": ncores}
 
 
 MLP_FLOPS = 2 * (73 * 256 + 2 * 256 * 256 + 256 * 25)   # per node, one forward pass
@@ -338,12 +355,21 @@ def model_variogram(spec, h):
     return (spec["sill"] - spec["nugget"]) * shape + spec["nugget"]
 
 
-def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, S, tol, z, N, R):
+def axis_variogram(fields, dims, axis, lags):
+    """Experimental semivariogram of realizations (R, N) along one grid axis,
+    averaged over realizations: gamma(h) = mean (x(i + h e) - x(i))^2 / 2."""
+    X = fields.reshape((fields.shape[0],) + tuple(reversed(dims)))  # column-major ids
+    Xm = np.moveaxis(X, len(dims) - axis, -1)
+    return np.array([0.5 * float(np.mean((Xm[..., h:] - Xm[..., :-h]) ** 2, dtype=np.float64))
+                     for h in lags])
+
+
+def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol, z, N, R):
     """RMS weight error of a learned plan against the exact FP64 plan, and the
     experimental variograms of learned and exact realizations drawn from the
-    same normals (paired), along each axis at lags 1..20."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    same normals (paired), along each axis at lags 1..20; beside them the same
+    variogram of exact realizations with the masked k-nearest search (the k
+    nearest among simulated points) against the model."""
     lp = _lib.SeqsimPlan(ctx, prob_fp32r, surrogate=S, tolerance=tol)
     ep = _lib.SeqsimPlan(ctx, prob_fp64)
     gl = lp.export(fields=("nbr", "lam", "sigma", "flags"))
@@ -358,7 +384,8 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, S, tol, z, N, R):
     Rv = min(R, 4)
     dims = spec["dims"]
     outs = []
-    for plan in (lp, ep):
+    mp = _lib.SeqsimPlan(ctx, prob_masked)
+    for plan in (lp, ep, mp):
         out = _lib.DeviceBuffer(ctx, (R, N))
         plan.sweep(z, None, out, device_pointers=True)
         ctx.synchronize()
@@ -367,20 +394,26 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, S, tol, z, N, R):
         outs.append(host.astype(np.float32))
         out.free()
     lags = list(range(1, 21))
-    paired, vs_model = [], []
+    paired, vs_model, exact_vs_model, masked_vs_model = [], [], [], []
+    gm = model_variogram(spec, lags)
     for axis in range(len(dims)):
-        gl_ = O.axis_variogram(outs[0], dims, axis, lags)
-        ge_ = O.axis_variogram(outs[1], dims, axis, lags)
+        gl_ = axis_variogram(outs[0], dims, axis, lags)
+        ge_ = axis_variogram(outs[1], dims, axis, lags)
+        gk_ = axis_variogram(outs[2], dims, axis, lags)
         paired.append(float(np.linalg.norm(gl_ - ge_) / np.linalg.norm(ge_)))
-        gm = model_variogram(spec, lags)
         vs_model.append(float(np.linalg.norm(gl_ - gm) / np.linalg.norm(gm)))
+        exact_vs_model.append(float(np.linalg.norm(ge_ - gm) / np.linalg.norm(gm)))
+        masked_vs_model.append(float(np.linalg.norm(gk_ - gm) / np.linalg.norm(gm)))
     lp.close()
     ep.close()
+    mp.close()
     return {"rms_weight_error_learned_nodes": rms_learned, "rms_weight_error_all_nodes": rms_all,
             "sigma_rel_error_median": float(np.median(sig)), "sigma_rel_error_p99": float(np.quantile(sig, 0.99)),
             "learned_fraction": float(learned.mean()),
             "variogram_error_paired_per_axis": [round(e, 5) for e in paired],
             "variogram_error_vs_model_per_axis": [round(e, 5) for e in vs_model],
+            "exact_variogram_error_vs_model_per_axis": [round(e, 5) for e in exact_vs_model],
+            "masked_knn_exact_variogram_error_vs_model_per_axis": [round(e, 5) for e in masked_vs_model],
             "variogram_sample": f"{Rv} realizations, lags 1..20 cells along each axis"}
 
 
@@ -721,23 +754,23 @@ def bench_lusim(args, spec):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         from threadpoolctl import threadpool_limits
-        model, ncores = cpu_info()
+        model, ncores, share = cpu_share()
         cents = O.grid_centroids(dims)
         res = {}
-        for threads in (1, ncores):
+        for threads in (1, share):
             with threadpool_limits(limits=threads):
                 t0 = time.perf_counter()
                 d2, L22, sinds = O.lusim_preprocess(cents, dinds, z1, spec["family"], spec["sill"],
                                                     spec["nugget"], spec["ranges"])
                 ref = O.lusim_sample(d2, L22, dinds, sinds, z1, 0.0, w)
                 res[threads] = time.perf_counter() - t0
-        cpu = {"value": N * R / res[ncores], "unit": "nodes*realizations/s", "cores": ncores,
+        cpu = {"value": N * R / res[share], "unit": "nodes*realizations/s", "cores": share,
                "kind": "port",
                "sample": (f"the whole workload: numpy/scipy (LAPACK potrf/trsm, BLAS gemm) "
-                          f"restatement of lusim.jl, FP64, {res[ncores]:.2f} s at {ncores} threads"),
+                          f"restatement of lusim.jl, FP64, {res[share]:.2f} s at {share} threads"),
                "one_thread": {"value": N * R / res[1], "cores": 1,
                               "sample": f"the same at 1 thread, {res[1]:.2f} s"},
-               "cpu_model": model}
+               "cpu_model": model, "cores_visible": ncores}
 
         def per_real(out):
             e = np.max(np.abs(out - ref), axis=1) / np.max(np.abs(ref), axis=1)
@@ -876,34 +909,39 @@ def main():
     kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned)
     roofline = roofline_of(kern, args.workload)
 
-    exact = accuracy = None
+    exact = exact64 = accuracy = None
     if S is not None:
-        def step_exact():
-            plan = _lib.SeqsimPlan(ctx, plan_arg)
-            plan.sweep(z, dvals, out, device_pointers=True)
-            ctx.synchronize()
-            return plan
-        ems, etim = timed(ctx, step_exact, args.steps, args.warmup)
+        def exact_step(arg):
+            def step_exact():
+                plan = _lib.SeqsimPlan(ctx, arg)
+                plan.sweep(z, dvals, out, device_pointers=True)
+                ctx.synchronize()
+                return plan
+            return step_exact
+        ems, etim = timed(ctx, exact_step(plan_arg), args.steps, args.warmup)
         exact = {"value": round(N * R / (ems / 1e3), 1), "ms_per_step": round(ems, 3),
-                 "kernels": kernel_table(_lib, etim, args.steps, N, R, K, kbar)}
+                 "precision": "fp32r", "kernels": kernel_table(_lib, etim, args.steps, N, R, K, kbar)}
+        ems64, etim64 = timed(ctx, exact_step(problems(_lib.FP64)), args.steps, args.warmup)
+        exact64 = {"value": round(N * R / (ems64 / 1e3), 1), "ms_per_step": round(ems64, 3),
+                   "precision": "fp64",
+                   "kernels": kernel_table(_lib, etim64, args.steps, N, R, K, kbar)}
         if not args.no_accuracy:
-            accuracy = learned_accuracy(ctx, _lib, spec, plan_arg, problems(_lib.FP64), S, tol, z, N, R)
+            masked = problems(_lib.FP32_REFINE)
+            masked.search_mode = _lib.MASKED_KNN
+            accuracy = learned_accuracy(ctx, _lib, spec, plan_arg, problems(_lib.FP64), masked, S,
+                                        tol, z, N, R)
             accuracy["tolerance"] = tol
 
     cpu = None
     if not args.no_cpu:
         if "batch" in spec:
-            cpu = cpu_baseline(reqs, precision, R_batch=R, node_budget=1_500_000)
-        elif "learned" in spec:
-            sub = dict(reqs[0], dims=spec["cpu_dims"],
-                       path=np.random.default_rng(7).permutation(int(np.prod(spec["cpu_dims"]))))
-            cpu = cpu_baseline([sub], precision)
-            cpu["sample"] = f"a {'x'.join(map(str, spec['cpu_dims']))} grid of the same variogram: " + cpu["sample"]
+            cpu = cpu_baseline(reqs, R_batch=R, node_budget=1_500_000)
         elif mean_only:
-            cpu = cpu_baseline(reqs, precision, node_limit=spec["cpu_nodes"])
+            cpu = cpu_baseline(reqs, node_limit=spec["cpu_nodes"])
             cpu["unit"] = "nodes/s"
+            cpu["one_thread"]["unit"] = "nodes/s"
         else:
-            cpu = cpu_baseline(reqs, precision)
+            cpu = cpu_baseline(reqs)
 
     if "batch" in spec:
         config = {"workload": args.workload, "requests": len(reqs), "nodes": N,
@@ -941,7 +979,9 @@ def main():
     if S is not None:
         line["dtype"] = "bf16 mlp (fp32 accumulate), f64 sweep; fallback solves fp32+fp64 refinement"
         line["exact"] = exact
+        line["exact_fp64"] = exact64
         line["speedup_vs_exact"] = round(value / exact["value"], 3)
+        line["speedup_vs_exact_fp64"] = round(value / exact64["value"], 3)
         line["learned_nodes"] = int(info.n_learned)
         line["training"] = training
         line["accuracy"] = accuracy

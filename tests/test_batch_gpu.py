@@ -119,3 +119,92 @@ def test_batch_errors(ctx):
         _lib.SeqsimPlan(ctx, [gpu_problem(a), p], keep=(a, bad2))
     with pytest.raises(_lib.ArgumentError):
         _lib.SeqsimPlan(ctx, [], keep=())
+
+
+def test_config5_full_batch(ctx):
+    """The full mixed-size workload: 512 requests, sides log-uniform in [32, 512]
+    (about 24M nodes), spherical/exponential/gaussian variograms, 0-500 data,
+    maxneighbors in {8, 16, 32}, R = 8, FP32 solves with refinement, in one plan
+    and one sweep. Checked at full size: every node's dependency level is 1 + the
+    largest level of its grid neighbours (0 for prior draws and nodes without
+    grid neighbours: the longest-chain definition) and the plan depth is the
+    largest level + 1; every realization value of two realizations satisfies the
+    sweep recurrence x_i = (mean + sum_j lambda_ij (x_j - mean)) + sigma_i z_i
+    bit for bit, with data neighbours reading the data values. Ten requests of
+    every family and maxneighbors (the largest grid among them) are checked
+    against the oracle bit for bit: neighbours, flags, levels, weights, sigmas
+    and all 8 realizations."""
+    R = 8
+    cases = config5(n_req=512)
+    sides = np.array([c.dims[0] for c in cases])
+    say(f"config5: {len(cases)} requests, {int((sides ** 2).sum())} nodes, "
+        f"{sum(c.n_data for c in cases)} data")
+    probs = [gpu_problem(c) for c in cases]
+    plan = _lib.SeqsimPlan(ctx, probs, keep=tuple(cases))
+    no, do = plan.node_offset, plan.data_offset
+    NT = int(no[-1])
+    assert plan.n_requests == 512 and NT == int((sides ** 2).sum())
+    # normals and data values in the batch layout, on the device
+    rng = np.random.default_rng(12)
+    z = rng.standard_normal((R, NT))
+    vals = np.concatenate([c.data_vals for c in cases])
+    dz, dv = _lib.DeviceBuffer.from_array(ctx, z), _lib.DeviceBuffer.from_array(ctx, vals)
+    dout = _lib.DeviceBuffer(ctx, (R, NT))
+    plan.sweep(dz, dv, dout, device_pointers=True)
+    ctx.synchronize()
+    out = dout.download()
+    say("config5: plan and sweep done")
+    g = plan.export(fields=("nfound", "nbr", "flags", "level"))
+    nbr, flags, level = g["nbr"], g["flags"], g["level"]
+    prior = (flags & 3) != 0
+    # levels: the longest-chain recursion at every node
+    grid_nb = (nbr >= 0) & (nbr < NT) & ~prior[:, None]
+    nb_level = np.where(grid_nb, level[np.where(grid_nb, nbr, 0)], -1)
+    expect = nb_level.max(axis=1) + 1
+    np.testing.assert_array_equal(level, expect)
+    assert plan.info.depth == int(level.max()) + 1
+    del nb_level, expect, grid_nb
+    say(f"config5: levels verified (depth {plan.info.depth})")
+    # sweep recurrence on two realizations, every node
+    ls = plan.export(fields=("lam", "sigma"))
+    lam, sigma = ls["lam"], ls["sigma"]
+    pos = np.empty(NT, np.int64)
+    for q, c in enumerate(cases):
+        pos[no[q] + c.path] = no[q] + np.arange(c.n_nodes)
+    mean = np.repeat([c.mean for c in cases], np.diff(no)).astype(np.float64)
+    k = np.where(prior, 0, g["nfound"])
+    for r in (0, R - 1):
+        xa = np.concatenate([out[r], vals])
+        acc = np.zeros(NT)
+        for j in range(plan.kmax):
+            use = j < k
+            term = lam[:, j] * (xa[np.where(use, nbr[:, j], 0)] - mean)
+            acc = np.where(use, acc + term, acc)
+        x = (mean + acc) + sigma * z[r, pos]
+        np.testing.assert_array_equal(out[r], x, err_msg=f"realization {r}")
+    del xa, acc, term, x
+    say("config5: sweep recurrence verified")
+    # ten requests against the oracle
+    pick = [int(np.argmax(sides))]
+    for kmax in (8, 16, 32):
+        for fam in (O.SPHERICAL, O.EXPONENTIAL, O.GAUSSIAN):
+            cand = [q for q, c in enumerate(cases)
+                    if c.kmax == kmax and c.family == fam and sides[q] <= 160 and q not in pick]
+            if cand:
+                pick.append(cand[0])
+    assert len(pick) >= 10
+    for q in pick[:10]:
+        c = cases[q]
+        say(f"config5 request {q}: side {c.dims[0]} family {c.family} k {c.kmax} data {c.n_data}")
+        o = O.seqsim_plan(c)
+        rows = slice(int(no[q]), int(no[q + 1]))
+        kq = min(c.kmax, c.n_nodes)
+        loc = nbr[rows].astype(np.int64)
+        loc = np.where(loc < 0, -1, np.where(loc < NT, loc - no[q], loc - NT - do[q] + c.n_nodes))
+        np.testing.assert_array_equal(g["nfound"][rows], o.nfound)
+        np.testing.assert_array_equal(loc[:, :kq], o.nbr)
+        np.testing.assert_array_equal(flags[rows], o.flags)
+        np.testing.assert_array_equal(level[rows], o.level)
+        np.testing.assert_array_equal(lam[rows][:, :kq], o.lam)
+        np.testing.assert_array_equal(sigma[rows], o.sigma)
+        np.testing.assert_array_equal(out[:, rows], O.seqsim_sweep(c, o, z[:, rows]))
