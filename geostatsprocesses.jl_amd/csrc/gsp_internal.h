@@ -168,6 +168,9 @@ struct gsp_ctx {
   gsp::DevBuf<int32_t> counters;  // [0] ticket, [1] error flag, [2..] scratch counters
   // FFTSIM transform plans by shape, reused across plans (gsp_fftsim.hip)
   std::shared_ptr<void> fft_plans;
+  // kriging kernel chosen for systems of 17..24 and 25..32 neighbours, per
+  // precision (FP64, FP32 + refinement): -1 until measured (gsp_seqsim.hip)
+  int krige_variant[2][2] = {{-1, -1}, {-1, -1}};
   // kernel timing
   bool timing = false;
   struct Pending {

@@ -976,6 +976,289 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
   }
 }
 
+// ---- two systems per wavefront (16 < n <= KB, KB = 24 or 32) ---------------
+// Lane 32 g + r holds row r of system g (rows r < KB take part). The Cholesky step of column col
+// scales the column in every row's lane, stores it to LDS and every lane of
+// the half reads the entries it needs back as LDS broadcasts: one LDS read and
+// the update per matrix element, where the one-system kernel spends two
+// readlanes and their wait states. Substitution values are made by the lane
+// that owns them and broadcast the same way. Padding to the larger of the two
+// sizes with sill * identity rows and the operation order are those of
+// k_krige16 and the one-system kernels (bitwise equal results).
+template <int PREC>
+struct Krige32Lds {
+  double x[2][32][3];
+  union {
+    double A[2][32][33];  // assembled matrix, later the FP64 factor
+    float F[2][32][33];   // FP32 factor (the FP64 rows stay in registers)
+  };
+  double col[2][32];      // broadcast slots
+  float colf[2][32];
+  DevModel M[2];
+};
+
+// Value of x in lane 32 g + m of this lane's half (m wave-uniform).
+__device__ inline double half_lane(double x, int m, int g) {
+  const double v0 = rl(x, m), v1 = rl(x, 32 + m);
+  return g ? v1 : v0;
+}
+
+__device__ inline float half_lane_f(float x, int m, int g) {
+  const float v0 = rlf(x, m), v1 = rlf(x, 32 + m);
+  return g ? v1 : v0;
+}
+
+// FP64 Cholesky and substitutions of the two systems (rows in a[]); true in
+// the lanes of a system whose factorisation met a non-positive pivot.
+template <int KB, int PREC>
+__device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax, double c,
+                                     double C0, double* a, double& lam_out, double& sigma_out) {
+  bool fail = false;
+  static_for<0, KB>([&](auto colc) {
+    constexpr int col = decltype(colc)::value;
+    if (col < nmax) {
+      const double piv = half_lane(a[col], col, g);
+      fail = fail || !(piv > 0.0);
+      const double lcc = __builtin_sqrt(piv);
+      a[col] = r == col ? lcc : (r > col ? a[col] / lcc : a[col]);
+      L.col[g][r] = a[col];
+      wave_sync();
+      static_for<col + 1, KB>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if (m < nmax) {
+          const double lmc = L.col[g][m];
+          a[m] = a[m] - a[col] * lmc;
+        }
+      });
+      wave_sync();
+    }
+  });
+  // forward: y_m = s_m / L_mm, s_r -= L_rm y_m below it
+  double s = c, y = 0.0;
+  static_for<0, KB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < nmax) {
+      const double ym = half_lane(s, m, g) / half_lane(a[m], m, g);
+      if (r == m) y = ym;
+      if (r > m) s = s - a[m] * ym;
+    }
+  });
+  static_for<0, KB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m <= r) L.A[g][r][m] = a[m];
+  });
+  wave_sync();
+  // back: lambda_m = s2_m / L_mm, s2_r -= L_mr lambda_m above it
+  double s2 = y, lam = 0.0;
+  static_for_down<KB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < nmax) {
+      const double lm = half_lane(s2, m, g) / L.A[g][m][m];
+      if (r == m) lam = lm;
+      if (r < m) s2 = s2 - L.A[g][m][r] * lm;
+    }
+  });
+  // sigma^2 = C0 - sum_j lambda_j c_j in ascending j
+  double acc = 0.0;
+  static_for<0, KB>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < nmax) acc = acc + half_lane(lam, j, g) * half_lane(c, j, g);
+  });
+  double var = C0 - acc;
+  if (var < 0.0) var = 0.0;
+  lam_out = lam;
+  sigma_out = __builtin_sqrt(var);
+  return fail;
+}
+
+constexpr int k32Waves = 1;
+
+template <int KB, int PREC>
+__global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, const int32_t* list,
+                                                         int32_t count) {
+  __shared__ Krige32Lds<PREC> s_lds[k32Waves];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int g = lane >> 5, r = lane & 31;
+  Krige32Lds<PREC>& L = s_lds[w];
+  const int64_t stride = (int64_t)gridDim.x * k32Waves * 2;
+  for (int64_t base = ((int64_t)blockIdx.x * k32Waves + w) * 2; base < count; base += stride) {
+    const int64_t slot = base + g;
+    const bool act = slot < count;
+    const int gi = act ? list[slot] : list[base];
+    const int p = A.node_req[gi];
+    const Req& R = A.reqs[p];
+    const int p0 = rli(p, 0);
+    const bool uniform = __ballot(p != p0) == 0;  // both systems of one request
+    const double C0 = R.M.sill;
+    const int n = act ? A.nfound[gi] : 0;
+    const int nb = (act && r < n) ? A.nbr[(int64_t)gi * A.K + r] : -1;
+    const bool solve = act && n >= R.kmin;
+    const int ns = solve ? n : 0;
+    const int nmax = rli(ns, 0) > rli(ns, 32) ? rli(ns, 0) : rli(ns, 32);
+    double lam = 0.0, sig = R.prior_sigma;
+    int flag = act && !solve ? GSP_FLAG_FEW : 0;
+    if (nmax > 0) {
+      double x0 = 0.0, x1 = 0.0, x2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+      if (r < ns) {
+        neighbour_coords(A, R, nb, x0, x1, x2);
+        L.x[g][r][0] = x0;
+        L.x[g][r][1] = x1;
+        L.x[g][r][2] = x2;
+      }
+      if (solve) node_coords(R, gi - R.node_off, t0, t1, t2);
+      if (!uniform && r < 16) ((double*)&L.M[g])[r] = ((const double*)&R.M)[r];
+      wave_sync();
+      // covariance entries of each system's lower triangle over its 32 lanes
+      const int T = ns * (ns + 1) / 2, Tmax = nmax * (nmax + 1) / 2;
+      auto assemble32 = [&](const DevModel& M) {
+        for (int e = r; e < Tmax; e += 32) {
+          if (e < T) {
+            int j, m;
+            tri_unpack(e, j, m);
+            const double v = cov_points(M, L.x[g][j][0], L.x[g][j][1], L.x[g][j][2],
+                                        L.x[g][m][0], L.x[g][m][1], L.x[g][m][2]);
+            L.A[g][j][m] = v;
+            L.A[g][m][j] = v;
+          }
+        }
+        return r < ns ? cov_points(M, x0, x1, x2, t0, t1, t2) : 0.0;
+      };
+      const double c = uniform ? assemble32(A.reqs[p0].M) : assemble32(L.M[g]);
+      wave_sync();
+      double a[KB];
+      static_for<0, KB>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        a[m] = (r < ns && m < ns) ? L.A[g][r][m] : (m == r ? C0 : 0.0);
+      });
+      wave_sync();
+      bool fail;
+      bool fell_back = false;
+      if constexpr (PREC == GSP_FP32_REFINE) {
+        const float tau = (float)(C0 * 0x1p-12);
+        float f[KB];
+        static_for<0, KB>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          f[m] = (float)a[m];
+        });
+        bool f32fail = false;
+        static_for<0, KB>([&](auto colc) {
+          constexpr int col = decltype(colc)::value;
+          if (col < nmax) {
+            const float piv = __builtin_bit_cast(
+                float, g ? __builtin_amdgcn_readlane(__builtin_bit_cast(int, f[col]), 32 + col)
+                         : __builtin_amdgcn_readlane(__builtin_bit_cast(int, f[col]), col));
+            f32fail = f32fail || !(piv > tau);
+            const float lcc = __builtin_sqrtf(piv);
+            f[col] = r == col ? lcc : (r > col ? f[col] / lcc : f[col]);
+            L.colf[g][r] = f[col];
+            wave_sync();
+            static_for<col + 1, KB>([&](auto mc) {
+              constexpr int m = decltype(mc)::value;
+              if (m < nmax) {
+                const float lmc = L.colf[g][m];
+                f[m] = f[m] - f[col] * lmc;
+              }
+            });
+            wave_sync();
+          }
+        });
+        static_for<0, KB>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          if (m <= r) L.F[g][r][m] = f[m];
+        });
+        wave_sync();
+        float rhs = (float)c, sol = 0.0f, l = 0.0f;
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+          float sv = rhs, y = 0.0f;
+          static_for<0, KB>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (m < nmax) {
+              const float ym = half_lane_f(sv, m, g) / half_lane_f(f[m], m, g);
+              if (r == m) y = ym;
+              if (r > m) sv = sv - f[m] * ym;
+            }
+          });
+          float s2 = y;
+          l = 0.0f;
+          static_for_down<KB>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (m < nmax) {
+              const float lm = half_lane_f(s2, m, g) / L.F[g][m][m];
+              if (r == m) l = lm;
+              if (r < m) s2 = s2 - L.F[g][m][r] * lm;
+            }
+          });
+          wave_sync();
+          if (pass == 0) {
+            sol = l;
+            L.colf[g][r] = l;
+            wave_sync();
+            double res = c;
+            static_for<0, KB>([&](auto mc) {
+              constexpr int m = decltype(mc)::value;
+              if (m < nmax) res = res - a[m] * (double)L.colf[g][m];
+            });
+            rhs = (float)res;
+            wave_sync();
+          }
+        }
+        lam = (double)sol + (double)l;
+        L.col[g][r] = lam;
+        wave_sync();
+        double acc = 0.0;
+        static_for<0, KB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < nmax) acc = acc + L.col[g][j] * half_lane(c, j, g);
+        });
+        double var = C0 - acc;
+        if (var < 0.0) var = 0.0;
+        sig = __builtin_sqrt(var);
+        fail = false;
+        wave_sync();
+        // systems whose FP32 factor failed are solved again in FP64
+        if (__ballot(f32fail && r == 0)) {
+          double lam64, sig64;
+          wave_sync();
+          const bool fail64 = packed32_fp64<KB, PREC>(L, g, r, nmax, c, C0, a, lam64, sig64);
+          if (f32fail) {
+            lam = lam64;
+            sig = sig64;
+            fail = fail64;
+            fell_back = true;
+          }
+        }
+      } else {
+        fail = packed32_fp64<KB, PREC>(L, g, r, nmax, c, C0, a, lam, sig);
+      }
+      wave_sync();
+      if (solve) {
+        if (fell_back) flag |= GSP_FLAG_FP64_FALLBACK;
+        if (fail) {
+          flag |= GSP_FLAG_FAIL;
+          lam = 0.0;
+          sig = R.prior_sigma;
+        }
+      } else {
+        lam = 0.0;
+        sig = R.prior_sigma;
+      }
+    }
+    if (act) {
+      for (int col = r; col < A.K; col += 32) A.lam[(int64_t)gi * A.K + col] = col < n ? lam : 0.0;
+    }
+    if (act && r == 0) {
+      A.sigma[gi] = sig;
+      A.flags[gi] = (uint8_t)flag;
+      if (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL)) atomicAdd(&A.counters[CTR_PRIOR], 1);
+      if (flag & GSP_FLAG_FAIL) atomicAdd(&A.counters[CTR_FAIL], 1);
+      if (flag & GSP_FLAG_FP64_FALLBACK) atomicAdd(&A.counters[CTR_FALLBACK], 1);
+    }
+    wave_sync();
+  }
+}
+
 // Nodes (ids[0..N), or 0..N-1 without ids) bucketed by neighbour count
 // (counting sort), so packed wavefronts hold systems of one size.
 __global__ __launch_bounds__(256) void k_n_hist(const int32_t* nfound, const int32_t* ids,
@@ -1891,6 +2174,69 @@ void launch_krige(const PlanArgs& A, int precision, const int32_t* list, int64_t
                        (int32_t)count);
 }
 
+// ---- kriging of 16 < n <= 32: two kernels, the faster one measured --------
+// k_krige32 (two systems per wavefront, LDS column broadcasts) and k_krige<KB>
+// (one system per wavefront, readlanes) give identical results. Which is
+// faster depends on the size bucket, the precision and the work beside them,
+// so the first plan of a context with at least kTuneMin systems in a bucket
+// times both, twice in alternation, on a slice of kTuneSample systems from
+// the middle of the bucket (which is sorted by size) and keeps the faster for
+// that bucket and precision (GSP_KRIGE_VARIANT=0 or 1 forces one; until
+// measured, one system per wavefront).
+enum { kVarPair = 0, kVarSingle = 1 };
+constexpr int64_t kTuneMin = 65536, kTuneSample = 32768;
+
+template <int KB>
+void launch_variant(int variant, const PlanArgs& A, int precision, const int32_t* list,
+                    int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  if (variant == kVarSingle) {
+    launch_krige<KB>(A, precision, list, count, s);
+    return;
+  }
+  const int blocks = grid_for((count + 1) / 2, k32Waves, 65536);
+  if (precision == GSP_FP32_REFINE)
+    hipLaunchKernelGGL((k_krige32<KB, GSP_FP32_REFINE>), dim3(blocks), dim3(64 * k32Waves), 0, s,
+                       A, list, (int32_t)count);
+  else
+    hipLaunchKernelGGL((k_krige32<KB, GSP_FP64>), dim3(blocks), dim3(64 * k32Waves), 0, s, A, list,
+                       (int32_t)count);
+}
+
+template <int KB>
+void launch_mid(gsp_ctx* ctx, const PlanArgs& A, int precision, const int32_t* list,
+                int64_t count, hipStream_t s) {
+  if (count <= 0) return;
+  int& v = ctx->krige_variant[precision == GSP_FP32_REFINE ? 1 : 0][KB == 24 ? 0 : 1];
+  if (const char* env = std::getenv("GSP_KRIGE_VARIANT")) {
+    v = std::atoi(env) == kVarSingle ? kVarSingle : kVarPair;
+  } else if (v < 0 && count >= kTuneMin) {
+    // the sample's outputs are written again by the full launch; its counters
+    // go to a scratch buffer
+    DevBuf<int32_t> ctr;
+    ctr.alloc(16);
+    PlanArgs T = A;
+    T.counters = ctr.p;
+    const int32_t* slice = list + (count - kTuneSample) / 2;
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 4; ++rep) {
+      const int var = rep & 1;
+      hipEvent_t e0 = take_event(ctx), e1 = take_event(ctx);
+      GSP_HIP(hipEventRecord(e0, s));
+      launch_variant<KB>(var, T, precision, slice, kTuneSample, s);
+      GSP_HIP(hipEventRecord(e1, s));
+      GSP_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      GSP_HIP(hipEventElapsedTime(&ms, e0, e1));
+      best[var] = std::min(best[var], ms);
+      ctx->event_pool.push_back(e0);
+      ctx->event_pool.push_back(e1);
+    }
+    v = best[kVarPair] < best[kVarSingle] ? kVarPair : kVarSingle;
+  }
+  launch_variant<KB>(v < 0 ? kVarSingle : v, A, precision, list, count, s);
+}
+
 // Runs `f` for request index q; on failure in a batch of several requests the
 // error message names the request.
 template <class F>
@@ -2520,8 +2866,8 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
         hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(64 * k16Waves), 0, s, A,
                            list.p, (int32_t)small);
     }
-    launch_krige<24>(A, precision, list.p + upto[16], upto[24] - upto[16], s);
-    launch_krige<32>(A, precision, list.p + upto[24], upto[32] - upto[24], s);
+    launch_mid<24>(ctx, A, precision, list.p + upto[16], upto[24] - upto[16], s);
+    launch_mid<32>(ctx, A, precision, list.p + upto[24], upto[32] - upto[24], s);
     launch_krige<48>(A, precision, list.p + upto[32], upto[48] - upto[32], s);
     launch_krige<64>(A, precision, list.p + upto[48], upto[64] - upto[48], s);
     GSP_HIP(hipGetLastError());
