@@ -909,6 +909,28 @@ def main():
     kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned)
     roofline = roofline_of(kern, args.workload)
 
+    # repeated sweeps of one plan (a serving loop): captured HIP graph vs one
+    # launch per level
+    sweep_only = None
+    if S is None and not mean_only:
+        plan = _lib.SeqsimPlan(ctx, plan_arg)
+        res = {}
+        for mode in ("0", "1"):
+            os.environ["GSP_SWEEP_GRAPH"] = mode
+            for _ in range(3):
+                plan.sweep(z, dvals, out, device_pointers=True)
+            ctx.synchronize()
+            reps = 20
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                plan.sweep(z, dvals, out, device_pointers=True)
+            ctx.synchronize()
+            res[mode] = (time.perf_counter() - t0) * 1e3 / reps
+        os.environ.pop("GSP_SWEEP_GRAPH", None)
+        plan.close()
+        sweep_only = {"eager_ms": round(res["0"], 4), "graph_ms": round(res["1"], 4),
+                      "graph_value": round(N * R / (res["1"] / 1e3), 1), "levels": info.depth}
+
     exact = exact64 = accuracy = None
     if S is not None:
         def exact_step(arg):
@@ -976,6 +998,8 @@ def main():
         "kernels": kern,
         "cpu_baseline": cpu,
     }
+    if sweep_only is not None:
+        line["sweep_only"] = sweep_only
     if S is not None:
         line["dtype"] = "bf16 mlp (fp32 accumulate), f64 sweep; fallback solves fp32+fp64 refinement"
         line["exact"] = exact

@@ -93,6 +93,28 @@ struct gsp_seqsim_plan {
   gsp::DevBuf<int32_t> pos;     // path position of every node
   gsp::DevBuf<int32_t> order;   // nodes grouped by level
   std::vector<int64_t> level_off;  // level l holds order[level_off[l] .. level_off[l+1])
+  // the last sweep captured as a HIP graph, replayed while every pointer and
+  // size its kernels were captured with stays the same
+  struct SweepKey {
+    int32_t R = 0, RC = 0;
+    const void *z = nullptr, *vals = nullptr, *out = nullptr, *X = nullptr, *Zt = nullptr;
+    bool operator==(const SweepKey& o) const {
+      return R == o.R && RC == o.RC && z == o.z && vals == o.vals && out == o.out && X == o.X &&
+             Zt == o.Zt;
+    }
+  };
+  mutable hipGraphExec_t sweep_graph = nullptr;
+  mutable hipStream_t sweep_stream = nullptr;
+  mutable SweepKey sweep_key;
+  // (an executable graph still in flight is freed when it completes; the
+  // plan's buffers go back to the context pool, which reuses them only for
+  // work queued after it on the same stream)
+  void drop_sweep_graph() const {
+    if (!sweep_graph) return;
+    (void)hipGraphExecDestroy(sweep_graph);
+    sweep_graph = nullptr;
+  }
+  ~gsp_seqsim_plan() { drop_sweep_graph(); }
 };
 
 namespace gsp {

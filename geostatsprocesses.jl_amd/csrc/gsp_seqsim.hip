@@ -2926,6 +2926,16 @@ int sweep_chunk(int64_t rows, int32_t R) {
   return rc;
 }
 
+// Sweeps with at least this many levels are captured into a HIP graph and
+// replayed (GSP_SWEEP_GRAPH=0 turns it off): one graph launch instead of one
+// kernel launch per level, transposes included.
+constexpr int32_t kGraphMinLevels = 8;
+
+bool sweep_graphs_on() {
+  const char* env = std::getenv("GSP_SWEEP_GRAPH");
+  return !(env && env[0] == '0');
+}
+
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out) {
   const int64_t N = plan->N, nD = plan->n_data;
@@ -2938,7 +2948,41 @@ void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, cons
   w.Zt = ctx->Zt.p;
   w.RCmax = RCmax;
   w.timed = true;
-  seqsim_sweep_into(ctx, plan, R, z, data_vals, out, w);
+  // with timing events on, launches stay individual (they are bracketed)
+  if (ctx->timing || plan->depth < kGraphMinLevels || !sweep_graphs_on()) {
+    seqsim_sweep_into(ctx, plan, R, z, data_vals, out, w);
+    return;
+  }
+  gsp_seqsim_plan::SweepKey key;
+  key.R = R;
+  key.RC = RCmax;
+  key.z = z;
+  key.vals = data_vals;
+  key.out = out;
+  key.X = w.X;
+  key.Zt = w.Zt;
+  hipStream_t s = ctx->stream;
+  if (!plan->sweep_graph || !(plan->sweep_key == key) || plan->sweep_stream != s) {
+    plan->drop_sweep_graph();
+    w.timed = false;
+    hipGraph_t graph = nullptr;
+    GSP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      seqsim_sweep_into(ctx, plan, R, z, data_vals, out, w);
+    } catch (...) {
+      if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    GSP_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GSP_HIP(e);
+    plan->sweep_graph = exec;
+    plan->sweep_stream = s;
+    plan->sweep_key = key;
+  }
+  GSP_HIP(hipGraphLaunch(plan->sweep_graph, s));
 }
 
 void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
