@@ -429,6 +429,41 @@ int32_t gsp_seqsim_sweep(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, c
   });
 }
 
+int32_t gsp_seqsim_sweep_multivariate(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                      int32_t nvars, const double* A, const double* mean,
+                                      const double* z, const double* data_vals, double* out,
+                                      int32_t device_pointers) {
+  if (!plan) {
+    gsp::set_error("null plan");
+    return GSP_ERR_ARGUMENT;
+  }
+  return gsp_guard(ctx, [&] {
+    GSP_REQUIRE(R >= 1, GSP_ERR_ARGUMENT, "number of realizations must be positive");
+    GSP_REQUIRE(A && mean && z && out, GSP_ERR_ARGUMENT, "null argument");
+    GSP_REQUIRE(plan->n_data == 0 || data_vals, GSP_ERR_ARGUMENT, "null data values");
+    GSP_REQUIRE(nvars >= 1 && nvars <= 4, GSP_ERR_UNSUPPORTED,
+                "multivariate SEQSIM supports 1 to 4 variables");
+    hipStream_t s = ctx->stream;
+    const size_t nz = (size_t)R * (size_t)plan->N * (size_t)nvars;
+    const size_t nd = (size_t)plan->n_data * (size_t)nvars;
+    if (device_pointers) {
+      gsp::seqsim_sweep_multivariate_run(ctx, plan, R, nvars, A, mean, z, data_vals, out);
+    } else {
+      ctx->stage_in.reserve(nz);
+      ctx->stage_in2.reserve(nd + 1);
+      ctx->stage_out.reserve(nz);
+      GSP_HIP(hipMemcpyAsync(ctx->stage_in.p, z, nz * 8, hipMemcpyHostToDevice, s));
+      if (nd > 0)
+        GSP_HIP(hipMemcpyAsync(ctx->stage_in2.p, data_vals, nd * 8, hipMemcpyHostToDevice, s));
+      gsp::seqsim_sweep_multivariate_run(ctx, plan, R, nvars, A, mean, ctx->stage_in.p,
+                                         ctx->stage_in2.p, ctx->stage_out.p);
+      GSP_HIP(hipMemcpyAsync(out, ctx->stage_out.p, nz * 8, hipMemcpyDeviceToHost, s));
+      GSP_HIP(hipStreamSynchronize(s));
+      gsp::flush_timings(ctx);
+    }
+  });
+}
+
 int32_t gsp_seqsim_sweep_indicator(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                                    int32_t n_cat, const double* prob, const double* u,
                                    const int32_t* data_cat, int32_t* out,

@@ -390,6 +390,7 @@ struct SweepScratch {
   double* Zt = nullptr;
   int RCmax = 64;
   bool timed = true;  // bracket the launches with timing events when enabled
+  const double* mean = nullptr;  // per-request means replacing the plan's (multivariate sweeps)
 };
 // Device address of the allocation-free sweep's scratch on `device` (current),
 // looked up once per device (gsp_noalloc.hip).
@@ -397,6 +398,9 @@ void noalloc_init(int device);
 // The sweep on device pointers with caller-provided scratch: allocates nothing.
 void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                        const double* data_vals, double* out, const SweepScratch& scratch);
+void seqsim_sweep_multivariate_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                   int32_t nv, const double* A, const double* mean,
+                                   const double* z, const double* data_vals, double* out);
 void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                                 int32_t n_cat, const double* prob, const double* u,
                                 const int32_t* data_cat, int32_t* out);

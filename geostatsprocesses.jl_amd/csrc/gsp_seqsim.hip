@@ -1712,6 +1712,29 @@ __global__ __launch_bounds__(256) void k_sweep_level(const LevelArgs A) {
   A.X[(int64_t)i * A.RC + r] = mu + A.sigma[i] * A.Zt[(int64_t)A.pos[i] * A.RC + r];
 }
 
+// Multivariate sweep of an intrinsic coregionalization A * c(h): the normals
+// of each path node, nv per node in draw order (the reference's MvNormal
+// draw, seqsim.jl:335-340), mixed by the Cholesky factor of A,
+// w_v = sum_{u <= v} L_vu z_u (ascending u), one array per variable.
+struct MvMix {
+  double L[4][4];
+};
+
+__global__ void k_mv_mix(const double* z, int64_t n_path, int32_t R, int32_t nv, const MvMix M,
+                         double* w) {
+  const int64_t total = (int64_t)R * n_path;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double zv[4];
+    for (int u = 0; u < nv; ++u) zv[u] = z[t * nv + u];
+    for (int v = 0; v < nv; ++v) {
+      double acc = 0.0;
+      for (int u = 0; u <= v; ++u) acc = acc + M.L[v][u] * zv[u];
+      w[(int64_t)v * total + t] = acc;
+    }
+  }
+}
+
 // Indicator sweep (an IndicatorProcess with an intrinsic coregionalization
 // A * c(h): the plan's weights lambda_j serve every category). X holds the
 // 0-based categories as doubles, Zt the uniforms of the categorical draws.
@@ -3016,7 +3039,7 @@ void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, con
     A.lam = plan->lam.p;
     A.sigma = plan->sigma.p;
     A.node_req = plan->node_req.p;
-    A.mean = plan->req_mean.p;
+    A.mean = W.mean ? W.mean : plan->req_mean.p;
     A.Zt = Zts;
     A.X = Xs;
     {
@@ -3041,6 +3064,67 @@ void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, con
   }
 }
 
+
+// Cholesky factor of the nv x nv coefficient matrix (row-major), ascending
+// sums: L_jj = sqrt(A_jj - sum_k L_jk^2), L_ij = (A_ij - sum_k L_ik L_jk) / L_jj.
+MvMix mv_factor(const double* A, int nv) {
+  MvMix M;
+  std::memset(&M, 0, sizeof(M));
+  for (int j = 0; j < nv; ++j) {
+    for (int i = j; i < nv; ++i) {
+      double s = A[i * nv + j];
+      for (int k = 0; k < j; ++k) s = s - M.L[i][k] * M.L[j][k];
+      if (i == j) {
+        GSP_REQUIRE(s > 0.0 && std::isfinite(s), GSP_ERR_POSDEF,
+                    "PosDefException: the coregionalization matrix is not positive definite");
+        M.L[j][j] = std::sqrt(s);
+      } else {
+        M.L[i][j] = s / M.L[j][j];
+      }
+    }
+  }
+  return M;
+}
+
+void seqsim_sweep_multivariate_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                   int32_t nv, const double* A, const double* mean,
+                                   const double* z, const double* data_vals, double* out) {
+  GSP_REQUIRE(nv >= 1 && nv <= 4, GSP_ERR_UNSUPPORTED,
+              "multivariate SEQSIM supports 1 to 4 variables");
+  GSP_REQUIRE(plan->n_req == 1, GSP_ERR_UNSUPPORTED,
+              "multivariate SEQSIM runs plans of one request");
+  for (int i = 0; i < nv; ++i) {
+    GSP_REQUIRE(std::isfinite(mean[i]), GSP_ERR_ARGUMENT, "invalid mean");
+    for (int j = 0; j < nv; ++j)
+      GSP_REQUIRE(A[i * nv + j] == A[j * nv + i], GSP_ERR_ARGUMENT,
+                  "the coregionalization matrix must be symmetric");
+  }
+  const MvMix M = mv_factor(A, nv);
+  hipStream_t s = ctx->stream;
+  const int64_t N = plan->N, nD = plan->n_data, total = (int64_t)R * N;
+  DevBuf<double> w, dmean;
+  w.alloc((size_t)std::max<int64_t>(total * nv, 1));
+  dmean.alloc((size_t)nv);
+  GSP_HIP(hipMemcpyAsync(dmean.p, mean, (size_t)nv * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mv_mix, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, z, N, R, nv, M,
+                     w.p);
+  GSP_HIP(hipGetLastError());
+  const int RCmax = sweep_chunk(N + nD, R);
+  const int RC0 = R < RCmax ? R : RCmax;
+  ctx->X.reserve((size_t)((N + nD) * RC0));
+  ctx->Zt.reserve((size_t)(N * RC0));
+  for (int v = 0; v < nv; ++v) {
+    SweepScratch sw;
+    sw.X = ctx->X.p;
+    sw.Zt = ctx->Zt.p;
+    sw.RCmax = RCmax;
+    sw.timed = true;
+    sw.mean = dmean.p + v;
+    seqsim_sweep_into(ctx, plan, R, w.p + (int64_t)v * total,
+                      nD > 0 ? data_vals + (int64_t)v * nD : nullptr, out + (int64_t)v * total, sw);
+  }
+  GSP_HIP(hipStreamSynchronize(s));  // w and the means are freed on return
+}
 
 void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                                 int32_t n_cat, const double* prob, const double* u,

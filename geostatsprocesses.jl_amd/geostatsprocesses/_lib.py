@@ -132,6 +132,8 @@ SIGNATURES = {
     "gsp_seqsim_plan_free": (_I32, [_VP]),
     "gsp_seqsim_sweep_noalloc": (_I32, [_VP, _VP, _I32, _VP, _VP, _VP]),
     "gsp_seqsim_sweep_indicator": (_I32, [_VP, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _I32]),
+    "gsp_seqsim_sweep_multivariate": (_I32, [_VP, _VP, _I32, _I32, _VP, _VP, _VP, _VP, _VP,
+                                             _I32]),
     "gsp_seqsim_plan_create_batch": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
                                             ctypes.POINTER(_VP)]),
     "gsp_seqsim_plan_layout": (_I32, [_VP, ctypes.POINTER(_I32), _VP, _VP]),
@@ -394,6 +396,26 @@ class SeqsimPlan:
         res = np.empty((R, self.info.n_nodes), np.int32) if out is None else out
         check(load().gsp_seqsim_sweep_indicator(self.ctx.handle, self._h, R, len(prob), ptr(prob),
                                                 ptr(u), ptr(dc), ptr(res), 0))
+        return res
+
+    def sweep_multivariate(self, z, A, mean, data_vals=None):
+        """z: (R, N, nvars) standard normals in draw order; A: (nvars, nvars)
+        coefficient matrix of A * c(h); mean: nvars means; data_vals:
+        (nvars, n_data). Returns (nvars, R, N)."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        R, N, nv = z.shape
+        if N != self.info.n_nodes:
+            raise ArgumentError("z must hold nvars standard normals per path node")
+        A = np.ascontiguousarray(A, dtype=np.float64).reshape(nv, nv)
+        mean = np.ascontiguousarray(np.broadcast_to(np.asarray(mean, dtype=np.float64), (nv,)))
+        dv = None
+        if self.info.n_data:
+            dv = np.ascontiguousarray(data_vals, dtype=np.float64)
+            if dv.shape != (nv, self.info.n_data):
+                raise ArgumentError("data_vals must hold one row of data values per variable")
+        res = np.empty((nv, R, N))
+        check(load().gsp_seqsim_sweep_multivariate(self.ctx.handle, self._h, R, nv, ptr(A),
+                                                   ptr(mean), ptr(z), ptr(dv), ptr(res), 0))
         return res
 
     def sweep(self, z, data_vals=None, out=None, device_pointers=False):

@@ -121,11 +121,16 @@ def _unique_in_order(x):
 
 # ---------------------------------------------------------------- SEQSIM ----
 class SeqsimPreproc:
-    def __init__(self, plan, names, data_vals, path):
+    """The plan and the values the sweeps need; `A` and `means` for a
+    multivariate process of A * c(h) (data_vals then one row per variable)."""
+
+    def __init__(self, plan, names, data_vals, path, A=None, means=None):
         self.plan = plan
         self.names = names
         self.data_vals = data_vals
         self.path = path
+        self.A = A
+        self.means = means
 
 
 def _seqsim_problem(process, method, domain, data, ctx, func=None, data_vals=None):
@@ -133,21 +138,35 @@ def _seqsim_problem(process, method, domain, data, ctx, func=None, data_vals=Non
     for the search and the kriging weights (the spatial part of an indicator
     process); `data_vals` replaces the data values."""
     f = process.func if func is None else func
+    A = means = None
     if f.nvariables() != 1:
-        raise _lib.UnsupportedError("multivariate SEQSIM is not supported by this backend")
+        # an intrinsic coregionalization A * c(h): the plan of c(h) serves every
+        # variable (simple cokriging weights lambda_j * I), the draws couple them
+        if not isinstance(f, CoregionalizedFunction) or not isinstance(process, GaussianProcess):
+            raise _lib.UnsupportedError(
+                "multivariate SEQSIM needs a coregionalized function A * c (intrinsic model)")
+        A = np.ascontiguousarray(f.A, dtype=np.float64)
+        means = np.ascontiguousarray(np.atleast_1d(process.mean), dtype=np.float64)
+        f = f.base
     grid = _grid_struct(domain)
     nd = domain.ndim
     p = _lib.GspSeqsimProblem()
     p.grid = grid
     p.model = f.struct(nd)
-    p.mean = float(process.mean) if isinstance(process, GaussianProcess) else 0.0
+    p.mean = (float(process.mean) if A is None else 0.0) if isinstance(process, GaussianProcess) \
+        else 0.0
     if data is not None:
         coords = np.ascontiguousarray(data.domain.centroids(), dtype=np.float64).reshape(-1, nd)
         names = data.names()
-        if len(names) != 1:
-            raise _lib.UnsupportedError("SEQSIM data must have exactly one variable")
-        vals = np.ascontiguousarray(data[names[0]] if data_vals is None else data_vals,
-                                    dtype=np.float64)
+        nvar = 1 if A is None else len(A)
+        if len(names) != nvar:
+            raise _lib.UnsupportedError(f"SEQSIM data must have {nvar} variable(s)")
+        if A is None:
+            vals = np.ascontiguousarray(data[names[0]] if data_vals is None else data_vals,
+                                        dtype=np.float64)
+        else:
+            vals = np.ascontiguousarray(np.stack([np.asarray(data[n], dtype=np.float64)
+                                                  for n in names]))
         if np.isnan(vals).any():
             raise _lib.UnsupportedError("missing data values are not supported by this backend")
     else:
@@ -174,7 +193,7 @@ def _seqsim_problem(process, method, domain, data, ctx, func=None, data_vals=Non
     p.precision = method.precision
     p.device_pointers = 0
     plan = _lib.SeqsimPlan(ctx, p, keep=(coords, path))
-    return SeqsimPreproc(plan, names, vals, path)
+    return SeqsimPreproc(plan, names, vals, path, A=A, means=means)
 
 
 class IndicatorPreproc:
@@ -364,6 +383,14 @@ def _draw(rng, preproc, R):
         u = rng.random((R, preproc.plan.n_nodes))
         cats = preproc.plan.sweep_indicator(u, preproc.prob, preproc.data_cat)
         return {name: preproc.labels[cats]}
+    if isinstance(preproc, SeqsimPreproc) and preproc.A is not None:
+        # nvars normals per path node (the MvNormal draws of seqsim.jl:335-340)
+        nv = len(preproc.A)
+        z = rng.standard_normal((R, preproc.plan.n_nodes, nv))
+        out = preproc.plan.sweep_multivariate(z, preproc.A, preproc.means,
+                                              preproc.data_vals if preproc.plan.info.n_data
+                                              else None)
+        return {n: out[v] for v, n in enumerate(preproc.names)}
     if isinstance(preproc, SeqsimPreproc):
         z = rng.standard_normal((R, preproc.plan.n_nodes))
         return {name: preproc.plan.sweep(z, preproc.data_vals)}

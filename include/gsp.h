@@ -283,6 +283,27 @@ int32_t gsp_seqsim_plan_free(gsp_seqsim_plan* plan);
  * contexts' streams) race. */
 int32_t gsp_seqsim_sweep_noalloc(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
                                  const double* z, const double* data_vals, double* out);
+/* gsp_seqsim_sweep_multivariate replaces R calls of randsingle(..., ::SEQSIM,
+ * ...) for a GaussianProcess with nvars variables whose function is an
+ * intrinsic coregionalization A * c(h) (seqsim.jl:72-156 with the MvNormal
+ * prior of seqsim.jl:287 and GeoStatsModels' cokriging). With every variable
+ * simulated at every node and known at every datum, simple cokriging with
+ * A * c(h) has the weights lambda_j * I of the scalar system of c(h) and the
+ * error covariance A * sigma^2, so the plan is the one built with c(h) and
+ * each node draws x = mu_SK + sigma L z with L the Cholesky factor of A
+ * (MvNormal's rand: nvars normals per node, z then L z) and mu_SK,v =
+ * mean_v + sum_j lambda_j (x_j,v - mean_v); prior nodes x = mean + sigma L z.
+ *   A:         nvars x nvars, row-major, symmetric positive definite (host memory)
+ *   mean:      nvars process means (host memory)
+ *   z:         R x n_path x nvars standard normals (draw order)
+ *   data_vals: nvars x n_data, one row of data values per variable
+ *   out:       nvars x R x N (variable-major; each variable R x N realization-major)
+ * The plan must hold one request; 1 <= nvars <= 4. GSP_ERR_POSDEF if A is not
+ * positive definite. */
+int32_t gsp_seqsim_sweep_multivariate(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R,
+                                      int32_t nvars, const double* A, const double* mean,
+                                      const double* z, const double* data_vals, double* out,
+                                      int32_t device_pointers);
 /* gsp_seqsim_sweep_indicator replaces R calls of randsingle(..., ::SEQSIM, ...)
  * for an IndicatorProcess (seqsim.jl:247-251,292-310,342-347) whose function is
  * an intrinsic coregionalization A * c(h): with every category observed at

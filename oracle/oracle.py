@@ -19,6 +19,8 @@ from __future__ import annotations
 
 import ctypes
 import os
+import dataclasses
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -297,6 +299,51 @@ def seqsim_sweep(case: SeqsimCase, plan: Plan, z: np.ndarray) -> np.ndarray:
 
 
 # -------------------------------------------------------- indicator SEQSIM ----
+def mv_factor(A):
+    """Cholesky factor of the coregionalization matrix A (lower), with the
+    sums taken in ascending order: L_jj = sqrt(A_jj - sum_k L_jk^2),
+    L_ij = (A_ij - sum_k L_ik L_jk) / L_jj."""
+    A = np.asarray(A, dtype=np.float64)
+    nv = A.shape[0]
+    L = [[0.0] * nv for _ in range(nv)]
+    for j in range(nv):
+        for i in range(j, nv):
+            acc = float(A[i, j])
+            for k in range(j):
+                acc = acc - L[i][k] * L[j][k]
+            if i == j:
+                if not acc > 0.0:
+                    raise np.linalg.LinAlgError("coregionalization matrix not positive definite")
+                L[j][j] = math.sqrt(acc)
+            else:
+                L[i][j] = acc / L[j][j]
+    return np.array(L)
+
+
+def seqsim_sweep_multivariate(case: SeqsimCase, plan: Plan, z, A, means, data_vals):
+    """seqsim.jl:72-156 for a GaussianProcess of A * c(h) with the MvNormal
+    prior (seqsim.jl:287) and draws (seqsim.jl:335-340: nv normals per node).
+    Simple cokriging with A * c(h) and every variable at every neighbour has the
+    weights lambda_j * I of c(h)'s system and error covariance A sigma^2, so a
+    node draws mu_SK + sigma L z, L = chol(A): the normals are mixed per node,
+    w_v = sum_{u<=v} L_vu z_u (ascending u), then every variable is the scalar
+    sweep of the plan of c(h) with its own mean and data values.
+    z: (R, N, nv); data_vals: (nv, n_data). Returns (nv, R, N)."""
+    z = np.asarray(z, dtype=np.float64)
+    R, N, nv = z.shape
+    L = mv_factor(A)
+    out = []
+    for v in range(nv):
+        w = np.zeros((R, N))
+        for u in range(v + 1):
+            w = w + L[v, u] * z[:, :, u]
+        cv = dataclasses.replace(case, mean=float(means[v]),
+                                 data_vals=np.asarray(data_vals[v], dtype=np.float64)
+                                 if case.n_data else case.data_vals)
+        out.append(seqsim_sweep(cv, plan, w))
+    return np.stack(out)
+
+
 def draw_category(q, u):
     """Distributions 0.25 `rand(rng, Categorical(q))` (DiscreteNonParametric,
     not vendored): cp = q[0]; i = 0; while cp <= u and i < n - 1: i += 1,

@@ -613,3 +613,48 @@ def test_indicator_sweep_prior_only_follows_probabilities():
     out = O.indicator_sweep(case, plan, u, [0.6, 0.3, 0.1], [])
     freq = np.bincount(out.ravel(), minlength=3) / out.size
     np.testing.assert_allclose(freq, [0.6, 0.3, 0.1], atol=0.02)
+
+
+# ------------------------------------------------- multivariate SEQSIM ----
+def test_mv_factor_is_cholesky():
+    A = np.array([[2.0, 0.9, -0.3], [0.9, 1.5, 0.4], [-0.3, 0.4, 1.1]])
+    L = O.mv_factor(A)
+    np.testing.assert_allclose(L, np.linalg.cholesky(A), rtol=1e-15, atol=1e-15)
+    assert np.all(np.triu(L, 1) == 0)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.mv_factor(np.array([[1.0, 2.0], [2.0, 1.0]]))
+
+
+def test_multivariate_sweep_one_variable_is_the_scalar_sweep():
+    rng = np.random.default_rng(31)
+    case = O.SeqsimCase(dims=(14, 11), family=O.SPHERICAL, ranges=(6.0,), kmax=8, radius=7.0,
+                        data_coords=rng.uniform(0, 11, (9, 2)), data_vals=rng.standard_normal(9),
+                        mean=0.4, path=rng.permutation(154))
+    plan = O.seqsim_plan(case)
+    z = rng.standard_normal((3, 154))
+    mv = O.seqsim_sweep_multivariate(case, plan, z[:, :, None], [[1.0]], [0.4],
+                                     case.data_vals[None, :])
+    np.testing.assert_array_equal(mv[0], O.seqsim_sweep(case, plan, z))
+
+
+def test_multivariate_sweep_covariance_is_A_times_scalar():
+    """Over many realizations the two fields of A * c(h) (A = [1 .8; .8 1])
+    have, at every pair of nodes, cross-covariance A12 / A11 times the
+    auto-covariance, the MvNormal semantics with covariance A sigma^2 (the
+    transposed factor would give 0.62 instead of 0.8)."""
+    rng = np.random.default_rng(32)
+    case = O.SeqsimCase(dims=(9, 8), family=O.EXPONENTIAL, ranges=(4.0,), kmax=6, radius=0.0,
+                        path=rng.permutation(72))
+    plan = O.seqsim_plan(case)
+    A = np.array([[1.0, 0.8], [0.8, 1.0]])
+    R = 6000
+    z = rng.standard_normal((R, 72, 2))
+    x = O.seqsim_sweep_multivariate(case, plan, z, A, [0.0, 0.0], np.zeros((2, 0)))
+    c11 = x[0].T @ x[0] / R
+    c12 = x[0].T @ x[1] / R
+    c22 = x[1].T @ x[1] / R
+    # lag 0 correlation at every node and the ratio of cross- to auto-covariance
+    corr = np.diag(c12) / np.sqrt(np.diag(c11) * np.diag(c22))
+    assert abs(corr.mean() - 0.8) < 0.01
+    big = np.abs(c11) > 0.3
+    assert abs(np.median(c12[big] / c11[big]) - 0.8) < 0.02
