@@ -13,9 +13,10 @@ from ._lib import (ArgumentError, Context, GspError, PosDefException, Unsupporte
                    default_context)
 from .ensembles import Ensemble
 from .expectation import KrigingPlan, expectedvalue, mean, posteriormean, priormean
-from .functions import (CoregionalizedFunction, ExponentialCovariance, ExponentialVariogram,
-                        GaussianCovariance, GaussianVariogram, SphericalCovariance,
-                        SphericalVariogram)
+from .functions import (CoregionalizedFunction, ExponentialCovariance, ExponentialTransiogram,
+                        ExponentialVariogram, GaussianCovariance, GaussianTransiogram,
+                        GaussianVariogram, SphericalCovariance, SphericalTransiogram,
+                        SphericalVariogram, Transiogram)
 from .geometry import CartesianGrid, GeoTable, PointSet, georef
 from .initialization import ExplicitInit, NearestInit, initialize
 from .processes import (FixedPath, GaussianProcess, IndicatorProcess, LinearPath, RandomPath,
@@ -35,7 +36,8 @@ __all__ = [
     "ArgumentError", "Context", "GspError", "PosDefException", "UnsupportedError",
     "default_context", "Ensemble", "ExponentialCovariance", "ExponentialVariogram",
     "GaussianCovariance", "GaussianVariogram", "SphericalCovariance", "SphericalVariogram",
-    "CoregionalizedFunction", "IndicatorProcess",
+    "CoregionalizedFunction", "IndicatorProcess", "Transiogram", "SphericalTransiogram",
+    "ExponentialTransiogram", "GaussianTransiogram",
     "CartesianGrid", "GeoTable", "PointSet", "georef", "ExplicitInit", "NearestInit",
     "initialize", "FixedPath", "GaussianProcess", "LinearPath", "RandomPath", "SourcePath",
     "FFTSIM", "LUSIM", "SEQSIM", "MetricBall", "defaultsimulation", "preprocess", "rand",

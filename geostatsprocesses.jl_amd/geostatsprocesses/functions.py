@@ -177,3 +177,78 @@ class ExponentialCovariance(_Exponential, GeoStatsFunction):
 
 class GaussianCovariance(_Gaussian, GeoStatsFunction):
     kind = _lib.COVARIANCE
+
+
+class Transiogram:
+    """Shape transiograms of GeoStatsFunctions 0.15 (un-vendored; the form is
+    not fixed by any reference test): transition probabilities
+    T_ij(h) = (1 - v(h)) delta_ij + v(h) p_j, where v is the family's unit-sill
+    variogram shape at the normalised lag (0 at h = 0, 1 at and beyond the
+    range) and p the proportions. The indicator cross-covariances are then
+    p_i (T_ij(h) - p_j) = c(h) (diag(p) - p p^T)_ij with c = 1 - v: an intrinsic
+    coregionalization of the unit-sill covariance c of the same family and
+    range, which is how SEQSIM simulates an IndicatorProcess of a transiogram
+    here (its simple-kriging weights are those of c)."""
+    covariance_type = None
+    __array_ufunc__ = None
+
+    def __init__(self, range=1.0, proportions=(0.5, 0.5), ranges=None, rotation=None):
+        self._ranges = tuple(float(r) for r in ranges) if ranges is not None else (float(range),)
+        prop = np.asarray(proportions, dtype=np.float64)
+        if prop.ndim != 1 or len(prop) < 2 or np.any(prop < 0):
+            raise ValueError("proportions must be two or more non-negative values")
+        self._prop = prop / prop.sum()
+        self.rotation = rotation
+
+    def proportions(self):
+        return self._prop.copy()
+
+    def nvariables(self):
+        return len(self._prop)
+
+    def isstationary(self):
+        return True
+
+    def issymmetric(self):
+        return False
+
+    def isbanded(self):
+        return False
+
+    def range(self):
+        return max(self._ranges)
+
+    def ranges(self, ndim=None):
+        return self.base_covariance().ranges(ndim)
+
+    def base_covariance(self):
+        """The unit-sill covariance c = 1 - v of the same family and range."""
+        return self.covariance_type(sill=1.0, nugget=0.0, ranges=self._ranges,
+                                    rotation=self.rotation)
+
+    def coregionalization(self):
+        """diag(p) - p p^T: the indicator covariances are c(h) times this."""
+        p = self._prop
+        return np.diag(p) - np.outer(p, p)
+
+    def transition(self, h):
+        """T(h) for a scalar lag (in the units of the range)."""
+        v = 1.0 - float(self.base_covariance().cov(np.asarray([h], dtype=np.float64))[0])
+        p = self._prop
+        return (1.0 - v) * np.eye(len(p)) + v * np.broadcast_to(p, (len(p), len(p)))
+
+    def __repr__(self):
+        return (f"{type(self).__name__}(ranges={self._ranges}, "
+                f"proportions={tuple(float(x) for x in self._prop)})")
+
+
+class SphericalTransiogram(Transiogram):
+    covariance_type = SphericalCovariance
+
+
+class ExponentialTransiogram(Transiogram):
+    covariance_type = ExponentialCovariance
+
+
+class GaussianTransiogram(Transiogram):
+    covariance_type = GaussianCovariance

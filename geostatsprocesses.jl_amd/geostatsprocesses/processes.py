@@ -33,11 +33,18 @@ class GaussianProcess(FieldProcess):
 
 
 class IndicatorProcess(FieldProcess):
-    """IndicatorProcess(func, prob) (src/processes/field/indicator.jl:29-47):
+    """IndicatorProcess(func, prob) or IndicatorProcess(transiogram)
+    (src/processes/field/indicator.jl:29-47):
     categorical field with multivariate function `func` (one variable per
     category) and a priori probabilities `prob` (rescaled to sum one)."""
 
-    def __init__(self, func, prob):
+    def __init__(self, func, prob=None):
+        if prob is None:
+            # IndicatorProcess(transiogram) takes the transiogram's proportions
+            # (indicator.jl:46)
+            if not hasattr(func, "proportions"):
+                raise TypeError("IndicatorProcess needs probabilities or a transiogram")
+            prob = func.proportions()
         prob = np.asarray(prob, dtype=np.float64)
         prob = prob / prob.sum()
         nf = func.nvariables()

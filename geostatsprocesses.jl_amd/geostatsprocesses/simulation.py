@@ -20,7 +20,7 @@ from . import _lib
 from .ensembles import Ensemble
 from .geometry import CartesianGrid, GeoTable, PointSet
 from .initialization import ExplicitInit, NearestInit, initialize
-from .functions import CoregionalizedFunction
+from .functions import CoregionalizedFunction, Transiogram
 from .processes import GaussianProcess, IndicatorProcess, LinearPath, SourcePath
 
 
@@ -214,10 +214,14 @@ def _indicator_problem(process, method, domain, data, ctx):
     times the identity, lambda the simple-kriging weights of c(h): the plan is
     the Gaussian one built with c(h)."""
     f = process.func
-    if not isinstance(f, CoregionalizedFunction):
+    if isinstance(f, Transiogram):
+        base = f.base_covariance()  # shape transiogram: A * c(h), A = diag(p) - p p^T
+    elif isinstance(f, CoregionalizedFunction):
+        base = f.base
+    else:
         raise _lib.UnsupportedError(
             "this backend simulates indicator processes of a coregionalized covariance "
-            "A * c(h); transiograms are not supported")
+            "A * c(h) or of a shape transiogram")
     if data is not None:
         names = data.names()
         if len(names) != 1:
@@ -232,7 +236,7 @@ def _indicator_problem(process, method, domain, data, ctx):
         labels = np.arange(1, f.nvariables() + 1)
         data_cat = np.zeros(0, np.int64)
     data_cat = np.ascontiguousarray(data_cat, dtype=np.int32)
-    pre = _seqsim_problem(process, method, domain, data, ctx, func=f.base,
+    pre = _seqsim_problem(process, method, domain, data, ctx, func=base,
                           data_vals=data_cat.astype(np.float64) if data is not None else None)
     return IndicatorPreproc(pre.plan, names, labels, data_cat, process.prob)
 

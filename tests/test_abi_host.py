@@ -163,3 +163,31 @@ def test_ensemble_container():
     assert np.array_equal(e[1].field, vals[1])
     assert [np.array_equal(t.field, vals[i]) for i, t in enumerate(e)] == [True] * 3
     assert repr(e) == "1D Ensemble"
+
+
+@pytest.mark.parametrize("cls,fam", [(gsp.SphericalTransiogram, 0), (gsp.ExponentialTransiogram, 1),
+                                     (gsp.GaussianTransiogram, 2)])
+def test_transiogram_is_a_coregionalization(cls, fam):
+    """Shape transiograms: T(0) = I, rows of T(h) sum to one, T(h) = 1 p^T at
+    and beyond the range (spherical) or tends to it, and the indicator
+    cross-covariances p_i (T_ij(h) - p_j) equal c(h) (diag(p) - p p^T) with c
+    the oracle's unit-sill covariance of the family (1 - v(h)): the reduction
+    that lets SEQSIM simulate an IndicatorProcess of a transiogram with the
+    simple-kriging weights of c."""
+    import oracle as O
+    p = np.array([0.7, 0.2, 0.1])
+    t = cls(range=35.0, proportions=tuple(p))
+    np.testing.assert_array_equal(t.transition(0.0), np.eye(3))
+    A = np.diag(p) - np.outer(p, p)
+    for h in (0.5, 3.0, 17.5, 34.0, 60.0, 200.0):
+        T = t.transition(h)
+        np.testing.assert_allclose(T.sum(axis=1), 1.0, rtol=0, atol=1e-15)
+        c = O.cov_u(fam, 1.0, 0.0, h / 35.0)
+        np.testing.assert_allclose(p[:, None] * (T - p[None, :]), c * A, rtol=1e-13, atol=1e-16)
+    if fam == 0:
+        np.testing.assert_allclose(t.transition(35.0), np.broadcast_to(p, (3, 3)), atol=1e-15)
+    proc = gsp.IndicatorProcess(t)
+    np.testing.assert_allclose(proc.prob, p)
+    assert proc.func.nvariables() == 3
+    # the default IndicatorProcess(transiogram) proportions are two equal ones
+    assert np.allclose(gsp.IndicatorProcess(cls(range=10.0)).prob, [0.5, 0.5])

@@ -2,8 +2,8 @@
 coregionalized covariance A * c(h): realizations equal the oracle's indicator
 sweep category for category (the same plan, the same operation order per node,
 the same uniforms), and the reference's IndicatorProcess testset
-(test/field.jl:181-204) through `rand`, with A * SphericalCovariance in place
-of the transiograms this backend does not provide."""
+(test/field.jl:181-204) through `rand`, with its shape transiograms and with
+A * SphericalCovariance."""
 import numpy as np
 import pytest
 
@@ -44,6 +44,34 @@ def test_indicator_sweep_matches_oracle(ctx, spec):
     np.testing.assert_array_equal(got, ref)
     say("category frequencies", np.bincount(got.ravel(), minlength=len(prob)) / got.size,
         "prior", np.round(prob, 3))
+
+
+def test_reference_indicator_testset_transiograms(ctx):
+    """test/field.jl:181-204 as written: IndicatorProcess(SphericalTransiogram(
+    range=35)) with two categories and with proportions (0.7, 0.2, 0.1), on a
+    100 x 100 grid, unconditional and with data (integer and string labels)."""
+    rng = np.random.default_rng(123)
+    proc = gsp.IndicatorProcess(gsp.SphericalTransiogram(range=35.0))
+    grid = gsp.CartesianGrid(100, 100)
+    data = gsp.georef({"C": [1, 3, 1]}, [(25.0, 25.0), (50.0, 75.0), (75.0, 50.0)])
+    real = gsp.rand(proc, grid, rng=rng, ctx=ctx)
+    assert np.issubdtype(real["field"].dtype, np.integer)
+    assert set(real["field"].tolist()) == {1, 2}
+    real = gsp.rand(proc, grid, rng=rng, data=data, ctx=ctx)
+    assert np.issubdtype(real["C"].dtype, np.integer)
+    assert set(real["C"].tolist()) == {1, 3}
+    rng = np.random.default_rng(123)
+    proc = gsp.IndicatorProcess(gsp.SphericalTransiogram(range=35.0, proportions=(0.7, 0.2, 0.1)))
+    real = gsp.rand(proc, grid, rng=rng, ctx=ctx)
+    assert set(real["field"].tolist()) == {1, 2, 3}
+    data = gsp.georef({"C": ["a", "b", "c"]}, [(25.0, 25.0), (50.0, 75.0), (75.0, 50.0)])
+    real = gsp.rand(proc, grid, rng=rng, data=data, ctx=ctx)
+    assert set(real["C"].tolist()) == {"a", "b", "c"}
+    # the category frequencies follow the proportions
+    ens = gsp.rand(proc, grid, 4, rng=rng, ctx=ctx)
+    freq = np.bincount(np.concatenate([ens[i]["field"] for i in range(4)]) - 1) / 40000
+    say("transiogram category frequencies", np.round(freq, 3))
+    assert freq[0] > freq[2]
 
 
 def test_reference_indicator_testset(ctx):
