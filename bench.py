@@ -282,6 +282,17 @@ def cpu_baseline(reqs, R_batch=None, node_budget=None, node_limit=0):
 
 MLP_FLOPS = 2 * (73 * 256 + 2 * 256 * 256 + 256 * 25)   # per node, one forward pass
 BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def krige_flops(nfound, flags):
+    """Algorithmic flops of the exact kriging solves of a plan: per solved
+    system of n neighbours n^3/3 + 2n^2 + 2n (factor and two substitutions)
+    + 10 (n(n+1)/2 + n) (covariance entries); prior draws (flags & 3) and
+    learned nodes (flags & 8) solve nothing."""
+    n = np.where((flags & 11) != 0, 0, nfound).astype(np.float64)
+    f = n ** 3 / 3 + 2 * n * n + 2 * n + 10 * (n * (n + 1) / 2 + n)
+    return float(f.sum())
 
 
 def upload_normals(ctx, _lib, R, N, seed):
@@ -310,7 +321,7 @@ def timed(ctx, step, steps, warmup):
     return (t1 - t0) * 1e3 / steps, tim
 
 
-def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0):
+def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0, kflops=None):
     """Average launch time per kernel family and its algorithmic bytes (flops for
     the surrogate) per launch."""
     sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
@@ -329,16 +340,27 @@ def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0):
             per_launch = nbytes / (launches / steps)
             kern[name]["algorithmic_bytes_per_launch"] = int(per_launch)
             kern[name]["achieved_GBs"] = round(per_launch / (avg * 1e-3) / 1e9, 1)
+        if name == "krige" and kflops is not None:
+            per_launch = kflops / (launches / steps)
+            kern[name]["algorithmic_flops_per_launch"] = int(per_launch)
+            kern[name]["achieved_TFLOPs"] = round(per_launch / (avg * 1e-3) / 1e12, 3)
         if name == "learned":
             kern[name]["algorithmic_flops_per_launch"] = int(MLP_FLOPS * n_mlp)
             kern[name]["achieved_TFLOPs"] = round(MLP_FLOPS * n_mlp / (avg * 1e-3) / 1e12, 2)
     return kern
 
 
-def roofline_of(kern, workload):
+def roofline_of(kern, workload, fp64=False):
     dominant = max(("search", "krige", "learned", "sweep"),
                    key=lambda k: kern.get(k, {}).get("avg_ms", 0) * kern.get(k, {}).get("launches_per_step", 0))
     d = kern[dominant]
+    if dominant == "krige" and "achieved_TFLOPs" in d:
+        # compute-bound (vector ALU): against the peak of the factor's type
+        peak = 78.6 if fp64 else FP32_PEAK_TFLOPS
+        return {"bound": "mfma", "kernel": "krige (k_krige16 + k_krige/k_krige32, vector ALU)",
+                "achieved": d["achieved_TFLOPs"], "peak": peak, "unit": "TFLOP/s",
+                "frac": round(d["achieved_TFLOPs"] / peak, 4),
+                "traffic": profiled_traffic(workload, dominant)}
     if dominant == "learned":
         return {"bound": "mfma", "kernel": dominant, "achieved": d["achieved_TFLOPs"],
                 "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -627,11 +649,12 @@ def bench_indicator(args, spec):
     step = Step()
     ex = step.plan.export(fields=("nfound", "flags"))
     kbar = float(np.where(ex["flags"] & 3, 0, ex["nfound"]).mean())
+    kfl = krige_flops(ex["nfound"], ex["flags"])
     K = step.plan.info.maxneighbors
     depth = step.plan.info.depth
     step.close()
     ms, tim = timed(ctx, Step, args.steps, args.warmup)
-    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar)
+    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, kflops=kfl)
     line = {
         "metric": "indicator SEQSIM nodes*realizations/s",
         "value": round(N * R / (ms / 1e3), 1),
@@ -643,7 +666,8 @@ def bench_indicator(args, spec):
         "config": {"workload": args.workload, "grid": list(spec["dims"]), "realizations": R,
                    "categories": len(prob), "maxneighbors": spec["kmax"], "radius": spec["radius"],
                    "n_data": spec["n_data"], "depth": depth, "mean_neighbours": round(kbar, 2)},
-        "roofline": roofline_of(kern, args.workload), "kernels": kern, "cpu_baseline": None,
+        "roofline": roofline_of(kern, args.workload, precision == _lib.FP64), "kernels": kern,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 
@@ -901,13 +925,15 @@ def main():
     plan.close()
     keff = np.where(ex["flags"] & 3, 0, ex["nfound"]).astype(np.float64)
     kbar = float(keff.mean())
+    kfl = krige_flops(ex["nfound"], ex["flags"])
+    kfl_exact = krige_flops(ex["nfound"], ex["flags"] & 7)   # every system solved exactly
     del ex
 
     ms_per_step, tim = timed(ctx, step, args.steps, args.warmup)
     value = N * R / (ms_per_step / 1e3)
     K = info.maxneighbors                                        # row stride of nbr / lambda
-    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned)
-    roofline = roofline_of(kern, args.workload)
+    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned, kflops=kfl)
+    roofline = roofline_of(kern, args.workload, precision == _lib.FP64)
 
     # repeated sweeps of one plan (a serving loop): captured HIP graph vs one
     # launch per level
@@ -942,11 +968,13 @@ def main():
             return step_exact
         ems, etim = timed(ctx, exact_step(plan_arg), args.steps, args.warmup)
         exact = {"value": round(N * R / (ems / 1e3), 1), "ms_per_step": round(ems, 3),
-                 "precision": "fp32r", "kernels": kernel_table(_lib, etim, args.steps, N, R, K, kbar)}
+                 "precision": "fp32r",
+                 "kernels": kernel_table(_lib, etim, args.steps, N, R, K, kbar, kflops=kfl_exact)}
         ems64, etim64 = timed(ctx, exact_step(problems(_lib.FP64)), args.steps, args.warmup)
         exact64 = {"value": round(N * R / (ems64 / 1e3), 1), "ms_per_step": round(ems64, 3),
                    "precision": "fp64",
-                   "kernels": kernel_table(_lib, etim64, args.steps, N, R, K, kbar)}
+                   "kernels": kernel_table(_lib, etim64, args.steps, N, R, K, kbar,
+                                           kflops=kfl_exact)}
         if not args.no_accuracy:
             masked = problems(_lib.FP32_REFINE)
             masked.search_mode = _lib.MASKED_KNN
