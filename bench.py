@@ -15,7 +15,7 @@ Workload "seqsim-learned-256x256x128" (default): a 256 x 256 x 128 unit grid
 (8,388,608 nodes), exponential variogram sill 1, range 20, nugget 0.01,
 maxneighbors 24 with the :range neighbourhood, no conditioning data, a random
 path, 16 realizations. The kriging weights and variances come from the learned
-surrogate (BF16 MLP 73-256-256-256-25 on the matrix cores) wherever it applies
+surrogate (BF16 MLP 97-256-256-256-33 on the matrix cores) wherever it applies
 and its variance check passes, from exact FP32 + refinement solves elsewhere.
 The surrogate is trained before timing on 2M kriging systems of exact plans of
 the same variogram (three 128 x 128 x 64 grids). One step is one learned plan
@@ -31,6 +31,14 @@ in [5, 60] cells and nugget uniform in [0, 0.1], 0-500 uniform data each,
 maxneighbors from {8, 16, 32}, :range neighbourhood, random paths, 8
 realizations each, FP32 solves with refinement. One step is one batched plan
 plus the batched sweep.
+
+Workload "seqsim-batch-512-learned": the same 512 requests with learned
+weights: one surrogate per variogram family, trained before timing on the
+exact plans of 3 x 48 other requests of the same generator; each request
+inside its family surrogate's trained spans of range and nugget takes learned
+weights where the variance check passes, the rest are solved exactly. The
+exact batch is timed beside it; the accuracy (RMS weight error against exact
+FP64, paired variograms) is measured on 32 of the requests.
 
 Workload "kriging-mean-256x256x128": the Gaussian posterior mean (simple
 kriging, `mean(process, grid; data)`) on a 256 x 256 x 128 unit grid
@@ -94,6 +102,10 @@ WORKLOADS = {
     "seqsim-2d-100x100": dict(dims=(100, 100), family=0, sill=1.0, ranges=(10.0, 10.0), nugget=0.0,
                               n_data=50, kmax=12, radius=30.0, R=10, seed=1001, precision="fp64"),
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
+    "seqsim-batch-512-learned": dict(
+        batch=512, side=(32, 512), R=8, seed=1005,
+        learned=dict(train_requests=48, train_seed=3005, samples=2_000_000, epochs=12, batch=1024,
+                     lr=2e-3, tolerance=0.05, accuracy_requests=32)),
     "seqsim-learned-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,
         kmax=24, radius=20.0, R=16, seed=1004,
@@ -280,7 +292,10 @@ def cpu_baseline(reqs, R_batch=None, node_budget=None, node_limit=0):
             "cpu_model": model, "cores_visible": ncores}
 
 
-MLP_FLOPS = 2 * (73 * 256 + 2 * 256 * 256 + 256 * 25)   # per node, one forward pass
+def mlp_flops(slots):
+    """Flops of one forward pass of the S-slot surrogate (3S+1 -> 256 -> 256 ->
+    256 -> S+1): 312,320 at S = 24, 328,704 at S = 32."""
+    return 2 * ((3 * slots + 1) * 256 + 2 * 256 * 256 + 256 * (slots + 1))
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -321,7 +336,7 @@ def timed(ctx, step, steps, warmup):
     return (t1 - t0) * 1e3 / steps, tim
 
 
-def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0, kflops=None):
+def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0, kflops=None, mlp_fl=0):
     """Average launch time per kernel family and its algorithmic bytes (flops for
     the surrogate) per launch."""
     sweep_bytes = N * R * (8 + 8) + N * (kbar * (4 + 8) + 8)     # z read, x write, plan read
@@ -345,8 +360,8 @@ def kernel_table(_lib, tim, steps, N, R, K, kbar, n_mlp=0, kflops=None):
             kern[name]["algorithmic_flops_per_launch"] = int(per_launch)
             kern[name]["achieved_TFLOPs"] = round(per_launch / (avg * 1e-3) / 1e12, 3)
         if name == "learned":
-            kern[name]["algorithmic_flops_per_launch"] = int(MLP_FLOPS * n_mlp)
-            kern[name]["achieved_TFLOPs"] = round(MLP_FLOPS * n_mlp / (avg * 1e-3) / 1e12, 2)
+            kern[name]["algorithmic_flops_per_launch"] = int(mlp_fl * n_mlp)
+            kern[name]["achieved_TFLOPs"] = round(mlp_fl * n_mlp / (avg * 1e-3) / 1e12, 2)
     return kern
 
 
@@ -437,6 +452,93 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol
             "exact_variogram_error_vs_model_per_axis": [round(e, 5) for e in exact_vs_model],
             "masked_knn_exact_variogram_error_vs_model_per_axis": [round(e, 5) for e in masked_vs_model],
             "variogram_sample": f"{Rv} realizations, lags 1..20 cells along each axis"}
+
+
+def train_batch_surrogates(ctx, _lib, spec, precision):
+    """One surrogate per variogram family for a mixed batch: exact plans of
+    `train_requests` other requests drawn like the batch's (another seed), the
+    requests of each family in one batch plan, then device training on up to
+    `samples` of that family's kriging systems. Each surrogate then serves the
+    batch requests of its family inside the ranges and nuggets it saw."""
+    L = spec["learned"]
+    treqs = batch_specs(dict(spec, batch=L["train_requests"] * 3, seed=L["train_seed"]))
+    surs, reports = [], []
+    for fam in (0, 1, 2):
+        fr = [r for r in treqs if r["family"] == fam]
+        sizes = [int(np.prod(r["dims"])) for r in fr]
+        dpath = _lib.DeviceBuffer.from_array(ctx, np.concatenate([r["path"] for r in fr]))
+        ddata = _lib.DeviceBuffer.from_array(ctx, np.concatenate([r["data"].ravel() for r in fr]
+                                                                 + [np.zeros(1)]))
+        probs, noff, coff = [], 0, 0
+        for r, n in zip(fr, sizes):
+            probs.append(problem(r, dpath.ptr + 8 * noff, ddata.ptr + 8 * coff if r["n_data"] else None,
+                                 precision, n))
+            noff += n
+            coff += r["data"].size
+        plan = _lib.SeqsimPlan(ctx, probs)
+        S = _lib.Surrogate(ctx, [plan], max_samples=L["samples"], epochs=L["epochs"],
+                           batch=L["batch"], lr=L["lr"], seed=L.get("seed", 0) + fam)
+        plan.close()
+        dpath.free()
+        ddata.free()
+        r = S.report
+        info = S.info
+        surs.append(S)
+        reports.append({"family": ["spherical", "exponential", "gaussian"][fam], "requests": len(fr),
+                        "systems": int(r.n_samples), "candidates": int(r.n_candidates),
+                        "steps": int(r.steps), "ms": round(r.ms, 1),
+                        "rms_weight_last_epoch": r.rms_weight_last_epoch,
+                        "nugget_ratio_span": [round(x, 4) for x in info["nugget_ratio"]],
+                        "range_span_cells": [round(1 / info["scale"][0][1], 2),
+                                             round(1 / info["scale"][0][0], 2)]})
+    return surs, {"per_family": reports, "epochs": L["epochs"], "batch": L["batch"],
+                  "ms": round(sum(x["ms"] for x in reports), 1),
+                  "source": f"{L['train_requests']} x 3 other requests of the same generator "
+                            f"(seed {L['train_seed']}), exact plans"}
+
+
+def batch_learned_accuracy(ctx, _lib, reqs, probs32, probs64, S, tol, R, n_req):
+    """Learned against exact FP64 on a sample of the batch's requests (every
+    (512 / n_req)-th): RMS weight error of learned nodes, sigma error, and the
+    paired experimental variograms (same normals) along both axes at lags
+    1..10, per request, aggregated as the RMS of the relative errors."""
+    pick = list(range(0, len(reqs), max(1, len(reqs) // n_req)))[:n_req]
+    p32 = [probs32[q] for q in pick]
+    p64 = [probs64[q] for q in pick]
+    lp = _lib.SeqsimPlan(ctx, p32, surrogate=S, tolerance=tol)
+    ep = _lib.SeqsimPlan(ctx, p64)
+    gl = lp.export(fields=("nbr", "lam", "sigma", "flags"))
+    ge = ep.export(fields=("lam", "sigma"))
+    learned = (gl["flags"] & _lib.FLAG_LEARNED) != 0
+    valid = gl["nbr"] >= 0
+    d = gl["lam"] - ge["lam"]
+    rms = float(np.sqrt(np.mean(d[learned][valid[learned]] ** 2))) if learned.any() else None
+    sig = np.abs(gl["sigma"] - ge["sigma"])[learned] / np.maximum(ge["sigma"][learned], 1e-300)
+    no = lp.node_offset
+    NT = int(no[-1])
+    rng = np.random.default_rng(77)
+    Rv = min(R, 4)
+    z = rng.standard_normal((Rv, NT))
+    vals = np.concatenate([reqs[q]["vals"] for q in pick] + [np.zeros(0)])
+    xl = lp.sweep(z, vals)
+    xe = ep.sweep(z, vals)
+    errs = []
+    for i, q in enumerate(pick):
+        dims = reqs[q]["dims"]
+        a, b = int(no[i]), int(no[i + 1])
+        lags = list(range(1, 11))
+        for axis in range(2):
+            gl_ = axis_variogram(xl[:, a:b], dims, axis, lags)
+            ge_ = axis_variogram(xe[:, a:b], dims, axis, lags)
+            errs.append(float(np.linalg.norm(gl_ - ge_) / np.linalg.norm(ge_)))
+    lp.close()
+    ep.close()
+    return {"requests_sampled": len(pick), "learned_fraction_sampled": float(learned.mean()),
+            "rms_weight_error_learned_nodes": rms,
+            "sigma_rel_error_median": float(np.median(sig)) if learned.any() else None,
+            "variogram_error_paired_rms": float(np.sqrt(np.mean(np.square(errs)))),
+            "variogram_error_paired_max": float(np.max(errs)),
+            "variogram_sample": f"{Rv} realizations per request, lags 1..10 cells along each axis"}
 
 
 def train_surrogate(ctx, _lib, spec, precision):
@@ -909,7 +1011,10 @@ def main():
     if "learned" in spec:
         if args.epochs is not None:
             spec = dict(spec, learned=dict(spec["learned"], epochs=args.epochs))
-        S, training = train_surrogate(ctx, _lib, spec, precision)
+        if "batch" in spec:
+            S, training = train_batch_surrogates(ctx, _lib, spec, precision)
+        else:
+            S, training = train_surrogate(ctx, _lib, spec, precision)
         tol = spec["learned"]["tolerance"]
 
     def step():
@@ -932,7 +1037,9 @@ def main():
     ms_per_step, tim = timed(ctx, step, args.steps, args.warmup)
     value = N * R / (ms_per_step / 1e3)
     K = info.maxneighbors                                        # row stride of nbr / lambda
-    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned, kflops=kfl)
+    slots = (S[0] if isinstance(S, list) else S).info["n_slots"] if S is not None else 0
+    kern = kernel_table(_lib, tim, args.steps, N, R, K, kbar, n_mlp=info.n_learned, kflops=kfl,
+                        mlp_fl=mlp_flops(slots) if slots else 0)
     roofline = roofline_of(kern, args.workload, precision == _lib.FP64)
 
     # repeated sweeps of one plan (a serving loop): captured HIP graph vs one
@@ -975,7 +1082,12 @@ def main():
                    "precision": "fp64",
                    "kernels": kernel_table(_lib, etim64, args.steps, N, R, K, kbar,
                                            kflops=kfl_exact)}
-        if not args.no_accuracy:
+        if not args.no_accuracy and "batch" in spec:
+            accuracy = batch_learned_accuracy(ctx, _lib, reqs, problems(precision),
+                                              problems(_lib.FP64), S, tol, R,
+                                              spec["learned"]["accuracy_requests"])
+            accuracy["tolerance"] = tol
+        elif not args.no_accuracy:
             masked = problems(_lib.FP32_REFINE)
             masked.search_mode = _lib.MASKED_KNN
             accuracy = learned_accuracy(ctx, _lib, spec, plan_arg, problems(_lib.FP64), masked, S,
@@ -1035,6 +1147,7 @@ def main():
         line["speedup_vs_exact"] = round(value / exact["value"], 3)
         line["speedup_vs_exact_fp64"] = round(value / exact64["value"], 3)
         line["learned_nodes"] = int(info.n_learned)
+        line["learned_fraction"] = round(info.n_learned / N, 4)
         line["training"] = training
         line["accuracy"] = accuracy
     print(json.dumps(line), flush=True)

@@ -270,7 +270,7 @@ int32_t gsp_seqsim_plan_create_batch(gsp_ctx* ctx, int32_t n, const gsp_seqsim_p
   }
   int32_t rc = gsp_guard(ctx, [&] {
     plan->pool = ctx->pool;
-    gsp::seqsim_plan_build(ctx, n, problems, plan, nullptr, 0.0);
+    gsp::seqsim_plan_build(ctx, n, problems, plan, nullptr, 0, 0.0);
   });
   if (rc != GSP_OK) {
     delete plan;
@@ -301,10 +301,23 @@ int32_t gsp_seqsim_plan_info(const gsp_seqsim_plan* plan, gsp_seqsim_info* info)
 int32_t gsp_seqsim_plan_create_learned(gsp_ctx* ctx, int32_t n, const gsp_seqsim_problem* problems,
                                        const gsp_surrogate* surrogate, double tolerance,
                                        gsp_seqsim_plan** out) {
-  if (!out || !surrogate) {
-    gsp::set_error("null surrogate or output pointer");
+  return gsp_seqsim_plan_create_learned_multi(ctx, n, problems, 1, &surrogate, tolerance, out);
+}
+
+int32_t gsp_seqsim_plan_create_learned_multi(gsp_ctx* ctx, int32_t n,
+                                             const gsp_seqsim_problem* problems,
+                                             int32_t n_surrogates,
+                                             const gsp_surrogate* const* surrogates,
+                                             double tolerance, gsp_seqsim_plan** out) {
+  if (!out || !surrogates || n_surrogates < 1 || n_surrogates > 64) {
+    gsp::set_error("null output pointer, or not 1..64 surrogates");
     return GSP_ERR_ARGUMENT;
   }
+  for (int32_t i = 0; i < n_surrogates; ++i)
+    if (!surrogates[i]) {
+      gsp::set_error("null surrogate %d", i);
+      return GSP_ERR_ARGUMENT;
+    }
   if (!(tolerance >= 0)) {
     gsp::set_error("tolerance must be non-negative");
     return GSP_ERR_ARGUMENT;
@@ -317,7 +330,7 @@ int32_t gsp_seqsim_plan_create_learned(gsp_ctx* ctx, int32_t n, const gsp_seqsim
   }
   int32_t rc = gsp_guard(ctx, [&] {
     plan->pool = ctx->pool;
-    gsp::seqsim_plan_build(ctx, n, problems, plan, surrogate, tolerance);
+    gsp::seqsim_plan_build(ctx, n, problems, plan, surrogates, n_surrogates, tolerance);
   });
   if (rc != GSP_OK) {
     delete plan;
@@ -360,6 +373,15 @@ int32_t gsp_surrogate_export(gsp_ctx* ctx, const gsp_surrogate* surrogate, float
     return GSP_ERR_ARGUMENT;
   }
   return gsp_guard(ctx, [&] { gsp::surrogate_export(ctx, surrogate, params, count, slots); });
+}
+
+int32_t gsp_surrogate_get_info(const gsp_surrogate* surrogate, gsp_surrogate_info* info) {
+  if (!surrogate || !info) {
+    gsp::set_error("null surrogate or info pointer");
+    return GSP_ERR_ARGUMENT;
+  }
+  gsp::surrogate_info(surrogate, info);
+  return GSP_OK;
 }
 
 int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,

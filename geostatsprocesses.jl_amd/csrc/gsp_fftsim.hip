@@ -370,7 +370,7 @@ void fftsim_build(gsp_ctx* ctx, const gsp_fftsim_problem* p, gsp_fftsim_plan* P)
     gsp_seqsim_problem q1 = kriging_problem(p, p->n_data, coords.data(), path.data(), N);
     std::unique_ptr<gsp_seqsim_plan> plan1(new gsp_seqsim_plan());
     plan1->pool = ctx->pool;
-    seqsim_plan_build(ctx, 1, &q1, plan1.get(), nullptr, 0.0);
+    seqsim_plan_build(ctx, 1, &q1, plan1.get(), nullptr, 0, 0.0);
     DevBuf<double> zeros, dvals;
     zeros.alloc((size_t)N);
     dvals.alloc((size_t)p->n_data);
@@ -395,7 +395,7 @@ void fftsim_build(gsp_ctx* ctx, const gsp_fftsim_problem* p, gsp_fftsim_plan* P)
     gsp_seqsim_problem q2 = kriging_problem(p, p->n_cells, cc.data(), path.data(), N);
     P->kplan.reset(new gsp_seqsim_plan());
     P->kplan->pool = ctx->pool;
-    seqsim_plan_build(ctx, 1, &q2, P->kplan.get(), nullptr, 0.0);
+    seqsim_plan_build(ctx, 1, &q2, P->kplan.get(), nullptr, 0, 0.0);
     P->n_cells = p->n_cells;
     P->cells.alloc((size_t)p->n_cells);
     GSP_HIP(hipMemcpyAsync(P->cells.p, cells.data(), (size_t)p->n_cells * 8,

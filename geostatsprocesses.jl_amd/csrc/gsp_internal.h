@@ -372,14 +372,18 @@ struct gsp_lusim_plan {
 
 // kernels launched from gsp_seqsim.hip / gsp_lusim.hip
 namespace gsp {
+// surrogates: n_surrogates learned surrogates (none for an exact plan); a
+// request takes its weights from the first one whose domain holds it
 void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
-                       gsp_seqsim_plan* plan, const gsp_surrogate* surrogate, double tolerance);
+                       gsp_seqsim_plan* plan, const gsp_surrogate* const* surrogates,
+                       int32_t n_surrogates, double tolerance);
 void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
                      const gsp_train_options* opt, gsp_surrogate* s, gsp_train_report* rep);
 gsp_surrogate* surrogate_new(gsp_ctx* ctx);
 void surrogate_delete(gsp_surrogate* s);
 void surrogate_export(gsp_ctx* ctx, const gsp_surrogate* s, float* params, int64_t* count,
                       double* slots);
+void surrogate_info(const gsp_surrogate* s, gsp_surrogate_info* out);
 void seqsim_sweep_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, const double* z,
                       const double* data_vals, double* out);
 // Device memory of one sweep beyond its inputs and outputs: the node-major

@@ -21,7 +21,7 @@ struct Req {
   int32_t hd[3], hexp;   // data-hash cells per axis; 1: expanded cell lists
   int32_t hB, n_data;    // hash cell width in grid cells; conditioning data
   int32_t node_off, data_off;
-  int32_t learned;       // 1: the plan's surrogate may give this request's weights
+  int32_t learned;       // s + 1: the plan's surrogate s may give this request's weights
   int32_t pad_;
   int64_t table_off, cell_off;  // first entry of the offset table / of hstart
   double r2, alpha, prior_sigma, mean;
@@ -31,12 +31,16 @@ struct Req {
   DevModel M;
 };
 
+// Neighbour slots of the learned surrogate: the first kSurrogateSlots entries
+// of a request's offset table.
+constexpr int kSurrogateSlots = 32;
+
 // Host-side description of one request, kept with the plan.
 struct ReqInfo {
   gsp_model model;
   int32_t ndim = 0, mode = 0, kmin = 0, kmax = 0, T = 0;
   double spacing[3] = {1.0, 1.0, 1.0};
-  int16_t table24[24][3] = {};  // the first min(T, 24) offsets of the offset table
+  int16_t table32[kSurrogateSlots][3] = {};  // the first min(T, 32) offsets of the offset table
 };
 
 // Kernel-side view of a plan's per-node results (the outputs of the kriging
@@ -119,14 +123,23 @@ struct gsp_seqsim_plan {
 
 namespace gsp {
 
-// The learned path of the plan builder (gsp_surrogate.hip): runs the
-// surrogate on every node of the eligible requests, writes the accepted
-// nodes' weights, sigmas and flags, and lists the other
-// nodes in `exact_list` (count in counters[CTR_EXACT]).
-void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, const NodeOut& out, double tolerance,
-                     int32_t* exact_list);
-// Whether request `r` matches the surrogate (family, nugget ratio, no
-// rotation, k nearest then mask, maxneighbors <= 24, the same 24 nearest
-// offsets in range units).
+// The learned path of the plan builder (gsp_surrogate.hip): runs surrogate
+// `s` on every node of the requests with learned == which, writes the
+// accepted nodes' weights, sigmas and flags, and lists the other nodes of
+// those requests (and, with list_plain, every node of a request with
+// learned == 0) in `exact_list` (count in counters[CTR_EXACT]). `feat` holds
+// each request's input row (kSurrogateInputs floats) and `cslot` the
+// covariances C(slot offset) / sill of its slots (kSurrogateSlots floats).
+constexpr int kSurrogateInputs = 128;
+void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, int32_t which, bool list_plain,
+                     const NodeOut& out, double tolerance, int32_t* exact_list, const float* feat,
+                     const float* cslot);
+// Whether request `r` lies in the surrogate's domain: its family, no
+// rotation, k nearest then mask, maxneighbors <= 32, the surrogate's slot
+// offsets as the start of its offset table, a nugget ratio and grid spacing in
+// range units inside the spans trained on (and, for the gaussian family, the
+// same sill).
 bool surrogate_accepts(const gsp_surrogate* s, const ReqInfo& r);
+// The input row and slot covariances of an accepted request.
+void surrogate_request_rows(const gsp_surrogate* s, const ReqInfo& r, float* feat, float* cslot);
 }  // namespace gsp

@@ -2467,8 +2467,10 @@ struct WorkerThread {
 }  // namespace
 
 void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* problems,
-                       gsp_seqsim_plan* plan, const gsp_surrogate* surrogate, double tolerance) {
+                       gsp_seqsim_plan* plan, const gsp_surrogate* const* surrogates,
+                       int32_t n_surrogates, double tolerance) {
   GSP_REQUIRE(problems, GSP_ERR_ARGUMENT, "null problem");
+  const bool learned_plan = surrogates && n_surrogates > 0;
   GSP_REQUIRE(n_req >= 1 && n_req <= 65535, GSP_ERR_ARGUMENT,
               "number of requests must be 1..65535");
   hipStream_t s = ctx->stream;
@@ -2648,12 +2650,17 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       I.kmax = h.kmax;
       I.T = R.T;
       for (int d = 0; d < 3; ++d) I.spacing[d] = d < h.nd ? g.spacing[d] : 1.0;
-      for (int t = 0; t < 24 && t < R.T; ++t) {
-        I.table24[t][0] = h.table[(size_t)t].dx;
-        I.table24[t][1] = h.table[(size_t)t].dy;
-        I.table24[t][2] = h.table[(size_t)t].dz;
+      for (int t = 0; t < kSurrogateSlots && t < R.T; ++t) {
+        I.table32[t][0] = h.table[(size_t)t].dx;
+        I.table32[t][1] = h.table[(size_t)t].dy;
+        I.table32[t][2] = h.table[(size_t)t].dz;
       }
-      R.learned = surrogate_accepts(surrogate, I) ? 1 : 0;
+      R.learned = 0;
+      for (int32_t si = 0; learned_plan && si < n_surrogates; ++si)
+        if (surrogate_accepts(surrogates[si], I)) {
+          R.learned = si + 1;
+          break;
+        }
       means[(size_t)q] = p->mean;
       std::copy(h.table.begin(), h.table.end(), table.begin() + t_off);
       for (size_t c = 0; c < h.hstart.size(); ++c)
@@ -2751,8 +2758,8 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   A.nfound = plan->nfound.p;
   A.nbr = plan->nbr.p;
   DevBuf<uint8_t> nbr_slot;
-  if (surrogate) nbr_slot.alloc((size_t)(NT * K));
-  A.nbr_slot = surrogate ? nbr_slot.p : nullptr;
+  if (learned_plan) nbr_slot.alloc((size_t)(NT * K));
+  A.nbr_slot = learned_plan ? nbr_slot.p : nullptr;
   A.lam = plan->lam.p;
   A.sigma = plan->sigma.p;
   A.flags = plan->flags.p;
@@ -2827,7 +2834,24 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     // for the exact solver
     DevBuf<int32_t> exact_ids;
     int64_t n_exact = NT;
-    if (surrogate) {
+    if (learned_plan) {
+      // each request's input row and slot covariances for the surrogate it uses
+      std::vector<float> feat((size_t)n_req * kSurrogateInputs, 0.0f),
+          cslot((size_t)n_req * kSurrogateSlots, 0.0f);
+      std::vector<uint8_t> used((size_t)n_surrogates, 0);
+      for (int32_t q = 0; q < n_req; ++q) {
+        const int32_t l = reqs[(size_t)q].learned;
+        if (l <= 0) continue;
+        used[(size_t)(l - 1)] = 1;
+        surrogate_request_rows(surrogates[l - 1], plan->info[(size_t)q],
+                               feat.data() + (size_t)q * kSurrogateInputs,
+                               cslot.data() + (size_t)q * kSurrogateSlots);
+      }
+      DevBuf<float> dfeat, dcslot;
+      dfeat.alloc(feat.size());
+      dcslot.alloc(cslot.size());
+      GSP_HIP(hipMemcpyAsync(dfeat.p, feat.data(), feat.size() * 4, hipMemcpyHostToDevice, s));
+      GSP_HIP(hipMemcpyAsync(dcslot.p, cslot.data(), cslot.size() * 4, hipMemcpyHostToDevice, s));
       exact_ids.alloc((size_t)NT);
       NodeOut O;
       O.NT = (int32_t)NT;
@@ -2844,7 +2868,15 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       O.counters = ctx->counters.p;
       {
         TimedLaunch tl(ctx, GSP_K_LEARNED);
-        surrogate_infer(ctx, surrogate, O, tolerance, exact_ids.p);
+        // one pass per surrogate in use; the first pass also lists the nodes
+        // of requests no surrogate takes
+        bool listed = false;
+        for (int32_t si = 0; si < n_surrogates; ++si) {
+          if (!used[(size_t)si] && (listed || si + 1 < n_surrogates)) continue;
+          surrogate_infer(ctx, surrogates[si], si + 1, !listed, O, tolerance, exact_ids.p, dfeat.p,
+                          dcslot.p);
+          listed = true;
+        }
       }
       int32_t ne = 0;
       GSP_HIP(hipMemcpyAsync(&ne, ctx->counters.p + CTR_EXACT, 4, hipMemcpyDeviceToHost, s));
@@ -2852,7 +2884,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       n_exact = ne;
       tr.mark("learned weights");
     }
-    const int32_t* ids = surrogate ? exact_ids.p : nullptr;
+    const int32_t* ids = learned_plan ? exact_ids.p : nullptr;
     // bucket the nodes by neighbour count: n <= 16 run four per wavefront,
     // larger systems in the smallest register tile that holds them
     DevBuf<int32_t> nhist, list;

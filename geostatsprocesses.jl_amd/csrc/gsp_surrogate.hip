@@ -1,17 +1,21 @@
 // gsp_surrogate.hip — learned kriging-weight surrogate for SEQSIM on gfx950.
 //
-// A multilayer perceptron 73 -> 256 -> 256 -> 256 -> 25 (GELU) maps a node's
-// neighbourhood to its 24 simple-kriging weights and log kriging variance,
+// A multilayer perceptron 3S+1 -> 256 -> 256 -> 256 -> S+1 (GELU), S = 24 or
+// 32 neighbour slots (73 -> ... -> 25 or 97 -> ... -> 33), maps a node's
+// neighbourhood to its S simple-kriging weights and log kriging variance,
 // with BF16 weights and activations and FP32 accumulation on the matrix cores
-// (v_mfma_f32_16x16x32_bf16). Inputs: 24 slots of neighbour offsets (dx, dy,
-// dz divided by the ranges), zero for an absent neighbour, and the distance to
-// the nearest datum. A neighbour's slot is the rank of its offset in the
-// request's sorted offset table, so an interior node of a k-nearest-then-mask
-// search always fills slots 0..23 of the same 24 offsets and the input is the
-// pattern of already simulated neighbours. Nodes whose neighbours do not fit
-// (grid boundary, conditioning data among them) and nodes whose predicted
-// variance disagrees with the variance implied by the predicted weights are
-// solved exactly.
+// (v_mfma_f32_16x16x32_bf16). Inputs: S slots of neighbour offsets (dx, dy,
+// dz times the grid spacing, divided by the ranges), zero for an absent
+// neighbour, and the model's nugget / sill. A neighbour's slot is the rank of
+// its offset in the request's sorted offset table, so an interior node of a
+// k-nearest-then-mask search fills slots among the same S offsets and the
+// input is the pattern of already simulated neighbours, scaled by the range.
+// One surrogate serves one variogram family and grid dimension over the spans
+// of ranges (in grid cells) and nugget ratios it was trained on, so requests
+// of a batch with different ranges and nuggets share it. Nodes whose
+// neighbours do not fit (grid boundary, conditioning data among them) and
+// nodes whose predicted variance disagrees with the variance implied by the
+// predicted weights are solved exactly.
 //
 // Training runs on the device: a BF16 NT GEMM with fused bias/GELU epilogues
 // (forward), GELU-derivative epilogues (backward), weight gradients as NT
@@ -31,15 +35,17 @@
 namespace gsp {
 namespace {
 
-constexpr int kIn = 128;     // input features, 73 used
+constexpr int kIn = kSurrogateInputs;  // input features, 3 S + 1 used
 constexpr int kHid = 256;
-constexpr int kOutT = 64;    // output rows in training GEMMs, 25 used
-constexpr int kOutI = 32;    // output rows at inference, 25 used
-constexpr int kSlots = 24;
-constexpr int kUsedIn = 73;
-constexpr int kUsedOut = 25;
+constexpr int kOutT = 64;    // output rows in training GEMMs, S + 1 used
+constexpr int kSlotsMax = kSurrogateSlots;
+// output rows at inference (S + 1 used): whole 16-row MFMA tiles
+__host__ __device__ constexpr int out_rows(int slots) { return slots <= 31 ? 32 : 48; }
+constexpr int kOutIMax = out_rows(kSlotsMax);
 constexpr float kFeatScale = 8.0f;   // offsets in ranges x 8: inputs of order one
-constexpr float kDataFeature = 4.0f; // nearest-datum distance (ranges), capped
+constexpr float kNugScale = 10.0f;   // nugget / sill x 10
+static_assert(3 * kSlotsMax + 1 <= kIn && kSlotsMax + 1 <= kOutIMax && kOutIMax <= kOutT,
+              "network shapes");
 
 // FP32 master parameters, one buffer
 constexpr int64_t OW1 = 0, OB1 = OW1 + (int64_t)kHid * kIn, OW2 = OB1 + kHid,
@@ -51,7 +57,7 @@ constexpr int64_t BW1 = 0, BW2 = BW1 + (int64_t)kHid * kIn, BW3 = BW2 + (int64_t
                   BW4 = BW3 + (int64_t)kHid * kHid, kBf = BW4 + (int64_t)kOutT * kHid;
 // BF16 weights in the fragment order of the inference kernel
 constexpr int64_t PW1 = 0, PW2 = PW1 + (int64_t)kHid * kIn, PW3 = PW2 + (int64_t)kHid * kHid,
-                  PW4 = PW3 + (int64_t)kHid * kHid, kPerm = PW4 + (int64_t)kOutI * kHid;
+                  PW4 = PW3 + (int64_t)kHid * kHid, kPermMax = PW4 + (int64_t)kOutIMax * kHid;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
@@ -113,25 +119,28 @@ __host__ __device__ inline int frag_k(int s, int g, int i) {
   return 32 * s + (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
 }
 
-// Input feature k of a node whose occupied slots are `mask`.
-__device__ inline float feature(const float* canonf, uint32_t mask, int k) {
-  if (k < 3 * kSlots) return ((mask >> (k / 3)) & 1u) ? canonf[k] : 0.0f;
-  return k == 3 * kSlots ? kDataFeature : 0.0f;
+// Input feature k of a node whose occupied slots are `mask`; `row` is its
+// request's input row (slot offsets in ranges x kFeatScale, then the nugget
+// feature, then zeros).
+__device__ inline float feature(const float* row, uint32_t mask, int k, int slots) {
+  if (k < 3 * slots) return ((mask >> (k / 3)) & 1u) ? row[k] : 0.0f;
+  return row[k];
 }
 
-// Slot of grid offset (dx, dy, dz): its index among the first 24 entries of
-// the request's offset table, or -1.
-__device__ inline int slot_of(const Offset* tab, int T, int dx, int dy, int dz) {
-  const int n = T < kSlots ? T : kSlots;
+// Slot of grid offset (dx, dy, dz): its index among the first `slots`
+// entries of the request's offset table, or -1.
+__device__ inline int slot_of(const Offset* tab, int T, int dx, int dy, int dz, int slots) {
+  const int n = T < slots ? T : slots;
   for (int s = 0; s < n; ++s)
     if (tab[s].dx == dx && tab[s].dy == dy && tab[s].dz == dz) return s;
   return -1;
 }
 
 // The surrogate's view of node i: its slot mask (and slot of each neighbour)
-// when every neighbour is a grid node of the 24 table offsets, else false.
-__device__ inline bool node_slots(const NodeOut& O, const Req& R, int i, int n, uint32_t& mask,
-                                  int8_t* slots) {
+// when every neighbour is a grid node of the first `ns` table offsets, else
+// false.
+__device__ inline bool node_slots(const NodeOut& O, const Req& R, int i, int n, int ns,
+                                  uint32_t& mask, int8_t* slots) {
   const int nx = R.dims[0], ny = R.dims[1];
   const int li = i - R.node_off;
   const int ix = li % nx, iy = (li / nx) % ny, iz = li / (nx * ny);
@@ -141,7 +150,7 @@ __device__ inline bool node_slots(const NodeOut& O, const Req& R, int i, int n, 
     const int nb = O.nbr[(int64_t)i * O.K + j];
     if (nb >= O.NT) return false;
     const int l = nb - R.node_off;
-    const int s = slot_of(tab, R.T, l % nx - ix, (l / nx) % ny - iy, l / (nx * ny) - iz);
+    const int s = slot_of(tab, R.T, l % nx - ix, (l / nx) % ny - iy, l / (nx * ny) - iz, ns);
     if (s < 0) return false;
     mask |= 1u << s;
     if (slots) slots[j] = (int8_t)s;
@@ -227,26 +236,26 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs P) {
 // One minibatch: inputs (both layouts), targets and slot masks of the
 // samples perm[off .. off + Bt).
 __global__ void k_train_batch(const int32_t* perm, int64_t off, int32_t Bt, const uint32_t* masks,
-                              const float* Y, const float* canonf, uint16_t* X, uint16_t* Xt,
-                              float* Yb, uint32_t* Mb) {
+                              const int32_t* srow, const float* Y, const float* rows, int32_t slots,
+                              uint16_t* X, uint16_t* Xt, float* Yb, uint32_t* Mb) {
   const int64_t total = (int64_t)Bt * kIn;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(t / kIn), k = (int)(t % kIn);
     const int smp = perm[off + b];
     const uint32_t m = masks[smp];
-    const uint16_t h = f2bf(feature(canonf, m, k));
+    const uint16_t h = f2bf(feature(rows + (int64_t)srow[smp] * kIn, m, k, slots));
     X[(int64_t)b * kIn + k] = h;
     Xt[(int64_t)k * Bt + b] = h;
-    if (k < 32) Yb[(int64_t)b * 32 + k] = Y[(int64_t)smp * 32 + k];
+    if (k < kOutT) Yb[(int64_t)b * kOutT + k] = Y[(int64_t)smp * kOutT + k];
     if (k == 0) Mb[b] = m;
   }
 }
 
-// Loss = mean over the batch of sum_s occupied (o_s - y_s)^2 + wv (o_24 - y_24)^2;
+// Loss = mean over the batch of sum_s occupied (o_s - y_s)^2 + wv (o_S - y_S)^2;
 // writes dLoss/dO in bf16 (both layouts) and accumulates the loss terms.
 __global__ void k_train_loss(const float* O, const float* Yb, const uint32_t* Mb, int32_t Bt,
-                             float wv, uint16_t* dO, uint16_t* dOt, float* acc) {
+                             int32_t slots, float wv, uint16_t* dO, uint16_t* dOt, float* acc) {
   __shared__ float sw[256], sv[256], sc[256];
   float lw = 0.0f, lv = 0.0f, cnt = 0.0f;
   const int64_t total = (int64_t)Bt * kOutT;
@@ -254,14 +263,14 @@ __global__ void k_train_loss(const float* O, const float* Yb, const uint32_t* Mb
        t += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(t / kOutT), o = (int)(t % kOutT);
     float d = 0.0f;
-    if (o < kSlots) {
+    if (o < slots) {
       if ((Mb[b] >> o) & 1u) {
-        d = O[t] - Yb[(int64_t)b * 32 + o];
+        d = O[t] - Yb[(int64_t)b * kOutT + o];
         lw += d * d;
         cnt += 1.0f;
       }
-    } else if (o == kSlots) {
-      const float e = O[t] - Yb[(int64_t)b * 32 + o];
+    } else if (o == slots) {
+      const float e = O[t] - Yb[(int64_t)b * kOutT + o];
       lv += e * e;
       d = wv * e;
     }
@@ -339,7 +348,7 @@ __device__ inline bool weight_of(int64_t p, int& row, int& col, int& nin, int64_
 }
 
 // Adam on the FP32 master parameters, then their BF16 copies in both layouts.
-// Padding (inputs 73..127, outputs 25..63) keeps zero gradients and stays zero.
+// Padding (inputs 3S+1..127, outputs S+1..63) keeps zero gradients and stays zero.
 __global__ void k_adam(float* params, const float* grad, float* m, float* v, float lr, float b1,
                        float b2, float c1, float c2, float eps, uint16_t* wbf, uint16_t* wbt) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < kParams;
@@ -364,8 +373,9 @@ __global__ void k_adam(float* params, const float* grad, float* m, float* v, flo
 
 // BF16 copies of the master weights in the inference fragment order:
 // fragment (tile rt, step s), lane l, item i holds W[16 rt + l % 16][frag_k(s, l / 16, i)].
-__global__ void k_pack_inference(const float* params, uint16_t* wp) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < kPerm;
+__global__ void k_pack_inference(const float* params, uint16_t* wp, int32_t out_i) {
+  const int64_t n = PW4 + (int64_t)out_i * kHid;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     int64_t base, po;
     int nin;
@@ -386,16 +396,20 @@ __global__ void k_pack_inference(const float* params, uint16_t* wp) {
 // Training samples of one plan: every node of an accepted request that was
 // solved exactly with at least one neighbour, all in table slots, drawn with
 // probability `keep` (a hash of the node id and the seed). count != nullptr:
-// counts the candidates only.
+// counts the candidates only. A sample keeps its slot mask, the input row of
+// its request (req_row; -1: request not used) and its targets: the weights
+// by slot and log(sigma^2 / sill), kOutT floats.
 struct ExtractArgs {
   NodeOut O;
-  const uint8_t* req_ok;
+  int32_t slots;
+  const int32_t* req_row;
   uint32_t keep24;  // acceptance threshold on 24 hash bits
   uint64_t seed;
   int32_t* count;
   int32_t* cursor;
   int32_t capacity;
   uint32_t* masks;
+  int32_t* srow;
   float* Y;
 };
 
@@ -413,13 +427,14 @@ __global__ void k_extract(const ExtractArgs E) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < O.NT;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int p = O.node_req[i];
-    if (!E.req_ok[p]) continue;
+    const int row = E.req_row[p];
+    if (row < 0) continue;
     if (O.flags[i] & (GSP_FLAG_FEW | GSP_FLAG_FAIL | GSP_FLAG_LEARNED)) continue;
     const int n = O.nfound[i];
     if (n < 1) continue;
     const Req& R = O.reqs[p];
     uint32_t mask;
-    if (!node_slots(O, R, (int)i, n, mask, nullptr)) continue;
+    if (!node_slots(O, R, (int)i, n, E.slots, mask, nullptr)) continue;
     if (E.count) {
       atomicAdd(E.count, 1);
       continue;
@@ -428,19 +443,20 @@ __global__ void k_extract(const ExtractArgs E) {
     const int slot = atomicAdd(E.cursor, 1);
     if (slot >= E.capacity) continue;
     E.masks[slot] = mask;
-    float* y = E.Y + (int64_t)slot * 32;
-    for (int s = 0; s < 32; ++s) y[s] = 0.0f;
+    E.srow[slot] = row;
+    float* y = E.Y + (int64_t)slot * kOutT;
+    for (int s = 0; s < kOutT; ++s) y[s] = 0.0f;
     const int nx = R.dims[0], ny = R.dims[1];
     const int li = (int)i - R.node_off;
     const int ix = li % nx, iy = (li / nx) % ny, iz = li / (nx * ny);
     for (int j = 0; j < n; ++j) {
       const int l = O.nbr[i * O.K + j] - R.node_off;
       const int s = slot_of(O.table + R.table_off, R.T, l % nx - ix, (l / nx) % ny - iy,
-                            l / (nx * ny) - iz);
+                            l / (nx * ny) - iz, E.slots);
       y[s] = (float)O.lam[i * O.K + j];
     }
     const double s2 = O.sigma[i] * O.sigma[i] / R.M.sill;
-    y[kSlots] = (float)log(s2 > 1e-12 ? s2 : 1e-12);
+    y[E.slots] = (float)log(s2 > 1e-12 ? s2 : 1e-12);
   }
 }
 
@@ -449,9 +465,11 @@ struct InferArgs {
   NodeOut O;
   const uint16_t* wp;   // fragment-ordered BF16 weights
   const float* params;  // FP32 master (biases)
-  const float* canonf;  // input features of the 24 slots (72 floats)
-  const float* cslot;   // C(slot offset) / sill
+  const float* feat;    // input row of every request (kIn floats)
+  const float* cslot;   // C(slot offset) / sill of every request (kSlotsMax floats)
   float tol;
+  int32_t which;        // requests with learned == which are this surrogate's
+  int32_t list_plain;   // also list the nodes of requests with learned == 0
   int32_t* exact_list;
 };
 
@@ -518,9 +536,9 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "me
 // and the vector ALU overlap). The step ends by waiting for the loads issued
 // one step earlier only (loads complete in order; this wavefront's copy of
 // tile rt + 2 stays in flight), so each copy has a whole step to land.
-template <int STEPS, int TILES, bool ACT>
+template <int STEPS, int TILES, bool ACT, int LT>
 __device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)[8][2][4],
-                                          uint32_t (&out)[8][2][4], float (&last)[2][2][4],
+                                          uint32_t (&out)[8][2][4], float (&last)[LT][2][4],
                                           const uint16_t* wp, const float* bias, int lane, int w) {
   const int g = lane >> 4;
   f32x4 b_next = *(const f32x4*)(bias + 4 * g);
@@ -616,11 +634,14 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
   }
 }
 
+template <int SLOTS>
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
+  constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1;
   __shared__ FragBuf s_w0, s_w1, s_w2;
   const FragLds s_frag{s_w0, s_w1, s_w2};
   __shared__ uint32_t s_mask[kInferWaves][32];
-  __shared__ int8_t s_slot[kInferWaves][32][kSlots];
+  __shared__ int32_t s_req[kInferWaves][32];
+  __shared__ int8_t s_slot[kInferWaves][32][SLOTS];
   __shared__ float s_out[kInferWaves][32][kUsedOut + 1];
   __shared__ int32_t s_any;
   const NodeOut& O = A.O;
@@ -632,22 +653,23 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
   if (tid == 0) s_any = 0;
   __syncthreads();
   // ---- slots of the wave's 32 nodes
-  int state = -1;  // -1: no node, 0: exact solve, 1: learned candidate, 2: learned
+  int state = -1;  // -1: not this launch's node, 0: exact solve, 1: learned candidate, 2: learned
   const int64_t i = base + lane;
+  int p = 0;
   if (lane < 32 && i < O.NT) {
-    state = 0;
-    const int p = O.node_req[i];
+    p = O.node_req[i];
     const Req& R = O.reqs[p];
     const int n = O.nfound[i];
-    if (R.learned && n >= R.kmin && n >= 1) {
+    if (R.learned == A.which || (A.list_plain && R.learned == 0)) state = 0;
+    if (R.learned == A.which && n >= R.kmin && n >= 1) {
       // slots from the table indices the search recorded
       uint32_t mask = 0;
       bool fits = true;
       for (int j = 0; j < n; ++j) {
         const int t = O.nbr_slot[i * O.K + j];
-        fits = fits && t < kSlots;
-        mask |= t < kSlots ? 1u << t : 0u;
-        s_slot[w][lane][j] = (int8_t)(t < kSlots ? t : 0);
+        fits = fits && t < SLOTS;
+        mask |= t < SLOTS ? 1u << t : 0u;
+        s_slot[w][lane][j] = (int8_t)(t < SLOTS ? t : 0);
       }
       if (fits) {
         state = 1;
@@ -656,29 +678,33 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
       }
     }
   }
-  if (lane < 32 && state != 1) s_mask[w][lane] = 0;
+  if (lane < 32) {
+    if (state != 1) s_mask[w][lane] = 0;
+    s_req[w][lane] = p;
+  }
   __syncthreads();
   if (s_any) {
     // ---- layer 1 operands from the slot masks
     const int r = lane & 15, g = lane >> 4;
     uint32_t h0[8][2][4], h1[8][2][4];
-    float last[2][2][4];
+    float last[kOutI / 16][2][4];
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct) {
       const uint32_t m = s_mask[w][16 * ct + r];
+      const float* row = A.feat + (int64_t)s_req[w][16 * ct + r] * kIn;
 #pragma unroll
       for (int s = 0; s < kIn / 32; ++s)
 #pragma unroll
         for (int it = 0; it < 4; ++it)
-          h0[s][ct][it] = pack2(feature(A.canonf, m, frag_k(s, g, 2 * it)),
-                                feature(A.canonf, m, frag_k(s, g, 2 * it + 1)));
+          h0[s][ct][it] = pack2(feature(row, m, frag_k(s, g, 2 * it), SLOTS),
+                                feature(row, m, frag_k(s, g, 2 * it + 1), SLOTS));
     }
     mlp_layer<kIn / 32, kHid / 16, true>(s_frag, h0, h1, last, A.wp + PW1, A.params + OB1, lane, w);
     mlp_layer<kHid / 32, kHid / 16, true>(s_frag, h1, h0, last, A.wp + PW2, A.params + OB2, lane, w);
     mlp_layer<kHid / 32, kHid / 16, true>(s_frag, h0, h1, last, A.wp + PW3, A.params + OB3, lane, w);
     mlp_layer<kHid / 32, kOutI / 16, false>(s_frag, h1, h0, last, A.wp + PW4, A.params + OB4, lane, w);
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < kOutI / 16; ++rt)
 #pragma unroll
       for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
@@ -689,14 +715,15 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
     wave_sync();
     // ---- accept, or hand to the exact solver
     if (state == 1) {
-      const Req& R = O.reqs[O.node_req[i]];
+      const Req& R = O.reqs[p];
       const int n = O.nfound[i];
       const uint32_t mask = s_mask[w][lane];
       const float* out = s_out[w][lane];
+      const float* cs = A.cslot + (int64_t)p * kSlotsMax;
       float chk = 1.0f;
-      for (int s = 0; s < kSlots; ++s)
-        if ((mask >> s) & 1u) chk -= out[s] * A.cslot[s];
-      const float pred = expf(out[kSlots]);
+      for (int s = 0; s < SLOTS; ++s)
+        if ((mask >> s) & 1u) chk -= out[s] * cs[s];
+      const float pred = expf(out[SLOTS]);
       if (fabsf(pred - chk) <= A.tol * pred) {
         for (int j = 0; j < O.K; ++j)
           O.lam[i * O.K + j] = j < n ? (double)out[s_slot[w][lane][j]] : 0.0;
@@ -732,50 +759,99 @@ int grid_for(int64_t work, int per_block, int cap) {
 // ----------------------------------------------------------- the object --
 struct gsp_surrogate {
   std::shared_ptr<gsp::Pool> pool;  // declared first: outlives the buffers below
-  // the variogram and neighbourhood it serves
+  // the domain it serves: one family and grid dimension, the slot offsets
+  // (grid cells), and the spans of nugget / sill and of spacing / range per
+  // axis of the requests it was trained on
   int32_t family = 0;
-  double nugget_ratio = 0.0;
-  double sill = 1.0;  // of the training variogram (gaussian models must match it)
-  double canon[gsp::kSlots][3] = {};  // slot offsets in ranges
   int32_t ndim = 0;
+  double sill = 1.0;  // of the training variogram (gaussian models must match it)
+  double nug_lo = 0.0, nug_hi = 0.0;
+  double scale_lo[3] = {0, 0, 0}, scale_hi[3] = {0, 0, 0};
+  int32_t n_slots = gsp::kSlotsMax;  // 24 or 32
+  int16_t table[gsp::kSlotsMax][3] = {};
+  int32_t n_train_requests = 0;
   gsp::DevBuf<float> params, adam_m, adam_v, grad;
   gsp::DevBuf<uint16_t> wbf, wbt, wperm;
-  gsp::DevBuf<float> canonf, cslot;
   int64_t n_samples = 0;
 };
 
 namespace gsp {
+namespace {
 
-bool surrogate_accepts(const gsp_surrogate* s, const ReqInfo& r) {
-  if (!s) return false;
+// Family, dimension, search, rotation, neighbourhood size and slot offsets
+// (the part of the domain that does not depend on the trained spans).
+bool surrogate_compatible(const gsp_surrogate* s, const ReqInfo& r) {
   const gsp_model& m = r.model;
   if (m.family != s->family || m.has_rotation || r.mode != GSP_KNN_THEN_MASK) return false;
-  if (r.kmax > kSlots || r.T < kSlots || r.ndim != s->ndim || !(m.sill > 0)) return false;
-  if (std::fabs(m.nugget / m.sill - s->nugget_ratio) > 1e-12) return false;
+  if (r.kmax > s->n_slots || r.ndim != s->ndim || !(m.sill > 0)) return false;
   // the gaussian model's stabilising nugget is absolute, so its weights
   // depend on the sill itself, not only on the nugget/sill ratio
   if (m.family == GSP_GAUSSIAN && std::fabs(m.sill - s->sill) > 1e-12 * s->sill) return false;
-  for (int t = 0; t < kSlots; ++t)
-    for (int d = 0; d < r.ndim; ++d) {
-      const double c = (double)r.table24[t][d] * r.spacing[d] / m.ranges[d];
-      if (std::fabs(c - s->canon[t][d]) > 1e-9 * (1.0 + std::fabs(c))) return false;
-    }
+  for (int t = 0; t < s->n_slots && t < r.T; ++t)
+    for (int d = 0; d < 3; ++d)
+      if (r.table32[t][d] != s->table[t][d]) return false;
   return true;
 }
 
-void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, const NodeOut& out, double tolerance,
-                     int32_t* exact_list) {
+double axis_scale(const ReqInfo& r, int d) { return r.spacing[d] / r.model.ranges[d]; }
+
+bool within(double v, double lo, double hi) {
+  const double eps = 1e-9 * std::max(std::fabs(lo), std::fabs(hi)) + 1e-15;
+  return v >= lo - eps && v <= hi + eps;
+}
+
+}  // namespace
+
+bool surrogate_accepts(const gsp_surrogate* s, const ReqInfo& r) {
+  if (!s || !surrogate_compatible(s, r)) return false;
+  if (!within(r.model.nugget / r.model.sill, s->nug_lo, s->nug_hi)) return false;
+  for (int d = 0; d < r.ndim; ++d)
+    if (!within(axis_scale(r, d), s->scale_lo[d], s->scale_hi[d])) return false;
+  return true;
+}
+
+void surrogate_request_rows(const gsp_surrogate* s, const ReqInfo& r, float* feat, float* cslot) {
+  DevModel M;
+  std::memset(&M, 0, sizeof(M));
+  M.family = s->family;
+  M.sill = r.model.sill;
+  M.nugget = r.model.nugget;
+  for (int k = 0; k < kIn; ++k) feat[k] = 0.0f;
+  if (cslot)
+    for (int t = 0; t < kSlotsMax; ++t) cslot[t] = 0.0f;
+  for (int t = 0; t < s->n_slots; ++t) {
+    double u2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double c = d < r.ndim ? (double)s->table[t][d] * axis_scale(r, d) : 0.0;
+      feat[3 * t + d] = (float)(c * kFeatScale);
+      u2 += c * c;
+    }
+    if (cslot) cslot[t] = (float)(cov_u(M, std::sqrt(u2)) / r.model.sill);
+  }
+  feat[3 * s->n_slots] = (float)(r.model.nugget / r.model.sill * kNugScale);
+}
+
+void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, int32_t which, bool list_plain,
+                     const NodeOut& out, double tolerance, int32_t* exact_list, const float* feat,
+                     const float* cslot) {
   InferArgs A;
   A.O = out;
   A.wp = s->wperm.p;
   A.params = s->params.p;
-  A.canonf = s->canonf.p;
-  A.cslot = s->cslot.p;
+  A.feat = feat;
+  A.cslot = cslot;
   A.tol = (float)tolerance;
+  A.which = which;
+  A.list_plain = list_plain ? 1 : 0;
   A.exact_list = exact_list;
   const int64_t blocks = (out.NT + kInferWaves * 32 - 1) / (kInferWaves * 32);
   GSP_REQUIRE(blocks < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "too many nodes");
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0, ctx->stream, A);
+  if (s->n_slots == 24)
+    hipLaunchKernelGGL(k_mlp_infer<24>, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0,
+                       ctx->stream, A);
+  else
+    hipLaunchKernelGGL(k_mlp_infer<32>, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0,
+                       ctx->stream, A);
   GSP_HIP(hipGetLastError());
 }
 
@@ -808,56 +884,90 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
               "batch must be a positive multiple of 64 (at most 65536)");
   GSP_REQUIRE(opt->epochs >= 1 && opt->lr > 0 && opt->var_weight >= 0, GSP_ERR_ARGUMENT,
               "invalid training options");
+  GSP_REQUIRE(opt->n_slots == 0 || opt->n_slots == 24 || opt->n_slots == 32, GSP_ERR_ARGUMENT,
+              "n_slots must be 0 (automatic), 24 or 32");
   hipStream_t st = ctx->stream;
-
-  // ---- the variogram and slot offsets the surrogate serves (first request)
-  const ReqInfo& r0 = plans[0]->info[0];
-  const gsp_model& m0 = r0.model;
-  GSP_REQUIRE(!m0.has_rotation && r0.mode == GSP_KNN_THEN_MASK && r0.T >= kSlots && m0.sill > 0,
-              GSP_ERR_UNSUPPORTED,
-              "the surrogate needs a k-nearest-then-mask search over at least 24 grid offsets "
-              "and an unrotated variogram");
-  S->family = m0.family;
-  S->nugget_ratio = m0.nugget / m0.sill;
-  S->sill = m0.sill;
-  S->ndim = r0.ndim;
-  for (int t = 0; t < kSlots; ++t)
-    for (int d = 0; d < 3; ++d)
-      S->canon[t][d] = d < r0.ndim ? (double)r0.table24[t][d] * r0.spacing[d] / m0.ranges[d] : 0.0;
-  std::vector<float> canonf(3 * kSlots), cslot(kSlots);
-  DevModel M;
-  std::memset(&M, 0, sizeof(M));
-  M.family = S->family;
-  M.sill = m0.sill;
-  M.nugget = m0.nugget;
-  for (int t = 0; t < kSlots; ++t) {
-    double u2 = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      canonf[(size_t)(3 * t + d)] = (float)(S->canon[t][d] * kFeatScale);
-      u2 += S->canon[t][d] * S->canon[t][d];
-    }
-    cslot[(size_t)t] = (float)(cov_u(M, std::sqrt(u2)) / m0.sill);
+  auto usable = [](const ReqInfo& r) {
+    return !r.model.has_rotation && r.mode == GSP_KNN_THEN_MASK && r.model.sill > 0;
+  };
+  // ---- slots: 24 when no usable request asks for more than 24 neighbours
+  int32_t slots = opt->n_slots;
+  if (slots == 0) {
+    int32_t kmax = 0;
+    for (int32_t q = 0; q < n_plans; ++q)
+      for (const ReqInfo& r : plans[q]->info)
+        if (usable(r) && r.kmax <= kSlotsMax) kmax = std::max(kmax, r.kmax);
+    slots = kmax > 24 ? 32 : 24;
   }
-  S->canonf.alloc(canonf.size());
-  S->cslot.alloc(cslot.size());
-  GSP_HIP(hipMemcpyAsync(S->canonf.p, canonf.data(), canonf.size() * 4, hipMemcpyHostToDevice, st));
-  GSP_HIP(hipMemcpyAsync(S->cslot.p, cslot.data(), cslot.size() * 4, hipMemcpyHostToDevice, st));
+  S->n_slots = slots;
+
+  // ---- the domain the surrogate serves: the family, dimension and slot
+  // offsets of the first usable request with at most `slots` neighbours and
+  // `slots` grid offsets; then the spans of nugget / sill and spacing / range
+  // of every compatible request
+  const ReqInfo* first = nullptr;
+  for (int32_t q = 0; q < n_plans && !first; ++q)
+    for (const ReqInfo& r : plans[q]->info)
+      if (usable(r) && r.T >= slots && r.kmax <= slots) {
+        first = &r;
+        break;
+      }
+  GSP_REQUIRE(first, GSP_ERR_UNSUPPORTED,
+              "the surrogate needs a request with a k-nearest-then-mask search over at least %d "
+              "grid offsets, maxneighbors <= %d and an unrotated variogram", slots, slots);
+  S->family = first->model.family;
+  S->sill = first->model.sill;
+  S->ndim = first->ndim;
+  std::memcpy(S->table, first->table32, sizeof(S->table));
+  std::vector<float> rows;
+  std::vector<std::vector<int32_t>> req_rows((size_t)n_plans);
+  bool any = false;
+  for (int32_t q = 0; q < n_plans; ++q) {
+    const gsp_seqsim_plan* p = plans[q];
+    std::vector<int32_t>& rr = req_rows[(size_t)q];
+    rr.assign(p->info.size(), -1);
+    for (size_t r = 0; r < p->info.size(); ++r) {
+      const ReqInfo& I = p->info[r];
+      if (!surrogate_compatible(S, I)) continue;
+      const double nr = I.model.nugget / I.model.sill;
+      S->nug_lo = any ? std::min(S->nug_lo, nr) : nr;
+      S->nug_hi = any ? std::max(S->nug_hi, nr) : nr;
+      for (int d = 0; d < I.ndim; ++d) {
+        const double a = axis_scale(I, d);
+        S->scale_lo[d] = any ? std::min(S->scale_lo[d], a) : a;
+        S->scale_hi[d] = any ? std::max(S->scale_hi[d], a) : a;
+      }
+      any = true;
+      rr[r] = (int32_t)(rows.size() / kIn);
+      rows.resize(rows.size() + kIn);
+    }
+  }
+  // input rows (after the spans are final: the rows do not depend on them)
+  for (int32_t q = 0; q < n_plans; ++q)
+    for (size_t r = 0; r < plans[q]->info.size(); ++r)
+      if (req_rows[(size_t)q][r] >= 0)
+        surrogate_request_rows(S, plans[q]->info[r], rows.data() + (size_t)req_rows[(size_t)q][r] * kIn,
+                               nullptr);
+  S->n_train_requests = (int32_t)(rows.size() / kIn);
+  DevBuf<float> drows;
+  drows.alloc(rows.size());
+  GSP_HIP(hipMemcpyAsync(drows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, st));
 
   // ---- training systems
   ctx->counters.reserve(16);
-  std::vector<DevBuf<uint8_t>> ok((size_t)n_plans);
+  std::vector<DevBuf<int32_t>> ok((size_t)n_plans);
   int64_t candidates = 0;
   for (int32_t q = 0; q < n_plans; ++q) {
     const gsp_seqsim_plan* p = plans[q];
-    std::vector<uint8_t> acc(p->info.size());
-    for (size_t r = 0; r < acc.size(); ++r) acc[r] = surrogate_accepts(S, p->info[r]) ? 1 : 0;
-    ok[(size_t)q].alloc(acc.size());
-    GSP_HIP(hipMemcpyAsync(ok[(size_t)q].p, acc.data(), acc.size(), hipMemcpyHostToDevice, st));
+    const std::vector<int32_t>& rr = req_rows[(size_t)q];
+    ok[(size_t)q].alloc(rr.size());
+    GSP_HIP(hipMemcpyAsync(ok[(size_t)q].p, rr.data(), rr.size() * 4, hipMemcpyHostToDevice, st));
     GSP_HIP(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int32_t), st));
     ExtractArgs E;
     std::memset(&E, 0, sizeof(E));
     E.O = node_view(p);
-    E.req_ok = ok[(size_t)q].p;
+    E.slots = slots;
+    E.req_row = ok[(size_t)q].p;
     E.count = ctx->counters.p;
     hipLaunchKernelGGL(k_extract, dim3(grid_for(p->N, 256, 4096)), dim3(256), 0, st, E);
     int32_t c = 0;
@@ -874,20 +984,24 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   const uint32_t keep24 = keep >= 1.0 ? (1u << 24) : (uint32_t)(keep * (double)(1u << 24));
   const int64_t capacity = std::min<int64_t>(candidates, want + want / 16 + 4096);
   DevBuf<uint32_t> masks;
+  DevBuf<int32_t> srow;
   DevBuf<float> Y;
   masks.alloc((size_t)capacity);
-  Y.alloc((size_t)(capacity * 32));
+  srow.alloc((size_t)capacity);
+  Y.alloc((size_t)(capacity * kOutT));
   GSP_HIP(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int32_t), st));
   for (int32_t q = 0; q < n_plans; ++q) {
     ExtractArgs E;
     std::memset(&E, 0, sizeof(E));
     E.O = node_view(plans[q]);
-    E.req_ok = ok[(size_t)q].p;
+    E.slots = slots;
+    E.req_row = ok[(size_t)q].p;
     E.keep24 = keep24;
     E.seed = opt->seed * 0x100000001B3ULL + (uint64_t)q;
     E.cursor = ctx->counters.p + 1;
     E.capacity = (int32_t)capacity;
     E.masks = masks.p;
+    E.srow = srow.p;
     E.Y = Y.p;
     hipLaunchKernelGGL(k_extract, dim3(grid_for(plans[q]->N, 256, 4096)), dim3(256), 0, st, E);
   }
@@ -909,21 +1023,22 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
       for (int cc = 0; cc < cols; ++cc)
         P0[(size_t)(off + (int64_t)rr * cols + cc)] = (rr < used_rows && cc < used_cols) ? U(rng) : 0.0f;
   };
-  fill(OW1, kHid, kIn, kHid, kUsedIn, kUsedIn);
-  fill(OB1, 1, kHid, 1, kHid, kUsedIn);
+  const int used_in = 3 * slots + 1, used_out = slots + 1;
+  fill(OW1, kHid, kIn, kHid, used_in, used_in);
+  fill(OB1, 1, kHid, 1, kHid, used_in);
   fill(OW2, kHid, kHid, kHid, kHid, kHid);
   fill(OB2, 1, kHid, 1, kHid, kHid);
   fill(OW3, kHid, kHid, kHid, kHid, kHid);
   fill(OB3, 1, kHid, 1, kHid, kHid);
-  fill(OW4, kOutT, kHid, kUsedOut, kHid, kHid);
-  fill(OB4, 1, kOutT, 1, kUsedOut, kHid);
+  fill(OW4, kOutT, kHid, used_out, kHid, kHid);
+  fill(OB4, 1, kOutT, 1, used_out, kHid);
   S->params.alloc((size_t)kParams);
   S->adam_m.alloc((size_t)kParams);
   S->adam_v.alloc((size_t)kParams);
   S->grad.alloc((size_t)kParams);
   S->wbf.alloc((size_t)kBf);
   S->wbt.alloc((size_t)kBf);
-  S->wperm.alloc((size_t)kPerm);
+  S->wperm.alloc((size_t)kPermMax);
   GSP_HIP(hipMemcpyAsync(S->params.p, P0.data(), P0.size() * 4, hipMemcpyHostToDevice, st));
   GSP_HIP(hipMemsetAsync(S->adam_m.p, 0, (size_t)kParams * 4, st));
   GSP_HIP(hipMemsetAsync(S->adam_v.p, 0, (size_t)kParams * 4, st));
@@ -949,7 +1064,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   G2.alloc(B * kHid);
   G3.alloc(B * kHid);
   Out.alloc(B * kOutT);
-  Yb.alloc(B * 32);
+  Yb.alloc(B * kOutT);
   Mb.alloc(B);
   lossacc.alloc(4);
   perm.alloc((size_t)S_n);
@@ -1002,7 +1117,8 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
         lr = opt->lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.14159265358979323846 * f)));
       }
       hipLaunchKernelGGL(k_train_batch, dim3(grid_for(B * kIn, 256, 1024)), dim3(256), 0, st,
-                         perm.p, b * Bt, Bt, masks.p, Y.p, S->canonf.p, X.p, Xt.p, Yb.p, Mb.p);
+                         perm.p, b * Bt, Bt, masks.p, srow.p, Y.p, drows.p, slots, X.p, Xt.p,
+                         Yb.p, Mb.p);
       // forward
       gemm(EPI_GELU, X.p, kIn, S->wbf.p + BW1, kIn, Bt, kHid, kIn, nullptr, 0, S->params.p + OB1,
            H1.p, H1t.p, G1.p, nullptr);
@@ -1013,7 +1129,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
       gemm(EPI_F32, H3.p, kHid, S->wbf.p + BW4, kHid, Bt, kOutT, kHid, Out.p, kOutT,
            S->params.p + OB4, nullptr, nullptr, nullptr, nullptr);
       hipLaunchKernelGGL(k_train_loss, dim3(grid_for(B * kOutT, 256, 256)), dim3(256), 0, st, Out.p,
-                         Yb.p, Mb.p, Bt, (float)opt->var_weight, dO.p, dOt.p, lossacc.p);
+                         Yb.p, Mb.p, Bt, slots, (float)opt->var_weight, dO.p, dOt.p, lossacc.p);
       // backward
       gemm(EPI_F32, dOt.p, Bt, H3t.p, Bt, kOutT, kHid, Bt, grad + OW4, kHid, nullptr, nullptr,
            nullptr, nullptr, nullptr);
@@ -1045,8 +1161,8 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
     last_v = std::sqrt(acc[1] / (double)(steps_per_epoch * Bt));
     if (first_w < 0) first_w = last_w;
   }
-  hipLaunchKernelGGL(k_pack_inference, dim3(grid_for(kPerm, 256, 1024)), dim3(256), 0, st,
-                     S->params.p, S->wperm.p);
+  hipLaunchKernelGGL(k_pack_inference, dim3(grid_for(kPermMax, 256, 1024)), dim3(256), 0, st,
+                     S->params.p, S->wperm.p, out_rows(slots));
   GSP_HIP(hipStreamSynchronize(st));
   if (rep) {
     rep->n_samples = S_n;
@@ -1065,8 +1181,8 @@ void surrogate_export(gsp_ctx* ctx, const gsp_surrogate* s, float* params, int64
                       double* slots) {
   if (count) *count = kParams;
   if (slots)
-    for (int t = 0; t < kSlots; ++t)
-      for (int d = 0; d < 3; ++d) slots[3 * t + d] = s->canon[t][d];
+    for (int t = 0; t < kSlotsMax; ++t)
+      for (int d = 0; d < 3; ++d) slots[3 * t + d] = t < s->n_slots ? (double)s->table[t][d] : 0.0;
   if (params) {
     GSP_HIP(hipMemcpyAsync(params, s->params.p, (size_t)kParams * 4, hipMemcpyDeviceToHost,
                            ctx->stream));
@@ -1082,6 +1198,22 @@ void mfma_probe_bf16(gsp_ctx* ctx, double* host_out) {
   GSP_HIP(hipMemcpyAsync(host_out, d.p, 3 * 64 * 4 * sizeof(double), hipMemcpyDeviceToHost,
                          ctx->stream));
   GSP_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void surrogate_info(const gsp_surrogate* s, gsp_surrogate_info* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->family = s->family;
+  out->ndim = s->ndim;
+  out->n_slots = s->n_slots;
+  out->n_train_requests = s->n_train_requests;
+  out->sill = s->sill;
+  out->nugget_ratio[0] = s->nug_lo;
+  out->nugget_ratio[1] = s->nug_hi;
+  for (int d = 0; d < 3; ++d) {
+    out->scale[d][0] = s->scale_lo[d];
+    out->scale[d][1] = s->scale_hi[d];
+  }
+  out->n_samples = s->n_samples;
 }
 
 gsp_surrogate* surrogate_new(gsp_ctx* ctx) {

@@ -96,7 +96,7 @@ STAT_MEAN, STAT_VAR, STAT_CDF, STAT_CCDF, STAT_QUANTILE = 0, 1, 2, 3, 4
 class GspTrainOptions(ctypes.Structure):
     _fields_ = [("max_samples", ctypes.c_int64), ("epochs", ctypes.c_int32),
                 ("batch", ctypes.c_int32), ("lr", ctypes.c_double), ("var_weight", ctypes.c_double),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("n_slots", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class GspTrainReport(ctypes.Structure):
@@ -104,6 +104,13 @@ class GspTrainReport(ctypes.Structure):
                 ("steps", ctypes.c_int64), ("rms_weight_first_epoch", ctypes.c_double),
                 ("rms_weight_last_epoch", ctypes.c_double),
                 ("rms_logvar_last_epoch", ctypes.c_double), ("ms", ctypes.c_double)]
+
+
+class GspSurrogateInfo(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_int32), ("ndim", ctypes.c_int32), ("n_slots", ctypes.c_int32),
+                ("n_train_requests", ctypes.c_int32), ("sill", ctypes.c_double),
+                ("nugget_ratio", ctypes.c_double * 2), ("scale", (ctypes.c_double * 2) * 3),
+                ("n_samples", ctypes.c_int64)]
 
 
 class GspTimings(ctypes.Structure):
@@ -143,6 +150,10 @@ SIGNATURES = {
     "gsp_surrogate_export": (_I32, [_VP, _VP, _VP, ctypes.POINTER(_I64), _VP]),
     "gsp_seqsim_plan_create_learned": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem), _VP,
                                               ctypes.c_double, ctypes.POINTER(_VP)]),
+    "gsp_seqsim_plan_create_learned_multi": (_I32, [_VP, _I32, ctypes.POINTER(GspSeqsimProblem),
+                                                    _I32, _VP, ctypes.c_double,
+                                                    ctypes.POINTER(_VP)]),
+    "gsp_surrogate_get_info": (_I32, [_VP, ctypes.POINTER(GspSurrogateInfo)]),
     "gsp_lusim_factor": (_I32, [_VP, ctypes.POINTER(GspLusimProblem), ctypes.POINTER(_VP)]),
     "gsp_lusim_sample": (_I32, [_VP, _VP, _I32, _VP, _VP, _I32]),
     "gsp_lusim_dims": (_I32, [_VP, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
@@ -331,12 +342,16 @@ class SeqsimPlan:
         self._keep = keep
         self._h = ctypes.c_void_p()
         if surrogate is not None:
+            # one surrogate, or a sequence of them (each request takes the
+            # first whose domain holds it)
             probs = [problem] if isinstance(problem, GspSeqsimProblem) else list(problem)
             arr = (GspSeqsimProblem * len(probs))(*probs)
-            self._keep = (keep, arr, surrogate)
-            check(load().gsp_seqsim_plan_create_learned(ctx.handle, len(probs), arr,
-                                                        surrogate.handle, float(tolerance),
-                                                        ctypes.byref(self._h)))
+            surs = list(surrogate) if isinstance(surrogate, (list, tuple)) else [surrogate]
+            handles = (ctypes.c_void_p * len(surs))(*[x.handle.value for x in surs])
+            self._keep = (keep, arr, surs, handles)
+            check(load().gsp_seqsim_plan_create_learned_multi(
+                ctx.handle, len(probs), arr, len(surs), handles, float(tolerance),
+                ctypes.byref(self._h)))
         elif isinstance(problem, GspSeqsimProblem):
             check(load().gsp_seqsim_plan_create(ctx.handle, ctypes.byref(problem),
                                                 ctypes.byref(self._h)))
@@ -457,11 +472,12 @@ class Surrogate:
     exact plans (gsp_surrogate_train)."""
 
     def __init__(self, ctx: Context, plans, max_samples=0, epochs=20, batch=1024, lr=2e-3,
-                 var_weight=0.01, seed=0):
+                 var_weight=0.01, seed=0, n_slots=0):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
         opts = GspTrainOptions(max_samples=int(max_samples), epochs=int(epochs), batch=int(batch),
-                               lr=float(lr), var_weight=float(var_weight), seed=int(seed))
+                               lr=float(lr), var_weight=float(var_weight), seed=int(seed),
+                               n_slots=int(n_slots))
         handles = (ctypes.c_void_p * len(plans))(*[p._h.value for p in plans])
         self.report = GspTrainReport()
         check(load().gsp_surrogate_train(ctx.handle, len(plans), handles, ctypes.byref(opts),
@@ -472,14 +488,25 @@ class Surrogate:
         return self._h
 
     def export(self):
-        """(params FP32 flat array, slot offsets in ranges (24, 3))."""
+        """(params FP32 flat array, slot offsets in grid cells (S, 3))."""
         n = ctypes.c_int64()
         check(load().gsp_surrogate_export(self.ctx.handle, self._h, None, ctypes.byref(n), None))
         params = np.zeros(n.value, np.float32)
-        slots = np.zeros((24, 3))
+        slots = np.zeros((32, 3))
         check(load().gsp_surrogate_export(self.ctx.handle, self._h, params.ctypes.data,
                                           ctypes.byref(n), slots.ctypes.data))
-        return params, slots
+        return params, slots[:self.info["n_slots"]]
+
+    @property
+    def info(self):
+        """The domain it serves: family, ndim, slots, the sill (gaussian), and
+        the trained spans of nugget / sill and spacing / range per axis."""
+        i = GspSurrogateInfo()
+        check(load().gsp_surrogate_get_info(self._h, ctypes.byref(i)))
+        return {"family": i.family, "ndim": i.ndim, "n_slots": i.n_slots,
+                "n_train_requests": i.n_train_requests, "sill": i.sill,
+                "nugget_ratio": tuple(i.nugget_ratio),
+                "scale": [tuple(i.scale[d]) for d in range(3)], "n_samples": i.n_samples}
 
     def close(self):
         if self._h and self._h.value:

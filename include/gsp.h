@@ -176,7 +176,22 @@ typedef struct {
   double lr;                    /* peak Adam step (one-cycle: warm-up, cosine decay) */
   double var_weight;            /* weight of the log-variance term in the loss */
   uint64_t seed;
+  int32_t n_slots;              /* neighbour slots S: 24, 32, or 0 for 24 unless a
+                                   training request asks for more than 24 neighbours */
+  int32_t pad_;
 } gsp_train_options;
+
+/* The domain a trained surrogate serves (gsp_surrogate_get_info). */
+typedef struct {
+  int32_t family;               /* gsp_family of every request it takes */
+  int32_t ndim;
+  int32_t n_slots;              /* neighbour slots S (24 or 32): maxneighbors <= S */
+  int32_t n_train_requests;     /* requests whose systems it was trained on */
+  double sill;                  /* gaussian requests must have this sill */
+  double nugget_ratio[2];       /* span of nugget / sill trained on */
+  double scale[3][2];           /* span of spacing / range per axis trained on */
+  int64_t n_samples;
+} gsp_surrogate_info;
 
 typedef struct {
   int64_t n_samples;            /* systems trained on */
@@ -351,30 +366,40 @@ int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,
 
 /* ---- learned kriging weights ----------------------------------------------
  * No counterpart in the reference: a surrogate for the fit / predictprob
- * steps of seqsim.jl:124-131. A BF16 multilayer perceptron 73 -> 256 -> 256 ->
- * 256 -> 25 (GELU) maps a node's neighbour offsets (divided by the ranges, in
- * slots given by the rank of the offset in the search's sorted offset table)
- * to its 24 kriging weights and log kriging variance.
+ * steps of seqsim.jl:124-131. A BF16 multilayer perceptron 3S+1 -> 256 ->
+ * 256 -> 256 -> S+1 (GELU; S = 24 slots: 73 -> ... -> 25, or S = 32: 97 ->
+ * ... -> 33) maps a node's neighbour offsets (times the grid spacing, divided
+ * by the ranges, in slots given by the rank of the offset in the search's
+ * sorted offset table) and the model's nugget / sill to its S kriging weights
+ * and log kriging variance.
  *
- * gsp_surrogate_train draws training systems from exact plans (every node of
- * a request with the first request's variogram family, nugget/sill ratio and
- * 24 nearest offsets in range units, solved exactly, with only grid neighbours
- * among those 24 offsets) and trains on the device. */
+ * gsp_surrogate_train draws training systems from exact plans: every node,
+ * solved exactly with only grid neighbours among the S slot offsets, of the
+ * requests compatible with the first usable one (k nearest then mask,
+ * unrotated, maxneighbors <= S, at least S table offsets): same variogram
+ * family, grid dimension and slot offsets (and sill, for the gaussian family),
+ * any range, nugget and maxneighbors. The surrogate then serves requests
+ * inside the spans of nugget / sill and spacing / range it was trained on. */
 int32_t gsp_surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
                             const gsp_train_options* options, gsp_surrogate** out,
                             gsp_train_report* report);
 int32_t gsp_surrogate_free(gsp_surrogate* surrogate);
 /* The surrogate's FP32 master parameters, layer by layer: W1 [256][128]
- * (inputs 73..127 zero), b1 [256], W2 [256][256], b2, W3 [256][256], b3,
- * W4 [64][256] (outputs 25..63 zero), b4 [64]; and its 24 slot offsets in
- * ranges (slots[24][3]). Inference uses the weights rounded to BF16. Either
- * pointer may be NULL; *count receives the number of parameters. */
+ * (inputs 3S+1..127 zero), b1 [256], W2 [256][256], b2, W3 [256][256], b3,
+ * W4 [64][256] (outputs S+1..63 zero), b4 [64]; and its S slot offsets in
+ * grid cells (slots[32][3], rows S..31 zero). Input k < 3S of a node is, when
+ * slot k / 3 holds a neighbour, 8 * slots[k / 3][d] * spacing[d] / range[d]
+ * (d = k % 3), else 0; input 3S is 10 * nugget / sill. Outputs 0..S-1 are the
+ * weights by slot, output S is log(sigma^2 / sill). Inference uses the
+ * weights rounded to BF16. Either pointer may be NULL; *count receives the
+ * number of parameters. */
 int32_t gsp_surrogate_export(gsp_ctx* ctx, const gsp_surrogate* surrogate, float* params,
                              int64_t* count, double* slots);
+int32_t gsp_surrogate_get_info(const gsp_surrogate* surrogate, gsp_surrogate_info* info);
 /* gsp_seqsim_plan_create_learned: gsp_seqsim_plan_create_batch whose nodes
  * take their weights and sigma from the surrogate where it applies (requests
- * matching its variogram and offsets, k nearest then mask, maxneighbors <= 24;
- * nodes whose neighbours all lie in the 24 table offsets). A node is solved
+ * inside its domain, see gsp_surrogate_get_info; nodes whose neighbours all
+ * lie in its S slot offsets). A node is solved
  * exactly instead when the variance implied by the predicted weights,
  * sill - sum_j lambda_j C(x_j, x_0), differs from the predicted variance by
  * more than `tolerance` (relative). Learned nodes carry GSP_FLAG_LEARNED;
@@ -383,6 +408,14 @@ int32_t gsp_seqsim_plan_create_learned(gsp_ctx* ctx, int32_t n,
                                        const gsp_seqsim_problem* problems,
                                        const gsp_surrogate* surrogate, double tolerance,
                                        gsp_seqsim_plan** out);
+/* The same with several surrogates (1..64, for example one per variogram
+ * family of a mixed batch): each request takes its weights from the first
+ * surrogate whose domain holds it, or is solved exactly. */
+int32_t gsp_seqsim_plan_create_learned_multi(gsp_ctx* ctx, int32_t n,
+                                             const gsp_seqsim_problem* problems,
+                                             int32_t n_surrogates,
+                                             const gsp_surrogate* const* surrogates,
+                                             double tolerance, gsp_seqsim_plan** out);
 
 /* ---- LUSIM ---------------------------------------------------------------
  * gsp_lusim_factor replaces preprocess(rng, process, ::LUSIM, ...) for one

@@ -643,7 +643,8 @@ def lusim_sample(d2, L22, dinds, sinds, z1, mean, w):
 
 # ------------------------------------------------------ learned kriging weights
 MLP_IN, MLP_HID, MLP_OUT = 128, 256, 64
-MLP_FEAT_SCALE, MLP_DATA_FEATURE = 8.0, 4.0
+MLP_SLOTS = 32
+MLP_FEAT_SCALE, MLP_NUG_SCALE = 8.0, 10.0
 
 
 def bf16(x):
@@ -668,14 +669,34 @@ def mlp_unpack(params):
     return out
 
 
-def mlp_features(slots, masks):
-    """Inputs of the surrogate for slot masks (bit s: neighbour in slot s)."""
+def mlp_request_row(case: SeqsimCase, slot_cells):
+    """The input row of a request for a surrogate of len(slot_cells) slots
+    (its slot offsets in ranges times the feature scale, then the nugget
+    feature), float32 as the device computes it: offset * (spacing / range)
+    in float64, rounded once."""
+    nd = case.ndim
+    S = len(slot_cells)
+    sp = list(case.spacing) + [1.0] * (3 - nd)
+    rg = list(case.ranges) * nd if len(case.ranges) == 1 else list(case.ranges)
+    row = np.zeros(MLP_IN, np.float32)
+    for t in range(S):
+        for d in range(3):
+            c = float(slot_cells[t][d]) * (sp[d] / rg[d]) if d < nd else 0.0
+            row[3 * t + d] = np.float32(c * MLP_FEAT_SCALE)
+    row[3 * S] = np.float32(case.nugget / case.sill * MLP_NUG_SCALE)
+    return row
+
+
+def mlp_features(row, masks, S):
+    """Inputs of an S-slot surrogate for slot masks (bit s: neighbour in slot
+    s) of one request's input row."""
     masks = np.asarray(masks, dtype=np.uint32)
     X = np.zeros((len(masks), MLP_IN), np.float32)
-    occ = ((masks[:, None] >> np.arange(24, dtype=np.uint32)[None, :]) & 1).astype(np.float32)
-    X[:, :72] = (occ[:, :, None] * (np.asarray(slots) * MLP_FEAT_SCALE)[None, :, :].astype(np.float32)
-                 ).reshape(len(masks), 72)
-    X[:, 72] = MLP_DATA_FEATURE
+    occ = ((masks[:, None] >> np.arange(S, dtype=np.uint32)[None, :]) & 1).astype(np.float32)
+    k = 3 * S
+    X[:, :k] = (occ[:, :, None] * np.asarray(row[:k], np.float32).reshape(S, 3)[None, :, :]
+                ).reshape(len(masks), k)
+    X[:, k:] = np.asarray(row[k:], np.float32)[None, :]
     return X
 
 
@@ -685,17 +706,18 @@ def gelu_tanh(z):
     return z / (np.float32(1.0) + np.exp(np.float32(-2.0) * u))
 
 
-def mlp_forward(params, slots, masks):
-    """Outputs (n, 25): 24 slot weights and log(sigma^2 / sill)."""
-    h = bf16(mlp_features(slots, masks))
+def mlp_forward(params, row, masks, S):
+    """Outputs (n, S + 1) of an S-slot surrogate: S slot weights and
+    log(sigma^2 / sill)."""
+    h = bf16(mlp_features(row, masks, S))
     layers = mlp_unpack(params)
     for li, (W, b) in enumerate(layers):
         z = h @ bf16(W).T + b
         h = bf16(gelu_tanh(z)) if li < 3 else z
-    return h[:, :25]
+    return h[:, :S + 1]
 
 
-def grid_slots(case: SeqsimCase, k=24):
+def grid_slots(case: SeqsimCase, k=MLP_SLOTS):
     """The first k grid offsets (cells) of the search's offset table: squared
     distance, then linear index delta. The zero offset (the node itself, a
     candidate of the k-nearest-then-mask search that the mask always drops)
@@ -703,7 +725,7 @@ def grid_slots(case: SeqsimCase, k=24):
     nd = case.ndim
     nx = case.dims[0]
     ny = case.dims[1] if nd > 1 else 1
-    r = 3
+    r = {1: 2 * k, 2: 5, 3: 3}[nd]
     rng = [range(-r, r + 1) if d < nd else range(0, 1) for d in range(3)]
     offs = [(dx, dy, dz) for dz in rng[2] for dy in rng[1] for dx in rng[0]]
     sp = list(case.spacing) + [1.0] * (3 - nd)

@@ -57,9 +57,16 @@ def test_training_reduces_error(surrogate):
     params, slots = surrogate.export()
     assert params.size == 181056
     W = O.mlp_unpack(params)
+    # maxneighbors 24 only: the 24-slot network 73 -> 256 -> 256 -> 256 -> 25
     assert np.all(W[0][0][:, 73:] == 0) and np.all(W[3][0][25:] == 0) and np.all(W[3][1][25:] == 0)
-    expect = O.grid_slots(c2_case(0)) / 20.0
-    np.testing.assert_allclose(slots, expect, rtol=0, atol=1e-12)
+    assert np.any(W[0][0][:, 72] != 0) and np.any(W[3][0][24] != 0)
+    np.testing.assert_array_equal(slots, O.grid_slots(c2_case(0), 24))
+    info = surrogate.info
+    assert info["family"] == O.EXPONENTIAL and info["ndim"] == 3 and info["n_slots"] == 24
+    assert info["n_train_requests"] == 6
+    assert info["nugget_ratio"] == (0.01, 0.01)
+    for d in range(3):
+        assert info["scale"][d] == (1 / 20.0, 1 / 20.0)
 
 
 def check_learned_plan(ctx, surrogate, case, tol=0.05):
@@ -79,9 +86,10 @@ def check_learned_plan(ctx, surrogate, case, tol=0.05):
     # learned nodes: the forward pass of the exported parameters
     params, slots = surrogate.export()
     nodes = np.flatnonzero(learned)
-    masks, nslot = O.node_slot_masks(case, o, nodes, O.grid_slots(case))
+    S = len(slots)
+    masks, nslot = O.node_slot_masks(case, o, nodes, O.grid_slots(case, S))
     assert (masks >= 0).all()
-    out = O.mlp_forward(params, slots, masks)
+    out = O.mlp_forward(params, O.mlp_request_row(case, slots), masks, S)
     pred = np.zeros_like(g["lam"][nodes])
     for j in range(o.kmax):
         ok = nslot[:, j] >= 0
@@ -90,11 +98,13 @@ def check_learned_plan(ctx, surrogate, case, tol=0.05):
     say(f"learned {len(nodes)} of {case.n_nodes}; |gpu - numpy forward| max {diff.max():.2e} "
         f"mean {diff.mean():.2e}")
     assert diff.max() <= 5e-3 and diff.mean() <= 1e-4
-    np.testing.assert_allclose(g["sigma"][nodes] ** 2, np.exp(out[:, 24].astype(np.float64)),
+    np.testing.assert_allclose(g["sigma"][nodes] ** 2, np.exp(out[:, S].astype(np.float64)),
                                rtol=5e-3)
     # the acceptance test on the exported weights: the implied variance
     # sill - sum_j lambda_j C(x_j, x_0) is within tol of the predicted one
-    cs = np.array([O.cov_u(case.family, 1.0, case.nugget, np.linalg.norm(s)) for s in slots])
+    scale = np.asarray(case.spacing, float) / np.asarray(case.ranges, float)
+    cs = np.array([O.cov_u(case.family, 1.0, case.nugget, np.linalg.norm(s[:case.ndim] * scale))
+                   for s in slots])
     lam = g["lam"][nodes]
     cj = np.where(nslot >= 0, cs[np.maximum(nslot, 0)], 0.0)
     chk = 1.0 - (lam * cj).sum(1)
@@ -194,3 +204,111 @@ def test_surrogate_errors(ctx):
     pm = _lib.SeqsimPlan(ctx, prob, keep=(m,))
     with pytest.raises(_lib.UnsupportedError):
         _lib.Surrogate(ctx, [pm])
+
+
+# ---- one surrogate over spans of ranges, nuggets and maxneighbors ----------
+def mixed_case(seed, family=O.EXPONENTIAL, rang=None, nugget=None, kmax=None, side=None, data=0):
+    """A 2D request like the mixed-batch workload's: range = search radius,
+    the k nearest then mask, FP32 + refinement."""
+    rng = np.random.default_rng(seed)
+    side = side if side is not None else int(rng.integers(40, 64))
+    rang = rang if rang is not None else float(rng.uniform(6.0, 20.0))
+    nugget = nugget if nugget is not None else float(rng.uniform(0.0, 0.1))
+    kmax = kmax if kmax is not None else int(rng.choice([8, 16, 32]))
+    N = side * side
+    return O.SeqsimCase(dims=(side, side), family=family, sill=1.0, ranges=(rang,), nugget=nugget,
+                        kmax=kmax, radius=rang, path=rng.permutation(N), precision=O.FP32_REFINE,
+                        data_coords=rng.uniform(0, side, (data, 2)),
+                        data_vals=rng.standard_normal(data))
+
+
+@pytest.fixture(scope="module")
+def span_surrogates(ctx):
+    """An exponential and a spherical surrogate, each trained on 12 requests
+    with ranges spread over [6, 20], nuggets over [0, 0.1] and maxneighbors
+    8/16/32."""
+    out = {}
+    for fam, base in ((O.EXPONENTIAL, 300), (O.SPHERICAL, 400)):
+        nug = np.random.default_rng(base).permutation(12) / 110.0
+        cases = [mixed_case(base + s, family=fam, rang=6.0 + 14.0 * s / 11, nugget=float(nug[s]),
+                            kmax=(8, 16, 32)[s % 3], side=96) for s in range(12)]
+        plan = _lib.SeqsimPlan(ctx, [gpu_problem(c) for c in cases], keep=tuple(cases))
+        s = _lib.Surrogate(ctx, [plan], max_samples=300_000, epochs=12, batch=1024, lr=2e-3, seed=4)
+        say(f"family {fam}: trained on {s.report.n_samples} systems, rms weight "
+            f"{s.report.rms_weight_last_epoch:.3e}")
+        out[fam] = (s, cases)
+    return out
+
+
+def test_span_surrogate_domain(span_surrogates):
+    for fam, (s, cases) in span_surrogates.items():
+        info = s.info
+        assert info["family"] == fam and info["ndim"] == 2 and info["n_train_requests"] == 12
+        assert info["n_slots"] == 32
+        nug = [c.nugget for c in cases]
+        assert info["nugget_ratio"] == (min(nug), max(nug)) == (0.0, 0.1)
+        sc = [1.0 / c.ranges[0] for c in cases]
+        for d in range(2):
+            assert info["scale"][d][0] == pytest.approx(min(sc), rel=1e-15)
+            assert info["scale"][d][1] == pytest.approx(max(sc), rel=1e-15)
+
+
+def test_learned_multi_batch(ctx, span_surrogates):
+    """One batch, two surrogates: exponential and spherical requests inside
+    the trained spans are learned by their family's surrogate (k = 8, 16 and
+    32; one with data), requests outside a span (range 3, nugget 0.3) and of
+    another family (gaussian) are solved exactly. Exact nodes equal the
+    oracle bit for bit; learned nodes equal the numpy forward pass of their
+    surrogate on their request's input row, and pass the variance check."""
+    exp_s, _ = span_surrogates[O.EXPONENTIAL]
+    sph_s, _ = span_surrogates[O.SPHERICAL]
+    cases = [mixed_case(501, rang=7.5, nugget=0.02, kmax=8),
+             mixed_case(502, rang=11.0, nugget=0.05, kmax=16),
+             mixed_case(503, rang=17.0, nugget=0.08, kmax=32),
+             mixed_case(504, family=O.SPHERICAL, rang=13.0, nugget=0.04),
+             mixed_case(505, rang=9.0, nugget=0.06, data=30),
+             mixed_case(506, rang=3.0), mixed_case(507, rang=10.0, nugget=0.3),
+             mixed_case(508, family=O.GAUSSIAN)]
+    expect_learned = [True, True, True, True, True, False, False, False]
+    plan = _lib.SeqsimPlan(ctx, [gpu_problem(c) for c in cases], keep=tuple(cases),
+                           surrogate=[exp_s, sph_s])
+    g = plan.export()
+    from helpers import local_plan
+    locs = [local_plan(g, q, plan, cases) for q in range(len(cases))]
+    frac = [float(((lc["flags"] & O.FLAG_LEARNED) != 0).mean()) for lc in locs]
+    for q, c in enumerate(cases):
+        say(f"request {q}: range {c.ranges[0]:.1f} nugget {c.nugget:.3f} k {c.kmax} family "
+            f"{c.family}: learned {frac[q]:.3f}")
+    assert [f > 0 for f in frac] == expect_learned
+    tot = 0
+    for q, c in enumerate(cases):
+        o = O.seqsim_plan(c)
+        loc = locs[q]
+        learned = (loc["flags"] & O.FLAG_LEARNED) != 0
+        tot += int(learned.sum())
+        np.testing.assert_array_equal(loc["nfound"], o.nfound)
+        np.testing.assert_array_equal(loc["nbr"], o.nbr)
+        ex = ~learned
+        np.testing.assert_array_equal(loc["flags"][ex], o.flags[ex])
+        np.testing.assert_array_equal(loc["lam"][ex], o.lam[ex])
+        np.testing.assert_array_equal(loc["sigma"][ex], o.sigma[ex])
+        if not learned.any():
+            continue
+        assert not (loc["nbr"][learned] >= c.n_nodes).any()
+        s = exp_s if c.family == O.EXPONENTIAL else sph_s
+        params, slots = s.export()
+        nodes = np.flatnonzero(learned)
+        S = len(slots)
+        assert S == 32
+        masks, nslot = O.node_slot_masks(c, o, nodes, O.grid_slots(c, S))
+        out = O.mlp_forward(params, O.mlp_request_row(c, slots), masks, S)
+        pred = np.zeros((len(nodes), o.kmax))
+        for j in range(o.kmax):
+            ok = nslot[:, j] >= 0
+            pred[ok, j] = out[ok, nslot[ok, j]]
+        diff = np.abs(loc["lam"][nodes] - pred)
+        assert diff.max() <= 5e-3 and diff.mean() <= 1e-4, (q, diff.max(), diff.mean())
+        err = (loc["lam"] - o.lam)[learned][o.nbr[learned] >= 0]
+        say(f"   |gpu - numpy| max {diff.max():.1e}; RMS weight error vs exact "
+            f"{np.sqrt(np.mean(err ** 2)):.2e}")
+    assert plan.info.n_learned == tot
