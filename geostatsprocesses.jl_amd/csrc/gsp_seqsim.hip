@@ -154,21 +154,26 @@ __device__ inline double assemble(KrigeLds<KB, PREC>& L, const DevModel& M, int 
 
 // Simple-kriging solve, one matrix row per lane. Same operation order as the
 // oracle: right-looking Cholesky whose per-element subtraction order is
-// ascending, forward substitution ascending, back substitution descending.
-// Expects L.A assembled; leaves the factor in L.A.
+// ascending, each column scaled by the reciprocal of its pivot (which the
+// lane of that row keeps for both substitutions), forward substitution
+// ascending, back substitution descending. Expects L.A assembled; leaves the
+// factor in L.A.
 template <int KB, int PREC>
 __device__ inline int krige_fp64(KrigeLds<KB, PREC>& L, int n, int lane, double c, double C0,
                                  double& lam_out, double& sigma_out) {
   double a[KB];
 #pragma unroll
   for (int m = 0; m < KB; ++m) a[m] = (m < n && lane < n) ? L.A[lane][m] : 0.0;
+  double dinv = 0.0;  // 1 / L[lane][lane]
 #pragma unroll
   for (int col = 0; col < KB; ++col) {
     if (col < n) {
       const double piv = rl(a[col], col);
       if (!(piv > 0.0)) return GSP_FLAG_FAIL;
       const double lcc = __builtin_sqrt(piv);
-      a[col] = lane == col ? lcc : (lane > col ? a[col] / lcc : a[col]);
+      const double rinv = 1.0 / lcc;
+      if (lane == col) dinv = rinv;
+      a[col] = lane == col ? lcc : (lane > col ? a[col] * rinv : a[col]);
 #pragma unroll
       for (int m = col + 1; m < KB; ++m) {
         if (m < n) {
@@ -182,7 +187,7 @@ __device__ inline int krige_fp64(KrigeLds<KB, PREC>& L, int n, int lane, double 
 #pragma unroll
   for (int m = 0; m < KB; ++m) {
     if (m < n) {
-      const double ym = rl(s, m) / rl(a[m], m);
+      const double ym = rl(s * dinv, m);
       if (lane == m) y = ym;
       if (lane > m) s = s - a[m] * ym;
     }
@@ -196,7 +201,7 @@ __device__ inline int krige_fp64(KrigeLds<KB, PREC>& L, int n, int lane, double 
   wave_sync();
   double s2 = y, lam = 0.0;
   for (int m = n - 1; m >= 0; --m) {
-    const double lm = rl(s2, m) / L.A[m][m];
+    const double lm = rl(s2 * dinv, m);
     if (lane == m) lam = lm;
     if (lane < m) s2 = s2 - L.A[m][lane] * lm;
   }
@@ -220,6 +225,7 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
 #pragma unroll
   for (int m = 0; m < KB; ++m) f[m] = (m < n && lane < n) ? (float)L.A[lane][m] : 0.0f;
   bool ok = true;
+  float dinv = 0.0f;  // 1 / L[lane][lane]
 #pragma unroll
   for (int col = 0; col < KB; ++col) {
     if (ok && col < n) {
@@ -228,7 +234,9 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
         ok = false;
       } else {
         const float lcc = __builtin_sqrtf(piv);
-        f[col] = lane == col ? lcc : (lane > col ? f[col] / lcc : f[col]);
+        const float rinv = 1.0f / lcc;
+        if (lane == col) dinv = rinv;
+        f[col] = lane == col ? lcc : (lane > col ? f[col] * rinv : f[col]);
 #pragma unroll
         for (int m = col + 1; m < KB; ++m) {
           if (m < n) {
@@ -255,14 +263,14 @@ __device__ inline int krige_fp32r(KrigeLds<KB, GSP_FP32_REFINE>& L, int n, int l
 #pragma unroll
     for (int m = 0; m < KB; ++m) {
       if (m < n) {
-        const float ym = rlf(s, m) / rlf(f[m], m);
+        const float ym = rlf(s * dinv, m);
         if (lane == m) y = ym;
         if (lane > m) s = s - f[m] * ym;
       }
     }
     float s2 = y, l = 0.0f;
     for (int m = n - 1; m >= 0; --m) {
-      const float lm = rlf(s2, m) / L.F[m][m];
+      const float lm = rlf(s2 * dinv, m);
       if (lane == m) l = lm;
       if (lane < m) s2 = s2 - L.F[m][lane] * lm;
     }
@@ -743,13 +751,16 @@ template <int PREC>
 __device__ inline bool packed_fp64(Krige16Lds<PREC>& L, int g, int r, int nmax, double c, double C0,
                                    double* a, double& lam_out, double& sigma_out) {
   bool fail = false;
+  double dinv = 0.0;  // 1 / L[r][r]
   static_for<0, 16>([&](auto colc) {
     constexpr int col = decltype(colc)::value;
     if (col < nmax) {
       const double piv = rbd<col>(a[col]);
       fail = fail || !(piv > 0.0);
       const double lcc = __builtin_sqrt(piv);
-      a[col] = r == col ? lcc : (r > col ? a[col] / lcc : a[col]);
+      const double rinv = 1.0 / lcc;
+      if (r == col) dinv = rinv;
+      a[col] = r == col ? lcc : (r > col ? a[col] * rinv : a[col]);
       static_for<col + 1, 16>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         if (m < nmax) {
@@ -763,7 +774,7 @@ __device__ inline bool packed_fp64(Krige16Lds<PREC>& L, int g, int r, int nmax, 
   static_for<0, 16>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     if (m < nmax) {
-      const double ym = rbd<m>(s) / rbd<m>(a[m]);
+      const double ym = rbd<m>(s * dinv);
       if (r == m) y = ym;
       if (r > m) s = s - a[m] * ym;
     }
@@ -778,7 +789,7 @@ __device__ inline bool packed_fp64(Krige16Lds<PREC>& L, int g, int r, int nmax, 
   static_for_down<16>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     if (m < nmax) {
-      const double lm = rbd<m>(s2) / L.A[g][m][m];
+      const double lm = rbd<m>(s2 * dinv);
       if (r == m) lam = lm;
       if (r < m) s2 = s2 - L.A[g][m][r] * lm;
     }
@@ -871,13 +882,16 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
           f[m] = (float)a[m];
         });
         bool f32fail = false;
+        float dinv = 0.0f;  // 1 / L[r][r]
         static_for<0, 16>([&](auto colc) {
           constexpr int col = decltype(colc)::value;
           if (col < nmax) {
             const float piv = rbf<col>(f[col]);
             f32fail = f32fail || !(piv > tau);
             const float lcc = __builtin_sqrtf(piv);
-            f[col] = r == col ? lcc : (r > col ? f[col] / lcc : f[col]);
+            const float rinv = 1.0f / lcc;
+            if (r == col) dinv = rinv;
+            f[col] = r == col ? lcc : (r > col ? f[col] * rinv : f[col]);
             static_for<col + 1, 16>([&](auto mc) {
               constexpr int m = decltype(mc)::value;
               if (m < nmax) {
@@ -900,7 +914,7 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
           static_for<0, 16>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if (m < nmax) {
-              const float ym = rbf<m>(s) / rbf<m>(f[m]);
+              const float ym = rbf<m>(s * dinv);
               if (r == m) y = ym;
               if (r > m) s = s - f[m] * ym;
             }
@@ -910,7 +924,7 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
           static_for_down<16>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if (m < nmax) {
-              const float lm = rbf<m>(s2) / L.F[g][m][m];
+              const float lm = rbf<m>(s2 * dinv);
               if (r == m) l = lm;
               if (r < m) s2 = s2 - L.F[g][m][r] * lm;
             }
@@ -1014,13 +1028,16 @@ template <int KB, int PREC>
 __device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax, double c,
                                      double C0, double* a, double& lam_out, double& sigma_out) {
   bool fail = false;
+  double dinv = 0.0;  // 1 / L[r][r]
   static_for<0, KB>([&](auto colc) {
     constexpr int col = decltype(colc)::value;
     if (col < nmax) {
       const double piv = half_lane(a[col], col, g);
       fail = fail || !(piv > 0.0);
       const double lcc = __builtin_sqrt(piv);
-      a[col] = r == col ? lcc : (r > col ? a[col] / lcc : a[col]);
+      const double rinv = 1.0 / lcc;
+      if (r == col) dinv = rinv;
+      a[col] = r == col ? lcc : (r > col ? a[col] * rinv : a[col]);
       L.col[g][r] = a[col];
       wave_sync();
       static_for<col + 1, KB>([&](auto mc) {
@@ -1038,7 +1055,7 @@ __device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax
   static_for<0, KB>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     if (m < nmax) {
-      const double ym = half_lane(s, m, g) / half_lane(a[m], m, g);
+      const double ym = half_lane(s * dinv, m, g);
       if (r == m) y = ym;
       if (r > m) s = s - a[m] * ym;
     }
@@ -1053,7 +1070,7 @@ __device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax
   static_for_down<KB>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     if (m < nmax) {
-      const double lm = half_lane(s2, m, g) / L.A[g][m][m];
+      const double lm = half_lane(s2 * dinv, m, g);
       if (r == m) lam = lm;
       if (r < m) s2 = s2 - L.A[g][m][r] * lm;
     }
@@ -1142,6 +1159,7 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
           f[m] = (float)a[m];
         });
         bool f32fail = false;
+        float dinv = 0.0f;  // 1 / L[r][r]
         static_for<0, KB>([&](auto colc) {
           constexpr int col = decltype(colc)::value;
           if (col < nmax) {
@@ -1150,7 +1168,9 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
                          : __builtin_amdgcn_readlane(__builtin_bit_cast(int, f[col]), col));
             f32fail = f32fail || !(piv > tau);
             const float lcc = __builtin_sqrtf(piv);
-            f[col] = r == col ? lcc : (r > col ? f[col] / lcc : f[col]);
+            const float rinv = 1.0f / lcc;
+            if (r == col) dinv = rinv;
+            f[col] = r == col ? lcc : (r > col ? f[col] * rinv : f[col]);
             L.colf[g][r] = f[col];
             wave_sync();
             static_for<col + 1, KB>([&](auto mc) {
@@ -1175,7 +1195,7 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
           static_for<0, KB>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if (m < nmax) {
-              const float ym = half_lane_f(sv, m, g) / half_lane_f(f[m], m, g);
+              const float ym = half_lane_f(sv * dinv, m, g);
               if (r == m) y = ym;
               if (r > m) sv = sv - f[m] * ym;
             }
@@ -1185,7 +1205,7 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
           static_for_down<KB>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             if (m < nmax) {
-              const float lm = half_lane_f(s2, m, g) / L.F[g][m][m];
+              const float lm = half_lane_f(s2 * dinv, m, g);
               if (r == m) l = lm;
               if (r < m) s2 = s2 - L.F[g][m][r] * lm;
             }

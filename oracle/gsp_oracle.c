@@ -416,9 +416,11 @@ int32_t or_search_bucket(const or_seqsim_problem* p, void* vb, int32_t nmax,
  * status is false when a pivot is not positive (LAPACK potrf failure); the
  * caller then draws from the prior Normal(mu, sqrt(sill)) (seqsim.jl:127-131).
  * Operation order (fixed, shared with the device code):
- *   L_jm = (A_jm - L_j0 L_m0 - L_j1 L_m1 - ...) / L_mm, L_mm = sqrt(pivot)
- *   forward  y_j = (c_j - L_j0 y_0 - ... - L_j,j-1 y_j-1) / L_jj
- *   backward l_j = (y_j - L_n-1,j l_n-1 - ... - L_j+1,j l_j+1) / L_jj
+ *   L_mm = sqrt(pivot), r_m = 1 / L_mm (the column is scaled by the
+ *   reciprocal of its pivot, as LAPACK's unblocked dpotf2 does),
+ *   L_jm = (A_jm - L_j0 L_m0 - L_j1 L_m1 - ...) * r_m
+ *   forward  y_j = (c_j - L_j0 y_0 - ... - L_j,j-1 y_j-1) * r_j
+ *   backward l_j = (y_j - L_n-1,j l_n-1 - ... - L_j+1,j l_j+1) * r_j
  *   var = C0 - (l_0 c_0 + l_1 c_1 + ...)
  */
 #define OR_KMAX 64
@@ -429,11 +431,12 @@ static int or_chol64(int n, double* A /* n x OR_KMAX, lower */) {
     for (int c = 0; c < m; ++c) s = s - A[m * OR_KMAX + c] * A[m * OR_KMAX + c];
     if (!(s > 0.0)) return 0;
     double lmm = sqrt(s);
+    double rinv = 1.0 / lmm;
     A[m * OR_KMAX + m] = lmm;
     for (int j = m + 1; j < n; ++j) {
       double v = A[j * OR_KMAX + m];
       for (int c = 0; c < m; ++c) v = v - A[j * OR_KMAX + c] * A[m * OR_KMAX + c];
-      A[j * OR_KMAX + m] = v / lmm;
+      A[j * OR_KMAX + m] = v * rinv;
     }
   }
   return 1;
@@ -444,12 +447,12 @@ static void or_solve64(int n, const double* L, const double* c, double* lam) {
   for (int j = 0; j < n; ++j) {
     double s = c[j];
     for (int m = 0; m < j; ++m) s = s - L[j * OR_KMAX + m] * y[m];
-    y[j] = s / L[j * OR_KMAX + j];
+    y[j] = s * (1.0 / L[j * OR_KMAX + j]);
   }
   for (int j = n - 1; j >= 0; --j) {
     double s = y[j];
     for (int m = n - 1; m > j; --m) s = s - L[m * OR_KMAX + j] * lam[m];
-    lam[j] = s / L[j * OR_KMAX + j];
+    lam[j] = s * (1.0 / L[j * OR_KMAX + j]);
   }
 }
 
@@ -460,11 +463,12 @@ static int or_chol32(int n, float* A, float tau) {
     for (int c = 0; c < m; ++c) s = s - A[m * OR_KMAX + c] * A[m * OR_KMAX + c];
     if (!(s > tau)) return 0;
     float lmm = sqrtf(s);
+    float rinv = 1.0f / lmm;
     A[m * OR_KMAX + m] = lmm;
     for (int j = m + 1; j < n; ++j) {
       float v = A[j * OR_KMAX + m];
       for (int c = 0; c < m; ++c) v = v - A[j * OR_KMAX + c] * A[m * OR_KMAX + c];
-      A[j * OR_KMAX + m] = v / lmm;
+      A[j * OR_KMAX + m] = v * rinv;
     }
   }
   return 1;
@@ -475,12 +479,12 @@ static void or_solve32(int n, const float* L, const float* c, float* lam) {
   for (int j = 0; j < n; ++j) {
     float s = c[j];
     for (int m = 0; m < j; ++m) s = s - L[j * OR_KMAX + m] * y[m];
-    y[j] = s / L[j * OR_KMAX + j];
+    y[j] = s * (1.0f / L[j * OR_KMAX + j]);
   }
   for (int j = n - 1; j >= 0; --j) {
     float s = y[j];
     for (int m = n - 1; m > j; --m) s = s - L[m * OR_KMAX + j] * lam[m];
-    lam[j] = s / L[j * OR_KMAX + j];
+    lam[j] = s * (1.0f / L[j * OR_KMAX + j]);
   }
 }
 
