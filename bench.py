@@ -33,8 +33,9 @@ realizations each, FP32 solves with refinement. One step is one batched plan
 plus the batched sweep.
 
 Workload "seqsim-batch-512-learned": the same 512 requests with learned
-weights: one surrogate per variogram family, trained before timing on the
-exact plans of 3 x 48 other requests of the same generator; each request
+weights: one surrogate per variogram family, trained before timing on up to
+4M systems (20 epochs) of the exact plans of 3 x 96 other requests of the same
+generator; each request
 inside its family surrogate's trained spans of range and nugget takes learned
 weights where the variance check passes, the rest are solved exactly. The
 exact batch is timed beside it; the accuracy (RMS weight error against exact
@@ -104,7 +105,7 @@ WORKLOADS = {
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
     "seqsim-batch-512-learned": dict(
         batch=512, side=(32, 512), R=8, seed=1005,
-        learned=dict(train_requests=48, train_seed=3005, samples=2_000_000, epochs=12, batch=1024,
+        learned=dict(train_requests=96, train_seed=3005, samples=4_000_000, epochs=20, batch=1024,
                      lr=2e-3, tolerance=0.05, accuracy_requests=32)),
     "seqsim-learned-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,

@@ -282,7 +282,7 @@ struct DevModel {
   int32_t pad_;
   double sill;
   double nugget;
-  double rs[3];   // scaled ranges
+  double irs[3];  // reciprocals of the scaled ranges
   double rot[9];  // row-major R
 };
 
@@ -311,7 +311,8 @@ __host__ __device__ inline double cov_u(const DevModel& m, double u) {
   return s - gamma;
 }
 
-// u = |diag(1/rs) R^T (a - b)|
+// u = |diag(1/rs) R^T (a - b)|, rs the scaled ranges (multiplied by their
+// reciprocals, computed once per model)
 __host__ __device__ inline double lag(const DevModel& m, double a0, double a1, double a2,
                                       double b0, double b1, double b2) {
   const int nd = m.ndim;
@@ -328,15 +329,15 @@ __host__ __device__ inline double lag(const DevModel& m, double a0, double a1, d
     dp1 = nd > 1 ? dp[1] : 0.0;
     dp2 = nd > 2 ? dp[2] : 0.0;
   }
-  double t = dp0 / m.rs[0];
+  double t = dp0 * m.irs[0];
   double u2 = 0.0;
   u2 = u2 + t * t;
   if (nd > 1) {
-    t = dp1 / m.rs[1];
+    t = dp1 * m.irs[1];
     u2 = u2 + t * t;
   }
   if (nd > 2) {
-    t = dp2 / m.rs[2];
+    t = dp2 * m.irs[2];
     u2 = u2 + t * t;
   }
   return __builtin_sqrt(u2);

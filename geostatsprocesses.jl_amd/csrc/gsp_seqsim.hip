@@ -2143,7 +2143,7 @@ DevModel make_dev_model(const gsp_model& m, int ndim, double alpha) {
   M.has_rotation = m.has_rotation ? 1 : 0;
   M.sill = m.sill;
   M.nugget = m.nugget;
-  for (int d = 0; d < 3; ++d) M.rs[d] = d < ndim ? alpha * m.ranges[d] : 1.0;
+  for (int d = 0; d < 3; ++d) M.irs[d] = d < ndim ? 1.0 / (alpha * m.ranges[d]) : 1.0;
   for (int q = 0; q < 9; ++q) M.rot[q] = m.rotation[q];
   return M;
 }

@@ -98,7 +98,9 @@ double or_cov_u(const or_model* m, double u) {
 /* Normalised lag u between two points in scaled coordinates. The metric is the
  * ellipsoidal one of a MetricBall with radii `ranges` and rotation R:
  * u = |diag(1/r) R^T (a - b)| (unpinned: the rotation convention). Ranges are
- * multiplied by alpha, as GeoStatsFunctions.scale(f, alpha) does. */
+ * multiplied by alpha, as GeoStatsFunctions.scale(f, alpha) does; each lag
+ * component is multiplied by the reciprocal of its scaled range (operation
+ * order shared with the device code). */
 static double or_lag(const or_model* m, int ndim, double alpha, const double* a,
                      const double* b) {
   double dl[3], dp[3];
@@ -114,7 +116,8 @@ static double or_lag(const or_model* m, int ndim, double alpha, const double* a,
   }
   double u2 = 0.0;
   for (int e = 0; e < ndim; ++e) {
-    double t = dp[e] / (alpha * m->ranges[e]);
+    double ir = 1.0 / (alpha * m->ranges[e]);
+    double t = dp[e] * ir;
     u2 = u2 + t * t;
   }
   return sqrt(u2);
