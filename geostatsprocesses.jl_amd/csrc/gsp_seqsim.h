@@ -59,6 +59,33 @@ struct NodeOut {
   int32_t* counters;
 };
 
+// Kernel arguments of the plan builder (search, kriging, degree counts).
+struct PlanArgs {
+  int32_t NT;  // nodes of all requests; data ids start here
+  int32_t K;   // row stride of nbr / lam (largest kmax)
+  const Req* reqs;
+  const int32_t* node_req;  // request of every node
+  const int64_t* path;      // (request << 32) | global node id, in global path order
+  const int32_t* pos;       // global path position of every node
+  const Offset* table;
+  const double* data;       // n_data_total x 3, physical
+  const int32_t* hstart;    // absolute item index of every hash cell
+  const DataItem* hitems;
+  int32_t* nfound;
+  int32_t* nbr;
+  uint8_t* nbr_slot;  // optional: table index of each grid neighbour (255: datum or >= 255)
+  double* lam;
+  double* sigma;
+  uint8_t* flags;
+  int32_t* counters;
+  // optional: the dependency degrees, counted by the search as it writes each
+  // node's neighbours (as the degree count of the level build does)
+  int32_t* outdeg;
+  int32_t* indeg;
+  uint8_t* mark;
+};
+
+
 enum {
   CTR_PRIOR = 2,
   CTR_FALLBACK = 3,
