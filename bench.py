@@ -938,7 +938,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="seqsim-learned-256x256x128", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default=None, choices=["fp32r", "fp64"],
-                    help="kriging solves (default: fp64 for the posterior mean, fp32r otherwise)")
+                    help="kriging solves (default: the workload's: fp64 for the 100x100 SEQSIM and the "
+                         "posterior mean, fp32r otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the learned-vs-exact accuracy pass")
     ap.add_argument("--epochs", type=int, default=None,
