@@ -450,8 +450,13 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
 template <int PREC>
 struct Krige16Lds {
   double x[4][16][3];
-  double A[4][16][17];
-  float F[PREC == GSP_FP32_REFINE ? 4 : 1][16][17];
+  // the assembled matrix (read into registers before the FP32 factor is
+  // stored), the FP32 factor, and in the FP64 fallback the FP64 factor: one
+  // buffer, the phases separated by wave_sync
+  union {
+    double A[4][16][17];
+    float F[PREC == GSP_FP32_REFINE ? 4 : 1][16][17];
+  };
   DevModel M[4];
 };
 
@@ -711,12 +716,14 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
 // that owns them and broadcast the same way. Padding to the larger of the two
 // sizes with sill * identity rows and the operation order are those of
 // k_krige16 and the one-system kernels (bitwise equal results).
-template <int PREC>
+// Sized by the bucket's largest system KB (LDS limits the resident
+// wavefronts); rows r >= KB of a half hold no system row.
+template <int KB, int PREC>
 struct Krige32Lds {
-  double x[2][32][3];
+  double x[2][KB][3];
   union {
-    double A[2][32][33];  // assembled matrix, later the FP64 factor
-    float F[2][32][33];   // FP32 factor (the FP64 rows stay in registers)
+    double A[2][KB][KB + 1];  // assembled matrix, later the FP64 factor
+    float F[2][KB][KB + 1];   // FP32 factor (the FP64 rows stay in registers)
   };
   double col[2][32];      // broadcast slots
   float colf[2][32];
@@ -737,7 +744,7 @@ __device__ inline float half_lane_f(float x, int m, int g) {
 // FP64 Cholesky and substitutions of the two systems (rows in a[]); true in
 // the lanes of a system whose factorisation met a non-positive pivot.
 template <int KB, int PREC>
-__device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax, double c,
+__device__ inline bool packed32_fp64(Krige32Lds<KB, PREC>& L, int g, int r, int nmax, double c,
                                      double C0, double* a, double& lam_out, double& sigma_out) {
   bool fail = false;
   double dinv = 0.0;  // 1 / L[r][r]
@@ -774,7 +781,7 @@ __device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax
   });
   static_for<0, KB>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    if (m <= r) L.A[g][r][m] = a[m];
+    if (m <= r && r < KB) L.A[g][r][m] = a[m];
   });
   wave_sync();
   // back: lambda_m = s2_m / L_mm, s2_r -= L_mr lambda_m above it
@@ -784,7 +791,7 @@ __device__ inline bool packed32_fp64(Krige32Lds<PREC>& L, int g, int r, int nmax
     if (m < nmax) {
       const double lm = half_lane(s2 * dinv, m, g);
       if (r == m) lam = lm;
-      if (r < m) s2 = s2 - L.A[g][m][r] * lm;
+      if (r < m) s2 = s2 - L.A[g][m][r < KB ? r : 0] * lm;
     }
   });
   // sigma^2 = C0 - sum_j lambda_j c_j in ascending j
@@ -805,11 +812,11 @@ constexpr int k32Waves = 1;
 template <int KB, int PREC>
 __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, const int32_t* list,
                                                          int32_t count) {
-  __shared__ Krige32Lds<PREC> s_lds[k32Waves];
+  __shared__ Krige32Lds<KB, PREC> s_lds[k32Waves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int g = lane >> 5, r = lane & 31;
-  Krige32Lds<PREC>& L = s_lds[w];
+  Krige32Lds<KB, PREC>& L = s_lds[w];
   const int64_t stride = (int64_t)gridDim.x * k32Waves * 2;
   for (int64_t base = ((int64_t)blockIdx.x * k32Waves + w) * 2; base < count; base += stride) {
     const int64_t slot = base + g;
@@ -897,7 +904,7 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
         });
         static_for<0, KB>([&](auto mc) {
           constexpr int m = decltype(mc)::value;
-          if (m <= r) L.F[g][r][m] = f[m];
+          if (m <= r && r < KB) L.F[g][r][m] = f[m];
         });
         wave_sync();
         float rhs = (float)c, sol = 0.0f, l = 0.0f;
@@ -919,7 +926,7 @@ __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, con
             if (m < nmax) {
               const float lm = half_lane_f(s2 * dinv, m, g);
               if (r == m) l = lm;
-              if (r < m) s2 = s2 - L.F[g][m][r] * lm;
+              if (r < m) s2 = s2 - L.F[g][m][r < KB ? r : 0] * lm;
             }
           });
           wave_sync();
