@@ -6,14 +6,6 @@
 
 namespace gsp {
 
-// A conditioning datum in a hash-cell list: physical coordinates inline, `id`
-// its global data index.
-struct DataItem {
-  double x[3];
-  int32_t id;
-  int32_t pad_;
-};
-
 // One request of a plan, as the kernels see it.
 struct Req {
   int32_t ndim, mode, kmin, kmax;
@@ -70,7 +62,7 @@ struct PlanArgs {
   const Offset* table;
   const double* data;       // n_data_total x 3, physical
   const int32_t* hstart;    // absolute item index of every hash cell
-  const DataItem* hitems;
+  const int32_t* hitems;    // hash-cell lists: global data index of every entry
   int32_t* nfound;
   int32_t* nbr;
   uint8_t* nbr_slot;  // optional: table index of each grid neighbour (255: datum or >= 255)

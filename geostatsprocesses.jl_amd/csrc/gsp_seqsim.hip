@@ -49,15 +49,16 @@ namespace {
 
 constexpr int kWaves = 4;  // wavefronts per 256-thread block
 
-// Squared physical distance between a datum and the centre of node t.
+// Squared physical distance between a datum (coordinates x) and the centre
+// of node t.
 template <class RQ>
-__device__ inline double data_d2(const RQ& R, const DataItem& it, const int* t) {
+__device__ inline double data_d2(const RQ& R, const double* x, const int* t) {
   double acc = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d >= R.ndim) break;
     double c = R.org[d] + ((double)t[d] + 0.5) * R.sp[d];
-    double u = it.x[d] - c;
+    double u = x[d] - c;
     acc = acc + u * u;
   }
   return acc;
@@ -212,9 +213,8 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
         int di = 0;
         double d2 = 0.0;
         if (valid) {
-          const DataItem it = A.hitems[idx];
-          di = it.id;
-          d2 = data_d2(R, it, ti);
+          di = A.hitems[idx];
+          d2 = data_d2(R, A.data + (size_t)di * 3, ti);
           ok = d2 <= lim && d2 <= R.r2;
         }
         const uint64_t b = __ballot(ok);
@@ -1740,7 +1740,7 @@ struct HostReq {
   int hexp = 0;
   int64_t hB = 1;
   std::vector<int32_t> hstart;
-  std::vector<DataItem> hitems;
+  std::vector<int32_t> hitems;  // request-local data indices
 };
 
 // Data hash. Expanded mode: cells of B^ndim grid cells, each listing every
@@ -1760,13 +1760,6 @@ void build_hash(const gsp_grid& g, int mode, HostReq& h) {
   const bool bounded = !h.table.empty() && (mode == GSP_KNN_THEN_MASK || h.r2 < inf);
   double smin = g.spacing[0];
   for (int d = 1; d < nd; ++d) smin = std::min(smin, g.spacing[d]);
-  auto make_item = [&](int64_t i) {
-    DataItem it;
-    std::memset(&it, 0, sizeof(it));
-    for (int d = 0; d < nd; ++d) it.x[d] = data[(size_t)(i * nd + d)];
-    it.id = (int32_t)i;
-    return it;
-  };
   if (bounded) {
     h.hB = std::max<int64_t>(1, (int64_t)std::llround(rho_max / smin));
     int64_t ncell = 1, span = 1;
@@ -1804,10 +1797,10 @@ void build_hash(const gsp_grid& g, int mode, HostReq& h) {
               h.hstart[(size_t)(cx + h.hd[0] * (cy + (int64_t)h.hd[1] * cz)) + 1]++;
       }
       for (int64_t c = 0; c < ncell; ++c) h.hstart[(size_t)c + 1] += h.hstart[(size_t)c];
-      h.hitems.assign((size_t)h.hstart[(size_t)ncell], DataItem{});
+      h.hitems.assign((size_t)h.hstart[(size_t)ncell], 0);
       std::vector<int32_t> fill((size_t)ncell, 0);
       for (int64_t i = 0; i < nD; ++i) {
-        const DataItem it = make_item(i);
+        const int32_t it = (int32_t)i;
         for (int64_t cz = lo[(size_t)(i * 3 + 2)]; cz <= hi[(size_t)(i * 3 + 2)]; ++cz)
           for (int64_t cy = lo[(size_t)(i * 3 + 1)]; cy <= hi[(size_t)(i * 3 + 1)]; ++cy)
             for (int64_t cx = lo[(size_t)(i * 3)]; cx <= hi[(size_t)(i * 3)]; ++cx) {
@@ -1844,11 +1837,11 @@ void build_hash(const gsp_grid& g, int mode, HostReq& h) {
       h.hstart[(size_t)c + 1]++;
     }
     for (int64_t c = 0; c < ncell; ++c) h.hstart[(size_t)c + 1] += h.hstart[(size_t)c];
-    h.hitems.assign((size_t)nD, DataItem{});
+    h.hitems.assign((size_t)nD, 0);
     std::vector<int32_t> fill((size_t)ncell, 0);
     for (int64_t i = 0; i < nD; ++i)
       h.hitems[(size_t)(h.hstart[(size_t)cell[(size_t)i]] + fill[(size_t)cell[(size_t)i]]++)] =
-          make_item(i);
+          (int32_t)i;
   }
 }
 
@@ -2327,7 +2320,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   std::vector<Req> reqs((size_t)n_req);
   std::vector<Offset> table((size_t)std::max<int64_t>(n_table, 1));
   std::vector<int32_t> hstart((size_t)n_cells);
-  std::vector<DataItem> hitems((size_t)std::max<int64_t>(n_items, 1));
+  std::vector<int32_t> hitems((size_t)std::max<int64_t>(n_items, 1));
   std::vector<double> data3((size_t)std::max<int64_t>(nDT * 3, 1), 0.0);
   std::vector<double> means((size_t)n_req);
   std::vector<ReqInfo> info((size_t)n_req);
@@ -2390,11 +2383,8 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       std::copy(h.table.begin(), h.table.end(), table.begin() + t_off);
       for (size_t c = 0; c < h.hstart.size(); ++c)
         hstart[(size_t)c_off + c] = (int32_t)(i_off + h.hstart[c]);
-      for (size_t t = 0; t < h.hitems.size(); ++t) {
-        DataItem it = h.hitems[t];
-        it.id += R.data_off;
-        hitems[(size_t)i_off + t] = it;
-      }
+      for (size_t t = 0; t < h.hitems.size(); ++t)
+        hitems[(size_t)i_off + t] = h.hitems[t] + R.data_off;
       for (int64_t i = 0; i < h.nD; ++i)
         for (int d = 0; d < h.nd; ++d)
           data3[(size_t)((data_off[(size_t)q] + i) * 3 + d)] = h.data[(size_t)(i * h.nd + d)];
@@ -2432,7 +2422,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   DevBuf<double> ddata;
   ddata.alloc(data3.size());
   DevBuf<int32_t> dhstart;
-  DevBuf<DataItem> dhitems;
+  DevBuf<int32_t> dhitems;
   dhstart.alloc(hstart.size());
   dhitems.alloc(hitems.size());
   GSP_HIP(hipMemcpyAsync(dreqs.p, reqs.data(), reqs.size() * sizeof(Req), hipMemcpyHostToDevice, s));
@@ -2444,7 +2434,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
                          hipMemcpyHostToDevice, s));
   GSP_HIP(hipMemcpyAsync(dhstart.p, hstart.data(), hstart.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, s));
-  GSP_HIP(hipMemcpyAsync(dhitems.p, hitems.data(), hitems.size() * sizeof(DataItem),
+  GSP_HIP(hipMemcpyAsync(dhitems.p, hitems.data(), hitems.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice, s));
   const int gn = grid_for(NT, 256, 4096);
   hipLaunchKernelGGL(k_batch_path, dim3(grid_for(NT / n_req + 1, 256, 1024), n_req), dim3(256), 0,
