@@ -473,7 +473,14 @@ struct InferArgs {
   int32_t* exact_list;
 };
 
-constexpr int kInferWaves = 8;  // 256 nodes per pass of the weights through LDS
+constexpr int kInferWaves = 8;  // wavefronts per block: one pass of the weights through LDS
+// 16-node column tiles per wavefront. Two (170 VGPRs, one block per CU) is
+// the measured choice. One (118 VGPRs, two blocks per CU) made inference 37 %
+// slower: each weight fragment read from LDS serves half the products. Three
+// (256 VGPRs) made it 15 % faster but fills the CU's registers, so the level
+// kernels that run beside it on the second stream stalled and the learned
+// 256x256x128 step got 9 % slower.
+constexpr int kInferTiles = 2;
 
 // Weight fragments of output tiles, staged in LDS for the block's
 // wavefronts: three buffers, so that the copy of tile rt + 2 runs while tile
@@ -528,7 +535,7 @@ __device__ __forceinline__ void wait_vmcnt() {
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 // One layer of the fused inference: out^T = act(W in^T + b) for the wave's
-// 32 nodes (2 column tiles); in/out hold the packed bf16 pairs of the
+// 16 CT nodes (CT column tiles); in/out hold the packed bf16 pairs of the
 // operand fragments, word q of step s, column tile ct at [s][ct][q]. Every
 // wavefront of the block runs it (block barriers inside). Step rt issues the
 // bias load of tile rt + 1 and the copy of tile rt + 2 into LDS, then the
@@ -536,9 +543,9 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "me
 // and the vector ALU overlap). The step ends by waiting for the loads issued
 // one step earlier only (loads complete in order; this wavefront's copy of
 // tile rt + 2 stays in flight), so each copy has a whole step to land.
-template <int STEPS, int TILES, bool ACT, int LT>
-__device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)[8][2][4],
-                                          uint32_t (&out)[8][2][4], float (&last)[LT][2][4],
+template <int STEPS, int TILES, bool ACT, int LT, int CT>
+__device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)[8][CT][4],
+                                          uint32_t (&out)[8][CT][4], float (&last)[LT][CT][4],
                                           const uint16_t* wp, const float* bias, int lane, int w) {
   const int g = lane >> 4;
   f32x4 b_next = *(const f32x4*)(bias + 4 * g);
@@ -549,7 +556,7 @@ __device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)
   else
     wait_vmcnt<0>();
   lds_barrier();
-  f32x4 acc[2][2];
+  f32x4 acc[2][CT];
   static_for<0, TILES + 1>([&](auto rtc) {
     constexpr int rt = decltype(rtc)::value;
     f32x4 b_cur = b_next;
@@ -557,22 +564,22 @@ __device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)
     if constexpr (rt + 2 < TILES) copy_tile<STEPS>(F.template buf<(rt + 2) % 3>(), wp, rt + 2, lane, w);
     if constexpr (rt < TILES) {
       // the accumulators start at the bias of their output rows
-      acc[rt & 1][0] = b_cur;
-      acc[rt & 1][1] = b_cur;
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[rt & 1][ct] = b_cur;
       static_for<0, STEPS>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const u32x4 a = F.template buf<rt % 3>()[s][lane];
-        acc[rt & 1][0] = mfma(a, u32x4{in[s][0][0], in[s][0][1], in[s][0][2], in[s][0][3]},
-                              acc[rt & 1][0]);
-        acc[rt & 1][1] = mfma(a, u32x4{in[s][1][0], in[s][1][1], in[s][1][2], in[s][1][3]},
-                              acc[rt & 1][1]);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          acc[rt & 1][ct] = mfma(a, u32x4{in[s][ct][0], in[s][ct][1], in[s][ct][2], in[s][ct][3]},
+                                 acc[rt & 1][ct]);
       });
     }
     if constexpr (rt >= 1) {
       constexpr int pt = rt - 1;
       if constexpr (ACT) {
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < CT; ++ct) {
           const f32x4 z = acc[pt & 1][ct];
           uint32_t w0 = pack2(gelu(z[0]), gelu(z[1]));
           uint32_t w1 = pack2(gelu(z[2]), gelu(z[3]));
@@ -584,7 +591,7 @@ __device__ __forceinline__ void mlp_layer(const FragLds& F, const uint32_t (&in)
         }
       } else {
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
           for (int v = 0; v < 4; ++v) last[pt][ct][v] = acc[pt & 1][ct][v];
       }
@@ -634,29 +641,32 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
   }
 }
 
-template <int SLOTS>
+// CT column tiles of 16 nodes per wavefront: every weight fragment read from
+// LDS serves CT products, and the two activation buffers of a layer take
+// 2 x 32 CT registers per lane.
+template <int SLOTS, int CT>
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
-  constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1;
+  constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1, kNodes = 16 * CT;
   __shared__ FragBuf s_w0, s_w1, s_w2;
   const FragLds s_frag{s_w0, s_w1, s_w2};
-  __shared__ uint32_t s_mask[kInferWaves][32];
-  __shared__ int32_t s_req[kInferWaves][32];
-  __shared__ int8_t s_slot[kInferWaves][32][SLOTS];
-  __shared__ float s_out[kInferWaves][32][kUsedOut + 1];
+  __shared__ uint32_t s_mask[kInferWaves][kNodes];
+  __shared__ int32_t s_req[kInferWaves][kNodes];
+  __shared__ int8_t s_slot[kInferWaves][kNodes][SLOTS];
+  __shared__ float s_out[kInferWaves][kNodes][kUsedOut + 1];
   __shared__ int32_t s_any;
   const NodeOut& O = A.O;
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // one group of 32 nodes per wavefront, four groups per block (no grid-stride
-  // loop: the weight addresses must not become loop invariants held in registers)
-  const int64_t base = ((int64_t)blockIdx.x * kInferWaves + w) * 32;
+  // one group of kNodes nodes per wavefront (no grid-stride loop: the weight
+  // addresses must not become loop invariants held in registers)
+  const int64_t base = ((int64_t)blockIdx.x * kInferWaves + w) * kNodes;
   if (tid == 0) s_any = 0;
   __syncthreads();
-  // ---- slots of the wave's 32 nodes
+  // ---- slots of the wave's nodes
   int state = -1;  // -1: not this launch's node, 0: exact solve, 1: learned candidate, 2: learned
   const int64_t i = base + lane;
   int p = 0;
-  if (lane < 32 && i < O.NT) {
+  if (lane < kNodes && i < O.NT) {
     p = O.node_req[i];
     const Req& R = O.reqs[p];
     const int n = O.nfound[i];
@@ -678,7 +688,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
       }
     }
   }
-  if (lane < 32) {
+  if (lane < kNodes) {
     if (state != 1) s_mask[w][lane] = 0;
     s_req[w][lane] = p;
   }
@@ -686,10 +696,10 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
   if (s_any) {
     // ---- layer 1 operands from the slot masks
     const int r = lane & 15, g = lane >> 4;
-    uint32_t h0[8][2][4], h1[8][2][4];
-    float last[kOutI / 16][2][4];
+    uint32_t h0[8][CT][4], h1[8][CT][4];
+    float last[kOutI / 16][CT][4];
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
+    for (int ct = 0; ct < CT; ++ct) {
       const uint32_t m = s_mask[w][16 * ct + r];
       const float* row = A.feat + (int64_t)s_req[w][16 * ct + r] * kIn;
 #pragma unroll
@@ -706,7 +716,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
 #pragma unroll
     for (int rt = 0; rt < kOutI / 16; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct)
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int o = 16 * rt + 4 * g + v;
@@ -844,14 +854,15 @@ void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, int32_t which, bool l
   A.which = which;
   A.list_plain = list_plain ? 1 : 0;
   A.exact_list = exact_list;
-  const int64_t blocks = (out.NT + kInferWaves * 32 - 1) / (kInferWaves * 32);
+  constexpr int kPerBlock = kInferWaves * 16 * kInferTiles;
+  const int64_t blocks = (out.NT + kPerBlock - 1) / kPerBlock;
   GSP_REQUIRE(blocks < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "too many nodes");
   if (s->n_slots == 24)
-    hipLaunchKernelGGL(k_mlp_infer<24>, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0,
-                       ctx->stream, A);
+    hipLaunchKernelGGL((k_mlp_infer<24, kInferTiles>), dim3((unsigned)blocks),
+                       dim3(64 * kInferWaves), 0, ctx->stream, A);
   else
-    hipLaunchKernelGGL(k_mlp_infer<32>, dim3((unsigned)blocks), dim3(64 * kInferWaves), 0,
-                       ctx->stream, A);
+    hipLaunchKernelGGL((k_mlp_infer<32, kInferTiles>), dim3((unsigned)blocks),
+                       dim3(64 * kInferWaves), 0, ctx->stream, A);
   GSP_HIP(hipGetLastError());
 }
 
