@@ -478,7 +478,8 @@ def train_batch_surrogates(ctx, _lib, spec, precision):
             coff += r["data"].size
         plan = _lib.SeqsimPlan(ctx, probs)
         S = _lib.Surrogate(ctx, [plan], max_samples=L["samples"], epochs=L["epochs"],
-                           batch=L["batch"], lr=L["lr"], seed=L.get("seed", 0) + fam)
+                           batch=L["batch"], lr=L["lr"], seed=L.get("seed", 0) + fam,
+                           data_slots=L.get("data_slots", 0))
         plan.close()
         dpath.free()
         ddata.free()
@@ -488,6 +489,7 @@ def train_batch_surrogates(ctx, _lib, spec, precision):
         reports.append({"family": ["spherical", "exponential", "gaussian"][fam], "requests": len(fr),
                         "systems": int(r.n_samples), "candidates": int(r.n_candidates),
                         "steps": int(r.steps), "ms": round(r.ms, 1),
+                        "data_slots": info["n_data_slots"],
                         "rms_weight_last_epoch": r.rms_weight_last_epoch,
                         "nugget_ratio_span": [round(x, 4) for x in info["nugget_ratio"]],
                         "range_span_cells": [round(1 / info["scale"][0][1], 2),
@@ -944,6 +946,9 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the learned-vs-exact accuracy pass")
     ap.add_argument("--epochs", type=int, default=None,
                     help="surrogate training epochs (default: the workload's; fewer for profiler runs)")
+    ap.add_argument("--data-slots", type=int, default=None, choices=[0, 4],
+                    help="surrogate data slots (learned batch workload; default 0: nodes with "
+                         "conditioning data among their neighbours are solved exactly)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (traffic-counter reference)")
     args = ap.parse_args()
@@ -1013,6 +1018,8 @@ def main():
     if "learned" in spec:
         if args.epochs is not None:
             spec = dict(spec, learned=dict(spec["learned"], epochs=args.epochs))
+        if args.data_slots is not None:
+            spec = dict(spec, learned=dict(spec["learned"], data_slots=args.data_slots))
         if "batch" in spec:
             S, training = train_batch_surrogates(ctx, _lib, spec, precision)
         else:

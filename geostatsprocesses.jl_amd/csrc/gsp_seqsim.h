@@ -45,6 +45,7 @@ struct NodeOut {
   const int32_t* nfound;
   const int32_t* nbr;
   const uint8_t* nbr_slot;  // learned plans: table index of each grid neighbour, 255 otherwise
+  const double* data;       // n_data_total x 3, physical coordinates of the data
   double* lam;
   double* sigma;
   uint8_t* flags;
@@ -115,6 +116,7 @@ struct gsp_seqsim_plan {
   gsp::DevBuf<uint8_t> flags;
   gsp::DevBuf<int32_t> pos;     // path position of every node
   gsp::DevBuf<int32_t> order;   // nodes grouped by level
+  gsp::DevBuf<double> data_xyz; // n_data x 3 physical data coordinates (padded to 3)
   std::vector<int64_t> level_off;  // level l holds order[level_off[l] .. level_off[l+1])
   // the last sweep captured as a HIP graph, replayed while every pointer and
   // size its kernels were captured with stays the same

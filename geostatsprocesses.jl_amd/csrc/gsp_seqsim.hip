@@ -2419,7 +2419,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   dreqs.alloc((size_t)n_req);
   DevBuf<Offset>& dtable = plan->table;
   dtable.alloc(table.size());
-  DevBuf<double> ddata;
+  DevBuf<double>& ddata = plan->data_xyz;  // kept: surrogate training reads the data offsets
   ddata.alloc(data3.size());
   DevBuf<int32_t> dhstart;
   DevBuf<int32_t> dhitems;
@@ -2577,6 +2577,7 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       O.nfound = plan->nfound.p;
       O.nbr = plan->nbr.p;
       O.nbr_slot = nbr_slot.p;
+      O.data = ddata.p;
       O.lam = plan->lam.p;
       O.sigma = plan->sigma.p;
       O.flags = plan->flags.p;

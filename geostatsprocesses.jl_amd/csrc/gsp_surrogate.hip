@@ -10,12 +10,18 @@
 // its offset in the request's sorted offset table, so an interior node of a
 // k-nearest-then-mask search fills slots among the same S offsets and the
 // input is the pattern of already simulated neighbours, scaled by the range.
+// A surrogate trained on conditional plans also takes up to kDataSlots
+// conditioning data among the neighbours: their offsets from the node (in
+// ranges) and the distance to the nearest of them enter 3 kDataSlots + 1 more
+// inputs, and their weights are kDataSlots more outputs (86 -> ... -> 29 for
+// S = 24, 110 -> ... -> 37 for S = 32).
 // One surrogate serves one variogram family and grid dimension over the spans
 // of ranges (in grid cells) and nugget ratios it was trained on, so requests
 // of a batch with different ranges and nuggets share it. Nodes whose
-// neighbours do not fit (grid boundary, conditioning data among them) and
-// nodes whose predicted variance disagrees with the variance implied by the
-// predicted weights are solved exactly.
+// neighbours do not fit (grid boundary, more data than data slots, or data at
+// all for a surrogate trained without them) and nodes whose predicted
+// variance disagrees with the variance implied by the predicted weights are
+// solved exactly.
 //
 // Training runs on the device: a BF16 NT GEMM with fused bias/GELU epilogues
 // (forward), GELU-derivative epilogues (backward), weight gradients as NT
@@ -44,7 +50,15 @@ __host__ __device__ constexpr int out_rows(int slots) { return slots <= 31 ? 32 
 constexpr int kOutIMax = out_rows(kSlotsMax);
 constexpr float kFeatScale = 8.0f;   // offsets in ranges x 8: inputs of order one
 constexpr float kNugScale = 10.0f;   // nugget / sill x 10
-static_assert(3 * kSlotsMax + 1 <= kIn && kSlotsMax + 1 <= kOutIMax && kOutIMax <= kOutT,
+// Data slots: the first kDataSlots conditioning data in a node's neighbour
+// list (nearest first). Inputs 3 S + 1 .. 3 S + 3 kDataSlots: their offsets
+// datum - node in ranges x kFeatScale; input 3 S + 3 kDataSlots + 1: the
+// length of the first offset (the nearest datum among the neighbours); zero
+// where absent. Outputs S + 1 .. S + kDataSlots: their weights.
+constexpr int kDataSlots = 4;
+constexpr int kDFeat = 3 * kDataSlots + 1;
+static_assert(3 * kSlotsMax + 1 + kDFeat <= kIn && kSlotsMax + 1 + kDataSlots <= kOutIMax &&
+                  kOutIMax <= kOutT,
               "network shapes");
 
 // FP32 master parameters, one buffer
@@ -121,10 +135,57 @@ __host__ __device__ inline int frag_k(int s, int g, int i) {
 
 // Input feature k of a node whose occupied slots are `mask`; `row` is its
 // request's input row (slot offsets in ranges x kFeatScale, then the nugget
-// feature, then zeros).
-__device__ inline float feature(const float* row, uint32_t mask, int k, int slots) {
+// feature, then zeros) and `dfeat` its kDFeat data features.
+// DATA = false: no data features (the row is zero beyond input 3S).
+template <bool DATA>
+__device__ inline float feature(const float* row, uint32_t mask, int k, int slots,
+                                const float* dfeat) {
   if (k < 3 * slots) return ((mask >> (k / 3)) & 1u) ? row[k] : 0.0f;
+  if constexpr (DATA) {
+    if (k > 3 * slots) {
+      const int q = k - 3 * slots - 1;
+      return q < kDFeat ? dfeat[q] : 0.0f;
+    }
+  }
   return row[k];
+}
+
+// Data features of node i (kDFeat floats, zero where absent) from its first
+// kDataSlots data neighbours, and cd[d] = C(datum d, node) / sill. The
+// offsets are taken in the kriging's scaled coordinates (alpha x physical)
+// times the reciprocal scaled ranges, which is the physical offset over the
+// range. Returns the node's number of data neighbours (possibly more than
+// kDataSlots).
+__device__ inline int node_data(const NodeOut& O, const Req& R, int i, int n, float* df,
+                                float* cd) {
+#pragma unroll
+  for (int q = 0; q < kDFeat; ++q) df[q] = 0.0f;
+#pragma unroll
+  for (int d = 0; d < kDataSlots; ++d) cd[d] = 0.0f;
+  const int l = i - R.node_off;
+  const int nx = R.dims[0], ny = R.dims[1];
+  const int c[3] = {l % nx, (l / nx) % ny, l / (nx * ny)};
+  double t[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) t[e] = R.ao[e] + ((double)c[e] + 0.5) * R.as[e];
+  int nd = 0;
+  for (int j = 0; j < n; ++j) {
+    const int nb = O.nbr[(int64_t)i * O.K + j];
+    if (nb < O.NT) continue;
+    if (nd < kDataSlots) {
+      const double* p = O.data + (int64_t)(nb - O.NT) * 3;
+      double x[3];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        x[e] = e < R.ndim ? R.alpha * p[e] : 0.0;
+        df[3 * nd + e] = e < R.ndim ? (float)((x[e] - t[e]) * R.M.irs[e] * kFeatScale) : 0.0f;
+      }
+      cd[nd] = (float)(cov_points(R.M, x[0], x[1], x[2], t[0], t[1], t[2]) / R.M.sill);
+    }
+    ++nd;
+  }
+  if (nd > 0) df[3 * kDataSlots] = sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]);
+  return nd;
 }
 
 // Slot of grid offset (dx, dy, dz): its index among the first `slots`
@@ -136,24 +197,27 @@ __device__ inline int slot_of(const Offset* tab, int T, int dx, int dy, int dz, 
   return -1;
 }
 
-// The surrogate's view of node i: its slot mask (and slot of each neighbour)
-// when every neighbour is a grid node of the first `ns` table offsets, else
-// false.
+// The surrogate's view of node i: its slot mask and number of data
+// neighbours when every grid neighbour lies in the first `ns` table offsets
+// and at most kDataSlots neighbours are data, else false.
 __device__ inline bool node_slots(const NodeOut& O, const Req& R, int i, int n, int ns,
-                                  uint32_t& mask, int8_t* slots) {
+                                  uint32_t& mask, int& ndata) {
   const int nx = R.dims[0], ny = R.dims[1];
   const int li = i - R.node_off;
   const int ix = li % nx, iy = (li / nx) % ny, iz = li / (nx * ny);
   const Offset* tab = O.table + R.table_off;
   mask = 0;
+  ndata = 0;
   for (int j = 0; j < n; ++j) {
     const int nb = O.nbr[(int64_t)i * O.K + j];
-    if (nb >= O.NT) return false;
+    if (nb >= O.NT) {
+      if (++ndata > kDataSlots) return false;
+      continue;
+    }
     const int l = nb - R.node_off;
     const int s = slot_of(tab, R.T, l % nx - ix, (l / nx) % ny - iy, l / (nx * ny) - iz, ns);
     if (s < 0) return false;
     mask |= 1u << s;
-    if (slots) slots[j] = (int8_t)s;
   }
   return true;
 }
@@ -233,29 +297,36 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmArgs P) {
     }
 }
 
-// One minibatch: inputs (both layouts), targets and slot masks of the
-// samples perm[off .. off + Bt).
+// One minibatch: inputs (both layouts), targets, slot masks and data counts
+// of the samples perm[off .. off + Bt).
 __global__ void k_train_batch(const int32_t* perm, int64_t off, int32_t Bt, const uint32_t* masks,
-                              const int32_t* srow, const float* Y, const float* rows, int32_t slots,
-                              uint16_t* X, uint16_t* Xt, float* Yb, uint32_t* Mb) {
+                              const int32_t* srow, const uint8_t* ndata, const float* dfeat,
+                              const float* Y, const float* rows, int32_t slots, uint16_t* X,
+                              uint16_t* Xt, float* Yb, uint32_t* Mb, uint8_t* Nb) {
   const int64_t total = (int64_t)Bt * kIn;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(t / kIn), k = (int)(t % kIn);
     const int smp = perm[off + b];
     const uint32_t m = masks[smp];
-    const uint16_t h = f2bf(feature(rows + (int64_t)srow[smp] * kIn, m, k, slots));
+    const uint16_t h = f2bf(feature<true>(rows + (int64_t)srow[smp] * kIn, m, k, slots,
+                                          dfeat + (int64_t)smp * kDFeat));
     X[(int64_t)b * kIn + k] = h;
     Xt[(int64_t)k * Bt + b] = h;
     if (k < kOutT) Yb[(int64_t)b * kOutT + k] = Y[(int64_t)smp * kOutT + k];
-    if (k == 0) Mb[b] = m;
+    if (k == 0) {
+      Mb[b] = m;
+      Nb[b] = ndata[smp];
+    }
   }
 }
 
-// Loss = mean over the batch of sum_s occupied (o_s - y_s)^2 + wv (o_S - y_S)^2;
-// writes dLoss/dO in bf16 (both layouts) and accumulates the loss terms.
-__global__ void k_train_loss(const float* O, const float* Yb, const uint32_t* Mb, int32_t Bt,
-                             int32_t slots, float wv, uint16_t* dO, uint16_t* dOt, float* acc) {
+// Loss = mean over the batch of sum_s occupied (o_s - y_s)^2 + wv (o_S - y_S)^2
+// + sum_{d < data count} (o_{S+1+d} - y_{S+1+d})^2; writes dLoss/dO in bf16
+// (both layouts) and accumulates the loss terms.
+__global__ void k_train_loss(const float* O, const float* Yb, const uint32_t* Mb,
+                             const uint8_t* Nb, int32_t Bt, int32_t slots, float wv, uint16_t* dO,
+                             uint16_t* dOt, float* acc) {
   __shared__ float sw[256], sv[256], sc[256];
   float lw = 0.0f, lv = 0.0f, cnt = 0.0f;
   const int64_t total = (int64_t)Bt * kOutT;
@@ -273,6 +344,10 @@ __global__ void k_train_loss(const float* O, const float* Yb, const uint32_t* Mb
       const float e = O[t] - Yb[(int64_t)b * kOutT + o];
       lv += e * e;
       d = wv * e;
+    } else if (o - slots - 1 < (int)Nb[b]) {
+      d = O[t] - Yb[(int64_t)b * kOutT + o];
+      lw += d * d;
+      cnt += 1.0f;
     }
     const uint16_t h = f2bf(2.0f * d / (float)Bt);
     dO[t] = h;
@@ -396,20 +471,25 @@ __global__ void k_pack_inference(const float* params, uint16_t* wp, int32_t out_
 // Training samples of one plan: every node of an accepted request that was
 // solved exactly with at least one neighbour, all in table slots, drawn with
 // probability `keep` (a hash of the node id and the seed). count != nullptr:
-// counts the candidates only. A sample keeps its slot mask, the input row of
-// its request (req_row; -1: request not used) and its targets: the weights
-// by slot and log(sigma^2 / sill), kOutT floats.
+// counts the candidates only (count[0]; count[1]: those with data
+// neighbours). A sample keeps its slot mask, the input row of its request
+// (req_row; -1: request not used), its number of data neighbours and data
+// features, and its targets: the weights by slot, log(sigma^2 / sill) and the
+// weights of its data neighbours, kOutT floats.
 struct ExtractArgs {
   NodeOut O;
   int32_t slots;
   const int32_t* req_row;
   uint32_t keep24;  // acceptance threshold on 24 hash bits
   uint64_t seed;
+  int32_t max_data;  // data neighbours a sample may have
   int32_t* count;
   int32_t* cursor;
   int32_t capacity;
   uint32_t* masks;
   int32_t* srow;
+  uint8_t* ndata;
+  float* dfeat;  // kDFeat per sample
   float* Y;
 };
 
@@ -434,9 +514,11 @@ __global__ void k_extract(const ExtractArgs E) {
     if (n < 1) continue;
     const Req& R = O.reqs[p];
     uint32_t mask;
-    if (!node_slots(O, R, (int)i, n, E.slots, mask, nullptr)) continue;
+    int nd;
+    if (!node_slots(O, R, (int)i, n, E.slots, mask, nd) || nd > E.max_data) continue;
     if (E.count) {
       atomicAdd(E.count, 1);
+      if (nd > 0) atomicAdd(E.count + 1, 1);
       continue;
     }
     if ((mix32(E.seed ^ (uint64_t)i * 0x9E3779B97F4A7C15ULL) & 0xFFFFFFu) >= E.keep24) continue;
@@ -444,13 +526,22 @@ __global__ void k_extract(const ExtractArgs E) {
     if (slot >= E.capacity) continue;
     E.masks[slot] = mask;
     E.srow[slot] = row;
+    E.ndata[slot] = (uint8_t)nd;
+    float cd[kDataSlots];
+    node_data(O, R, (int)i, n, E.dfeat + (int64_t)slot * kDFeat, cd);
     float* y = E.Y + (int64_t)slot * kOutT;
     for (int s = 0; s < kOutT; ++s) y[s] = 0.0f;
     const int nx = R.dims[0], ny = R.dims[1];
     const int li = (int)i - R.node_off;
     const int ix = li % nx, iy = (li / nx) % ny, iz = li / (nx * ny);
+    int d = 0;
     for (int j = 0; j < n; ++j) {
-      const int l = O.nbr[i * O.K + j] - R.node_off;
+      const int nb = O.nbr[i * O.K + j];
+      if (nb >= O.NT) {
+        y[E.slots + 1 + d++] = (float)O.lam[i * O.K + j];
+        continue;
+      }
+      const int l = nb - R.node_off;
       const int s = slot_of(O.table + R.table_off, R.T, l % nx - ix, (l / nx) % ny - iy,
                             l / (nx * ny) - iz, E.slots);
       y[s] = (float)O.lam[i * O.K + j];
@@ -468,6 +559,7 @@ struct InferArgs {
   const float* feat;    // input row of every request (kIn floats)
   const float* cslot;   // C(slot offset) / sill of every request (kSlotsMax floats)
   float tol;
+  int32_t dslots;       // data neighbours a node may have (the surrogate's data slots)
   int32_t which;        // requests with learned == which are this surrogate's
   int32_t list_plain;   // also list the nodes of requests with learned == 0
   int32_t* exact_list;
@@ -643,15 +735,20 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
 
 // CT column tiles of 16 nodes per wavefront: every weight fragment read from
 // LDS serves CT products, and the two activation buffers of a layer take
-// 2 x 32 CT registers per lane.
-template <int SLOTS, int CT>
+// 2 x 32 CT registers per lane. DATA: the surrogate has data slots (nodes
+// with data neighbours may be learned); without, the kernel carries none of
+// their features, LDS or outputs.
+template <int SLOTS, int CT, bool DATA>
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
-  constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1, kNodes = 16 * CT;
+  constexpr int kD = DATA ? kDataSlots : 0;
+  constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1 + kD, kNodes = 16 * CT;
   __shared__ FragBuf s_w0, s_w1, s_w2;
   const FragLds s_frag{s_w0, s_w1, s_w2};
   __shared__ uint32_t s_mask[kInferWaves][kNodes];
   __shared__ int32_t s_req[kInferWaves][kNodes];
-  __shared__ int8_t s_slot[kInferWaves][kNodes][SLOTS];
+  __shared__ int8_t s_slot[kInferWaves][kNodes][SLOTS];  // output of each neighbour's weight
+  __shared__ float s_dfeat[kInferWaves][kNodes][DATA ? kDFeat : 1];
+  __shared__ float s_cd[kInferWaves][kNodes][DATA ? kDataSlots : 1];
   __shared__ float s_out[kInferWaves][kNodes][kUsedOut + 1];
   __shared__ int32_t s_any;
   const NodeOut& O = A.O;
@@ -666,30 +763,51 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
   int state = -1;  // -1: not this launch's node, 0: exact solve, 1: learned candidate, 2: learned
   const int64_t i = base + lane;
   int p = 0;
+  int ndl = 0;  // data neighbours of a learned candidate
   if (lane < kNodes && i < O.NT) {
     p = O.node_req[i];
     const Req& R = O.reqs[p];
     const int n = O.nfound[i];
     if (R.learned == A.which || (A.list_plain && R.learned == 0)) state = 0;
     if (R.learned == A.which && n >= R.kmin && n >= 1) {
-      // slots from the table indices the search recorded
+      // slots from the table indices the search recorded; data neighbours
+      // (table index 255) take the data slots in list order
       uint32_t mask = 0;
       bool fits = true;
+      int nd = 0;
       for (int j = 0; j < n; ++j) {
         const int t = O.nbr_slot[i * O.K + j];
-        fits = fits && t < SLOTS;
-        mask |= t < SLOTS ? 1u << t : 0u;
-        s_slot[w][lane][j] = (int8_t)(t < SLOTS ? t : 0);
+        if (t < SLOTS) {
+          mask |= 1u << t;
+          s_slot[w][lane][j] = (int8_t)t;
+        } else if (DATA && nd < A.dslots && O.nbr[i * O.K + j] >= O.NT) {
+          s_slot[w][lane][j] = (int8_t)(SLOTS + 1 + nd);
+          ++nd;
+        } else {
+          fits = false;
+        }
       }
       if (fits) {
         state = 1;
         s_mask[w][lane] = mask;
         s_any = 1;
+        ndl = nd;
+        if constexpr (DATA)
+          if (nd > 0) node_data(O, R, (int)i, n, s_dfeat[w][lane], s_cd[w][lane]);
       }
     }
   }
   if (lane < kNodes) {
     if (state != 1) s_mask[w][lane] = 0;
+    if constexpr (DATA) {
+      if (state != 1 || ndl == 0) {
+        // no data neighbours: zero data inputs and covariances
+#pragma unroll
+        for (int q = 0; q < kDFeat; ++q) s_dfeat[w][lane][q] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < kDataSlots; ++d) s_cd[w][lane][d] = 0.0f;
+      }
+    }
     s_req[w][lane] = p;
   }
   __syncthreads();
@@ -702,12 +820,13 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
     for (int ct = 0; ct < CT; ++ct) {
       const uint32_t m = s_mask[w][16 * ct + r];
       const float* row = A.feat + (int64_t)s_req[w][16 * ct + r] * kIn;
+      const float* df = DATA ? s_dfeat[w][16 * ct + r] : nullptr;
 #pragma unroll
       for (int s = 0; s < kIn / 32; ++s)
 #pragma unroll
         for (int it = 0; it < 4; ++it)
-          h0[s][ct][it] = pack2(feature(row, m, frag_k(s, g, 2 * it), SLOTS),
-                                feature(row, m, frag_k(s, g, 2 * it + 1), SLOTS));
+          h0[s][ct][it] = pack2(feature<DATA>(row, m, frag_k(s, g, 2 * it), SLOTS, df),
+                                feature<DATA>(row, m, frag_k(s, g, 2 * it + 1), SLOTS, df));
     }
     mlp_layer<kIn / 32, kHid / 16, true>(s_frag, h0, h1, last, A.wp + PW1, A.params + OB1, lane, w);
     mlp_layer<kHid / 32, kHid / 16, true>(s_frag, h1, h0, last, A.wp + PW2, A.params + OB2, lane, w);
@@ -733,6 +852,8 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs 
       float chk = 1.0f;
       for (int s = 0; s < SLOTS; ++s)
         if ((mask >> s) & 1u) chk -= out[s] * cs[s];
+      if constexpr (DATA)
+        for (int d = 0; d < kDataSlots; ++d) chk -= out[SLOTS + 1 + d] * s_cd[w][lane][d];
       const float pred = expf(out[SLOTS]);
       if (fabsf(pred - chk) <= A.tol * pred) {
         for (int j = 0; j < O.K; ++j)
@@ -778,6 +899,7 @@ struct gsp_surrogate {
   double nug_lo = 0.0, nug_hi = 0.0;
   double scale_lo[3] = {0, 0, 0}, scale_hi[3] = {0, 0, 0};
   int32_t n_slots = gsp::kSlotsMax;  // 24 or 32
+  int32_t data_slots = 0;            // data neighbours a node may have (0: none)
   int16_t table[gsp::kSlotsMax][3] = {};
   int32_t n_train_requests = 0;
   gsp::DevBuf<float> params, adam_m, adam_v, grad;
@@ -851,18 +973,23 @@ void surrogate_infer(gsp_ctx* ctx, const gsp_surrogate* s, int32_t which, bool l
   A.feat = feat;
   A.cslot = cslot;
   A.tol = (float)tolerance;
+  A.dslots = s->data_slots;
   A.which = which;
   A.list_plain = list_plain ? 1 : 0;
   A.exact_list = exact_list;
   constexpr int kPerBlock = kInferWaves * 16 * kInferTiles;
   const int64_t blocks = (out.NT + kPerBlock - 1) / kPerBlock;
   GSP_REQUIRE(blocks < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "too many nodes");
-  if (s->n_slots == 24)
-    hipLaunchKernelGGL((k_mlp_infer<24, kInferTiles>), dim3((unsigned)blocks),
-                       dim3(64 * kInferWaves), 0, ctx->stream, A);
+  const dim3 grid((unsigned)blocks), block(64 * kInferWaves);
+  const bool data = s->data_slots > 0;
+  if (s->n_slots == 24 && !data)
+    hipLaunchKernelGGL((k_mlp_infer<24, kInferTiles, false>), grid, block, 0, ctx->stream, A);
+  else if (s->n_slots == 24)
+    hipLaunchKernelGGL((k_mlp_infer<24, kInferTiles, true>), grid, block, 0, ctx->stream, A);
+  else if (!data)
+    hipLaunchKernelGGL((k_mlp_infer<32, kInferTiles, false>), grid, block, 0, ctx->stream, A);
   else
-    hipLaunchKernelGGL((k_mlp_infer<32, kInferTiles>), dim3((unsigned)blocks),
-                       dim3(64 * kInferWaves), 0, ctx->stream, A);
+    hipLaunchKernelGGL((k_mlp_infer<32, kInferTiles, true>), grid, block, 0, ctx->stream, A);
   GSP_HIP(hipGetLastError());
 }
 
@@ -877,6 +1004,7 @@ NodeOut node_view(const gsp_seqsim_plan* p) {
   O.nfound = p->nfound.p;
   O.nbr = p->nbr.p;
   O.nbr_slot = nullptr;
+  O.data = p->data_xyz.p;
   O.lam = p->lam.p;
   O.sigma = p->sigma.p;
   O.flags = p->flags.p;
@@ -897,6 +1025,8 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
               "invalid training options");
   GSP_REQUIRE(opt->n_slots == 0 || opt->n_slots == 24 || opt->n_slots == 32, GSP_ERR_ARGUMENT,
               "n_slots must be 0 (automatic), 24 or 32");
+  GSP_REQUIRE(opt->data_slots == 0 || opt->data_slots == kDataSlots, GSP_ERR_ARGUMENT,
+              "data_slots must be 0 or %d", kDataSlots);
   hipStream_t st = ctx->stream;
   auto usable = [](const ReqInfo& r) {
     return !r.model.has_rotation && r.mode == GSP_KNN_THEN_MASK && r.model.sill > 0;
@@ -967,7 +1097,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   // ---- training systems
   ctx->counters.reserve(16);
   std::vector<DevBuf<int32_t>> ok((size_t)n_plans);
-  int64_t candidates = 0;
+  int64_t candidates = 0, data_candidates = 0;
   for (int32_t q = 0; q < n_plans; ++q) {
     const gsp_seqsim_plan* p = plans[q];
     const std::vector<int32_t>& rr = req_rows[(size_t)q];
@@ -979,13 +1109,20 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
     E.O = node_view(p);
     E.slots = slots;
     E.req_row = ok[(size_t)q].p;
+    E.max_data = opt->data_slots;
     E.count = ctx->counters.p;
     hipLaunchKernelGGL(k_extract, dim3(grid_for(p->N, 256, 4096)), dim3(256), 0, st, E);
-    int32_t c = 0;
-    GSP_HIP(hipMemcpyAsync(&c, ctx->counters.p, 4, hipMemcpyDeviceToHost, st));
+    int32_t c[2] = {0, 0};
+    GSP_HIP(hipMemcpyAsync(c, ctx->counters.p, 8, hipMemcpyDeviceToHost, st));
     GSP_HIP(hipStreamSynchronize(st));
-    candidates += c;
+    candidates += c[0];
+    data_candidates += c[1];
   }
+  // data slots when asked for and the plans hold systems with data
+  // neighbours: otherwise their weights stay zero and such nodes are left to
+  // the exact solver
+  const int dslots = opt->data_slots > 0 && data_candidates > 0 ? kDataSlots : 0;
+  S->data_slots = dslots;
   GSP_REQUIRE(candidates >= Bt, GSP_ERR_ARGUMENT,
               "the plans hold %lld usable systems, fewer than one batch (%d)",
               (long long)candidates, Bt);
@@ -996,9 +1133,12 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   const int64_t capacity = std::min<int64_t>(candidates, want + want / 16 + 4096);
   DevBuf<uint32_t> masks;
   DevBuf<int32_t> srow;
-  DevBuf<float> Y;
+  DevBuf<uint8_t> ndata;
+  DevBuf<float> Y, dfeat;
   masks.alloc((size_t)capacity);
   srow.alloc((size_t)capacity);
+  ndata.alloc((size_t)capacity);
+  dfeat.alloc((size_t)(capacity * kDFeat));
   Y.alloc((size_t)(capacity * kOutT));
   GSP_HIP(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int32_t), st));
   for (int32_t q = 0; q < n_plans; ++q) {
@@ -1009,10 +1149,13 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
     E.req_row = ok[(size_t)q].p;
     E.keep24 = keep24;
     E.seed = opt->seed * 0x100000001B3ULL + (uint64_t)q;
+    E.max_data = dslots;
     E.cursor = ctx->counters.p + 1;
     E.capacity = (int32_t)capacity;
     E.masks = masks.p;
     E.srow = srow.p;
+    E.ndata = ndata.p;
+    E.dfeat = dfeat.p;
     E.Y = Y.p;
     hipLaunchKernelGGL(k_extract, dim3(grid_for(plans[q]->N, 256, 4096)), dim3(256), 0, st, E);
   }
@@ -1034,7 +1177,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
       for (int cc = 0; cc < cols; ++cc)
         P0[(size_t)(off + (int64_t)rr * cols + cc)] = (rr < used_rows && cc < used_cols) ? U(rng) : 0.0f;
   };
-  const int used_in = 3 * slots + 1, used_out = slots + 1;
+  const int used_in = 3 * slots + 1 + (dslots ? kDFeat : 0), used_out = slots + 1 + dslots;
   fill(OW1, kHid, kIn, kHid, used_in, used_in);
   fill(OB1, 1, kHid, 1, kHid, used_in);
   fill(OW2, kHid, kHid, kHid, kHid, kHid);
@@ -1063,6 +1206,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   DevBuf<uint16_t> X, Xt, H1, H1t, H2, H2t, H3, H3t, dO, dOt, dZ1, dZ1t, dZ2, dZ2t, dZ3, dZ3t;
   DevBuf<float> G1, G2, G3, Out, Yb, lossacc;
   DevBuf<uint32_t> Mb;
+  DevBuf<uint8_t> Nb;
   DevBuf<int32_t> perm;
   const size_t B = (size_t)Bt;
   X.alloc(B * kIn);
@@ -1077,6 +1221,7 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
   Out.alloc(B * kOutT);
   Yb.alloc(B * kOutT);
   Mb.alloc(B);
+  Nb.alloc(B);
   lossacc.alloc(4);
   perm.alloc((size_t)S_n);
 
@@ -1128,8 +1273,8 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
         lr = opt->lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.14159265358979323846 * f)));
       }
       hipLaunchKernelGGL(k_train_batch, dim3(grid_for(B * kIn, 256, 1024)), dim3(256), 0, st,
-                         perm.p, b * Bt, Bt, masks.p, srow.p, Y.p, drows.p, slots, X.p, Xt.p,
-                         Yb.p, Mb.p);
+                         perm.p, b * Bt, Bt, masks.p, srow.p, ndata.p, dfeat.p, Y.p, drows.p,
+                         slots, X.p, Xt.p, Yb.p, Mb.p, Nb.p);
       // forward
       gemm(EPI_GELU, X.p, kIn, S->wbf.p + BW1, kIn, Bt, kHid, kIn, nullptr, 0, S->params.p + OB1,
            H1.p, H1t.p, G1.p, nullptr);
@@ -1140,7 +1285,8 @@ void surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const
       gemm(EPI_F32, H3.p, kHid, S->wbf.p + BW4, kHid, Bt, kOutT, kHid, Out.p, kOutT,
            S->params.p + OB4, nullptr, nullptr, nullptr, nullptr);
       hipLaunchKernelGGL(k_train_loss, dim3(grid_for(B * kOutT, 256, 256)), dim3(256), 0, st, Out.p,
-                         Yb.p, Mb.p, Bt, slots, (float)opt->var_weight, dO.p, dOt.p, lossacc.p);
+                         Yb.p, Mb.p, Nb.p, Bt, slots, (float)opt->var_weight, dO.p, dOt.p,
+                         lossacc.p);
       // backward
       gemm(EPI_F32, dOt.p, Bt, H3t.p, Bt, kOutT, kHid, Bt, grad + OW4, kHid, nullptr, nullptr,
            nullptr, nullptr, nullptr);
@@ -1225,6 +1371,7 @@ void surrogate_info(const gsp_surrogate* s, gsp_surrogate_info* out) {
     out->scale[d][1] = s->scale_hi[d];
   }
   out->n_samples = s->n_samples;
+  out->n_data_slots = s->data_slots;
 }
 
 gsp_surrogate* surrogate_new(gsp_ctx* ctx) {

@@ -96,7 +96,8 @@ STAT_MEAN, STAT_VAR, STAT_CDF, STAT_CCDF, STAT_QUANTILE = 0, 1, 2, 3, 4
 class GspTrainOptions(ctypes.Structure):
     _fields_ = [("max_samples", ctypes.c_int64), ("epochs", ctypes.c_int32),
                 ("batch", ctypes.c_int32), ("lr", ctypes.c_double), ("var_weight", ctypes.c_double),
-                ("seed", ctypes.c_uint64), ("n_slots", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("seed", ctypes.c_uint64), ("n_slots", ctypes.c_int32),
+                ("data_slots", ctypes.c_int32)]
 
 
 class GspTrainReport(ctypes.Structure):
@@ -110,7 +111,8 @@ class GspSurrogateInfo(ctypes.Structure):
     _fields_ = [("family", ctypes.c_int32), ("ndim", ctypes.c_int32), ("n_slots", ctypes.c_int32),
                 ("n_train_requests", ctypes.c_int32), ("sill", ctypes.c_double),
                 ("nugget_ratio", ctypes.c_double * 2), ("scale", (ctypes.c_double * 2) * 3),
-                ("n_samples", ctypes.c_int64)]
+                ("n_samples", ctypes.c_int64), ("n_data_slots", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 class GspTimings(ctypes.Structure):
@@ -472,12 +474,12 @@ class Surrogate:
     exact plans (gsp_surrogate_train)."""
 
     def __init__(self, ctx: Context, plans, max_samples=0, epochs=20, batch=1024, lr=2e-3,
-                 var_weight=0.01, seed=0, n_slots=0):
+                 var_weight=0.01, seed=0, n_slots=0, data_slots=0):
         self.ctx = ctx
         self._h = ctypes.c_void_p()
         opts = GspTrainOptions(max_samples=int(max_samples), epochs=int(epochs), batch=int(batch),
                                lr=float(lr), var_weight=float(var_weight), seed=int(seed),
-                               n_slots=int(n_slots))
+                               n_slots=int(n_slots), data_slots=int(data_slots))
         handles = (ctypes.c_void_p * len(plans))(*[p._h.value for p in plans])
         self.report = GspTrainReport()
         check(load().gsp_surrogate_train(ctx.handle, len(plans), handles, ctypes.byref(opts),
@@ -506,7 +508,8 @@ class Surrogate:
         return {"family": i.family, "ndim": i.ndim, "n_slots": i.n_slots,
                 "n_train_requests": i.n_train_requests, "sill": i.sill,
                 "nugget_ratio": tuple(i.nugget_ratio),
-                "scale": [tuple(i.scale[d]) for d in range(3)], "n_samples": i.n_samples}
+                "scale": [tuple(i.scale[d]) for d in range(3)], "n_samples": i.n_samples,
+                "n_data_slots": i.n_data_slots}
 
     def close(self):
         if self._h and self._h.value:

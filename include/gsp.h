@@ -178,7 +178,10 @@ typedef struct {
   uint64_t seed;
   int32_t n_slots;              /* neighbour slots S: 24, 32, or 0 for 24 unless a
                                    training request asks for more than 24 neighbours */
-  int32_t pad_;
+  int32_t data_slots;           /* 0: nodes with conditioning data among their
+                                   neighbours are left to the exact solver; 4: such
+                                   nodes with at most 4 data neighbours are training
+                                   systems and may be learned */
 } gsp_train_options;
 
 /* The domain a trained surrogate serves (gsp_surrogate_get_info). */
@@ -191,6 +194,9 @@ typedef struct {
   double nugget_ratio[2];       /* span of nugget / sill trained on */
   double scale[3][2];           /* span of spacing / range per axis trained on */
   int64_t n_samples;
+  int32_t n_data_slots;         /* data neighbours a learned node may have (4, or 0:
+                                   trained without data_slots or without such systems) */
+  int32_t pad_;
 } gsp_surrogate_info;
 
 typedef struct {
@@ -371,28 +377,37 @@ int32_t gsp_seqsim_plan_layout(const gsp_seqsim_plan* plan, int32_t* n_requests,
  * ... -> 33) maps a node's neighbour offsets (times the grid spacing, divided
  * by the ranges, in slots given by the rank of the offset in the search's
  * sorted offset table) and the model's nugget / sill to its S kriging weights
- * and log kriging variance.
+ * and log kriging variance. Trained with data_slots = 4 on conditional plans,
+ * it also takes up to 4 conditioning data among a node's neighbours (their
+ * offsets from the node over the ranges, and the distance to the nearest of
+ * them: 13 more inputs) and gives their weights (4 more outputs): 86 -> ...
+ * -> 29 for S = 24, 110 -> ... -> 37 for S = 32.
  *
  * gsp_surrogate_train draws training systems from exact plans: every node,
- * solved exactly with only grid neighbours among the S slot offsets, of the
- * requests compatible with the first usable one (k nearest then mask,
- * unrotated, maxneighbors <= S, at least S table offsets): same variogram
- * family, grid dimension and slot offsets (and sill, for the gaussian family),
- * any range, nugget and maxneighbors. The surrogate then serves requests
- * inside the spans of nugget / sill and spacing / range it was trained on. */
+ * solved exactly with its grid neighbours among the S slot offsets (and, with
+ * data_slots = 4, at most 4 data neighbours), of the requests compatible with the first usable
+ * one (k nearest then mask, unrotated, maxneighbors <= S, at least S table
+ * offsets): same variogram family, grid dimension and slot offsets (and sill,
+ * for the gaussian family), any range, nugget and maxneighbors. The surrogate
+ * then serves requests inside the spans of nugget / sill and spacing / range
+ * it was trained on. */
 int32_t gsp_surrogate_train(gsp_ctx* ctx, int32_t n_plans, const gsp_seqsim_plan* const* plans,
                             const gsp_train_options* options, gsp_surrogate** out,
                             gsp_train_report* report);
 int32_t gsp_surrogate_free(gsp_surrogate* surrogate);
 /* The surrogate's FP32 master parameters, layer by layer: W1 [256][128]
- * (inputs 3S+1..127 zero), b1 [256], W2 [256][256], b2, W3 [256][256], b3,
- * W4 [64][256] (outputs S+1..63 zero), b4 [64]; and its S slot offsets in
- * grid cells (slots[32][3], rows S..31 zero). Input k < 3S of a node is, when
- * slot k / 3 holds a neighbour, 8 * slots[k / 3][d] * spacing[d] / range[d]
- * (d = k % 3), else 0; input 3S is 10 * nugget / sill. Outputs 0..S-1 are the
- * weights by slot, output S is log(sigma^2 / sill). Inference uses the
- * weights rounded to BF16. Either pointer may be NULL; *count receives the
- * number of parameters. */
+ * (inputs beyond the used ones zero), b1 [256], W2 [256][256], b2,
+ * W3 [256][256], b3, W4 [64][256] (outputs beyond the used ones zero), b4
+ * [64]; and its S slot offsets in grid cells (slots[32][3], rows S..31 zero).
+ * Input k < 3S of a node is, when slot k / 3 holds a neighbour,
+ * 8 * slots[k / 3][d] * spacing[d] / range[d] (d = k % 3), else 0; input 3S
+ * is 10 * nugget / sill; inputs 3S+1+3q+d (q < 4, d < 3) are 8 * (x_q[d] -
+ * c[d]) / range[d] for the q-th datum among the node's neighbours (neighbour
+ * order) at x_q, c the node's centre, and input 3S+13 is the length of the
+ * first of these offsets (all 0 without data neighbours). Outputs 0..S-1 are
+ * the weights by slot, output S is log(sigma^2 / sill), outputs S+1+q the
+ * weights of the data neighbours. Inference uses the weights rounded to BF16.
+ * Either pointer may be NULL; *count receives the number of parameters. */
 int32_t gsp_surrogate_export(gsp_ctx* ctx, const gsp_surrogate* surrogate, float* params,
                              int64_t* count, double* slots);
 int32_t gsp_surrogate_get_info(const gsp_surrogate* surrogate, gsp_surrogate_info* info);

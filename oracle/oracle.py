@@ -645,6 +645,8 @@ def lusim_sample(d2, L22, dinds, sinds, z1, mean, w):
 MLP_IN, MLP_HID, MLP_OUT = 128, 256, 64
 MLP_SLOTS = 32
 MLP_FEAT_SCALE, MLP_NUG_SCALE = 8.0, 10.0
+MLP_DATA_SLOTS = 4                     # data neighbours a node may have
+MLP_DFEAT = 3 * MLP_DATA_SLOTS + 1     # their offsets, then the nearest one's length
 
 
 def bf16(x):
@@ -687,9 +689,10 @@ def mlp_request_row(case: SeqsimCase, slot_cells):
     return row
 
 
-def mlp_features(row, masks, S):
+def mlp_features(row, masks, S, dfeat=None):
     """Inputs of an S-slot surrogate for slot masks (bit s: neighbour in slot
-    s) of one request's input row."""
+    s) of one request's input row, and the nodes' data features (n,
+    MLP_DFEAT) when given."""
     masks = np.asarray(masks, dtype=np.uint32)
     X = np.zeros((len(masks), MLP_IN), np.float32)
     occ = ((masks[:, None] >> np.arange(S, dtype=np.uint32)[None, :]) & 1).astype(np.float32)
@@ -697,6 +700,8 @@ def mlp_features(row, masks, S):
     X[:, :k] = (occ[:, :, None] * np.asarray(row[:k], np.float32).reshape(S, 3)[None, :, :]
                 ).reshape(len(masks), k)
     X[:, k:] = np.asarray(row[k:], np.float32)[None, :]
+    if dfeat is not None:
+        X[:, k + 1:k + 1 + MLP_DFEAT] = np.asarray(dfeat, np.float32)
     return X
 
 
@@ -706,15 +711,15 @@ def gelu_tanh(z):
     return z / (np.float32(1.0) + np.exp(np.float32(-2.0) * u))
 
 
-def mlp_forward(params, row, masks, S):
-    """Outputs (n, S + 1) of an S-slot surrogate: S slot weights and
-    log(sigma^2 / sill)."""
-    h = bf16(mlp_features(row, masks, S))
+def mlp_forward(params, row, masks, S, dfeat=None):
+    """Outputs (n, S + 1 + MLP_DATA_SLOTS) of an S-slot surrogate: S slot
+    weights, log(sigma^2 / sill) and the weights of the data slots."""
+    h = bf16(mlp_features(row, masks, S, dfeat))
     layers = mlp_unpack(params)
     for li, (W, b) in enumerate(layers):
         z = h @ bf16(W).T + b
         h = bf16(gelu_tanh(z)) if li < 3 else z
-    return h[:, :S + 1]
+    return h[:, :S + 1 + MLP_DATA_SLOTS]
 
 
 def grid_slots(case: SeqsimCase, k=MLP_SLOTS):
@@ -736,21 +741,56 @@ def grid_slots(case: SeqsimCase, k=MLP_SLOTS):
 def node_slot_masks(case: SeqsimCase, plan: Plan, nodes, slot_cells):
     """Slot mask and per-neighbour slot of each node (-1 mask when a neighbour
     is a datum or outside the slot offsets)."""
+    masks, slots, _, _ = node_inputs(case, plan, nodes, slot_cells, 0)
+    return masks, slots
+
+
+def node_inputs(case: SeqsimCase, plan: Plan, nodes, slot_cells, data_slots):
+    """The surrogate's view of each node: slot mask, the output index of each
+    neighbour's weight (its slot, or S + 1 + q for the q-th datum among the
+    neighbours), the data features (MLP_DFEAT float32) and C(datum q, node) /
+    sill of the data slots. Mask -1 when a grid neighbour lies outside the
+    slot offsets or more than `data_slots` neighbours are data. The data
+    offsets are taken as the device takes them: in the kriging's scaled
+    coordinates (alpha x physical, cell centres alpha (origin + (c + 1/2)
+    spacing)) times 1 / (alpha range), scaled by MLP_FEAT_SCALE, in float64
+    and rounded once; the distance is the float32 length of the first."""
     nd = case.ndim
     nx = case.dims[0]
     ny = case.dims[1] if nd > 1 else 1
+    S = len(slot_cells)
     key = {tuple(o): s for s, o in enumerate(slot_cells)}
     N = case.n_nodes
+    alpha = scale_factor(case)
+    org = list(case.origin) + [0.0] * (3 - nd)
+    sp = list(case.spacing) + [0.0] * (3 - nd)
+    rg = list(case.ranges) * nd if len(case.ranges) == 1 else list(case.ranges)
+    irs = [1.0 / (alpha * rg[e]) if e < nd else 1.0 for e in range(3)]
     masks = np.zeros(len(nodes), np.int64)
     slots = np.full((len(nodes), plan.kmax), -1, np.int64)
+    dfeat = np.zeros((len(nodes), MLP_DFEAT), np.float32)
+    cd = np.zeros((len(nodes), MLP_DATA_SLOTS))
     for t, i in enumerate(nodes):
         ci = (i % nx, (i // nx) % ny, i // (nx * ny))
-        m = 0
+        cen = [alpha * org[e] + (ci[e] + 0.5) * (alpha * sp[e]) if e < nd else 0.0
+               for e in range(3)]
+        m, q = 0, 0
         for j in range(plan.nfound[i]):
             a = int(plan.nbr[i, j])
             if a >= N:
-                m = -1
-                break
+                if q >= data_slots:
+                    m = -1
+                    break
+                p = case.data_coords[a - N]
+                x = [alpha * float(p[e]) if e < nd else 0.0 for e in range(3)]
+                u = [(x[e] - cen[e]) * irs[e] for e in range(3)]
+                for e in range(3):
+                    dfeat[t, 3 * q + e] = np.float32(u[e] * MLP_FEAT_SCALE) if e < nd else 0.0
+                h = math.sqrt(sum(v * v for v in u[:nd]))
+                cd[t, q] = cov_u(case.family, case.sill, case.nugget, h) / case.sill
+                slots[t, j] = S + 1 + q
+                q += 1
+                continue
             c = (a % nx - ci[0], (a // nx) % ny - ci[1], a // (nx * ny) - ci[2])
             s = key.get(c)
             if s is None:
@@ -759,7 +799,11 @@ def node_slot_masks(case: SeqsimCase, plan: Plan, nodes, slot_cells):
             slots[t, j] = s
             m |= 1 << s
         masks[t] = m
-    return masks, slots
+        if m >= 0 and q > 0:
+            f = dfeat[t, :3]
+            dfeat[t, 3 * MLP_DATA_SLOTS] = np.sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2],
+                                                   dtype=np.float32)
+    return masks, slots, dfeat, cd
 
 
 def axis_variogram(fields, dims, axis, lags):

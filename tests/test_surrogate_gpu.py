@@ -87,9 +87,10 @@ def check_learned_plan(ctx, surrogate, case, tol=0.05):
     params, slots = surrogate.export()
     nodes = np.flatnonzero(learned)
     S = len(slots)
-    masks, nslot = O.node_slot_masks(case, o, nodes, O.grid_slots(case, S))
+    masks, nslot, dfeat, cd = O.node_inputs(case, o, nodes, O.grid_slots(case, S),
+                                            surrogate.info["n_data_slots"])
     assert (masks >= 0).all()
-    out = O.mlp_forward(params, O.mlp_request_row(case, slots), masks, S)
+    out = O.mlp_forward(params, O.mlp_request_row(case, slots), masks, S, dfeat)
     pred = np.zeros_like(g["lam"][nodes])
     for j in range(o.kmax):
         ok = nslot[:, j] >= 0
@@ -106,7 +107,9 @@ def check_learned_plan(ctx, surrogate, case, tol=0.05):
     cs = np.array([O.cov_u(case.family, 1.0, case.nugget, np.linalg.norm(s[:case.ndim] * scale))
                    for s in slots])
     lam = g["lam"][nodes]
-    cj = np.where(nslot >= 0, cs[np.maximum(nslot, 0)], 0.0)
+    cj = np.where((nslot >= 0) & (nslot < S), cs[np.clip(nslot, 0, S - 1)], 0.0)
+    for q in range(O.MLP_DATA_SLOTS):
+        cj = np.where(nslot == S + 1 + q, cd[:, q:q + 1], cj)
     chk = 1.0 - (lam * cj).sum(1)
     p = g["sigma"][nodes] ** 2
     assert np.all(np.abs(p - chk) <= tol * p + 1e-5)
@@ -191,6 +194,46 @@ def test_learned_batch(ctx, surrogate):
         # no learned node has a datum among its neighbours
         N = c.n_nodes
         assert not (loc["nbr"][~ex] >= N).any()
+
+
+@pytest.fixture(scope="module")
+def cond_surrogate(ctx):
+    """The config-2 variogram trained with data slots on conditional plans (400
+    data in the 32 x 32 x 16 grid): nodes with up to 4 data among their
+    neighbours are training systems."""
+    cases = [c2_case(600 + s, data=400) for s in range(6)]
+    plans = [_lib.SeqsimPlan(ctx, gpu_problem(c), keep=(c,)) for c in cases]
+    s = _lib.Surrogate(ctx, plans, max_samples=200_000, epochs=8, batch=1024, lr=2e-3, seed=3,
+                       data_slots=4)
+    say(f"conditional surrogate: {s.report.n_samples} systems, rms weight "
+        f"{s.report.rms_weight_last_epoch:.3e}")
+    return s
+
+
+def test_data_slots(ctx, cond_surrogate, surrogate):
+    """Learned nodes with conditioning data among their neighbours: the
+    surrogate trained with data slots uses its data inputs (86 -> ... -> 29),
+    one trained without has them at zero; in a conditional plan the
+    learned nodes with data neighbours equal the numpy forward pass with the
+    data features and pass the variance check including the data terms."""
+    assert cond_surrogate.info["n_data_slots"] == 4
+    assert surrogate.info["n_data_slots"] == 0
+    W = O.mlp_unpack(cond_surrogate.export()[0])
+    assert np.any(W[0][0][:, 73:86] != 0) and np.all(W[0][0][:, 86:] == 0)
+    assert np.any(W[3][0][25:29] != 0) and np.all(W[3][0][29:] == 0)
+    case = c2_case(607, data=400)
+    lp, g, o, learned = check_learned_plan(ctx, cond_surrogate, case)
+    N = case.n_nodes
+    ndat = (g["nbr"] >= N).sum(1)
+    with_data = learned & (ndat > 0)
+    say(f"learned {int(learned.sum())} of {N}; with data neighbours: {int(with_data.sum())} of "
+        f"{int((ndat > 0).sum())}")
+    assert with_data.sum() > 0
+    assert ndat[learned].max() <= O.MLP_DATA_SLOTS
+    err = (g["lam"] - o.lam)[with_data][o.nbr[with_data] >= 0]
+    rms = float(np.sqrt(np.mean(err ** 2)))
+    say(f"RMS weight error vs exact, learned nodes with data neighbours: {rms:.3e}")
+    assert rms <= 0.05
 
 
 def test_surrogate_errors(ctx):
