@@ -112,6 +112,12 @@ WORKLOADS = {
         kmax=24, radius=20.0, R=16, seed=1004,
         learned=dict(train_dims=(128, 128, 64), train_plans=3, samples=2_000_000, epochs=20,
                      batch=1024, lr=2e-3, tolerance=0.05)),
+    "seqsim-learned-256x256x128-conditional": dict(
+        dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01,
+        n_data=16000, kmax=24, radius=20.0, R=16, seed=1010,
+        learned=dict(train_dims=(128, 128, 64), train_plans=3, train_data=2000,
+                     samples=2_000_000, epochs=20, batch=1024, lr=2e-3, tolerance=0.05,
+                     data_slots=4)),
     "kriging-mean-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01,
         n_data=100_000, kmax=10, radius=0.0, R=1, seed=1006, mode="data_only", precision="fp64",
@@ -402,7 +408,7 @@ def axis_variogram(fields, dims, axis, lags):
                      for h in lags])
 
 
-def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol, z, N, R):
+def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol, z, dvals, N, R):
     """RMS weight error of a learned plan against the exact FP64 plan, and the
     experimental variograms of learned and exact realizations drawn from the
     same normals (paired), along each axis at lags 1..20; beside them the same
@@ -414,8 +420,12 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol
     ge = ep.export(fields=("lam", "sigma"))
     learned = (gl["flags"] & _lib.FLAG_LEARNED) != 0
     valid = gl["nbr"] >= 0
+    with_data = learned & (gl["nbr"] >= N).any(axis=1)
     d = gl["lam"] - ge["lam"]
     rms_learned = float(np.sqrt(np.mean(d[learned][valid[learned]] ** 2)))
+    rms_data = float(np.sqrt(np.mean(d[with_data][valid[with_data]] ** 2))) if with_data.any() \
+        else None
+    data_nodes = int((gl["nbr"] >= N).any(axis=1).sum())
     rms_all = float(np.sqrt(np.mean(d[valid] ** 2)))
     sig = np.abs(gl["sigma"] - ge["sigma"])[learned] / np.maximum(ge["sigma"][learned], 1e-300)
     del gl, ge, d, valid
@@ -425,7 +435,7 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol
     mp = _lib.SeqsimPlan(ctx, prob_masked)
     for plan in (lp, ep, mp):
         out = _lib.DeviceBuffer(ctx, (R, N))
-        plan.sweep(z, None, out, device_pointers=True)
+        plan.sweep(z, dvals, out, device_pointers=True)
         ctx.synchronize()
         host = np.empty((Rv, N))
         _lib.check(_lib.load().gsp_memcpy_to_host(ctx.handle, host.ctypes.data, out.ptr, Rv * N * 8))
@@ -448,6 +458,9 @@ def learned_accuracy(ctx, _lib, spec, prob_fp32r, prob_fp64, prob_masked, S, tol
     return {"rms_weight_error_learned_nodes": rms_learned, "rms_weight_error_all_nodes": rms_all,
             "sigma_rel_error_median": float(np.median(sig)), "sigma_rel_error_p99": float(np.quantile(sig, 0.99)),
             "learned_fraction": float(learned.mean()),
+            "nodes_with_data_neighbours": data_nodes,
+            "learned_with_data_neighbours": int(with_data.sum()),
+            "rms_weight_error_learned_with_data": rms_data,
             "variogram_error_paired_per_axis": [round(e, 5) for e in paired],
             "variogram_error_vs_model_per_axis": [round(e, 5) for e in vs_model],
             "exact_variogram_error_vs_model_per_axis": [round(e, 5) for e in exact_vs_model],
@@ -545,19 +558,28 @@ def batch_learned_accuracy(ctx, _lib, reqs, probs32, probs64, S, tol, R, n_req):
 
 
 def train_surrogate(ctx, _lib, spec, precision):
-    """Exact plans of the same variogram on smaller grids (random paths), then
-    device training on up to `samples` of their kriging systems."""
+    """Exact plans of the same variogram on smaller grids (random paths; with
+    `train_data` uniform conditioning points each when given), then device
+    training on up to `samples` of their kriging systems."""
     L = spec["learned"]
     plans, bufs = [], []
+    nd_train = L.get("train_data", 0)
     for t in range(L["train_plans"]):
         tdims = L["train_dims"]
         n = int(np.prod(tdims))
         path = np.random.default_rng(2000 + t).permutation(n).astype(np.int64)
         dp = _lib.DeviceBuffer.from_array(ctx, path)
         bufs.append(dp)
-        plans.append(_lib.SeqsimPlan(ctx, problem(dict(spec, dims=tdims), dp.ptr, None, precision, n)))
+        dd = None
+        if nd_train:
+            xy = np.random.default_rng(2100 + t).uniform(0, 1, (nd_train, len(tdims))) \
+                * np.asarray(tdims, dtype=np.float64)
+            dd = _lib.DeviceBuffer.from_array(ctx, np.ascontiguousarray(xy))
+            bufs.append(dd)
+        plans.append(_lib.SeqsimPlan(ctx, problem(dict(spec, dims=tdims, n_data=nd_train), dp.ptr,
+                                                  dd.ptr if dd is not None else None, precision, n)))
     S = _lib.Surrogate(ctx, plans, max_samples=L["samples"], epochs=L["epochs"], batch=L["batch"],
-                       lr=L["lr"], seed=L.get("seed", 0))
+                       lr=L["lr"], seed=L.get("seed", 0), data_slots=L.get("data_slots", 0))
     r = S.report
     for p in plans:
         p.close()
@@ -565,8 +587,9 @@ def train_surrogate(ctx, _lib, spec, precision):
                "epochs": L["epochs"], "batch": L["batch"], "ms": round(r.ms, 1),
                "rms_weight_first_epoch": r.rms_weight_first_epoch,
                "rms_weight_last_epoch": r.rms_weight_last_epoch,
+               "data_slots": S.info["n_data_slots"],
                "source": f"{L['train_plans']} exact plans of a {'x'.join(map(str, L['train_dims']))} grid, "
-                         "same variogram, random paths"}
+                         f"same variogram, random paths, {nd_train} data each"}
 
 
 def fftsim_problem(_lib, spec):
@@ -1100,7 +1123,7 @@ def main():
             masked = problems(_lib.FP32_REFINE)
             masked.search_mode = _lib.MASKED_KNN
             accuracy = learned_accuracy(ctx, _lib, spec, plan_arg, problems(_lib.FP64), masked, S,
-                                        tol, z, N, R)
+                                        tol, z, dvals, N, R)
             accuracy["tolerance"] = tol
 
     cpu = None
