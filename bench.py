@@ -105,7 +105,7 @@ WORKLOADS = {
     "seqsim-batch-512": dict(batch=512, side=(32, 512), R=8, seed=1005),
     "seqsim-batch-512-learned": dict(
         batch=512, side=(32, 512), R=8, seed=1005,
-        learned=dict(train_requests=96, train_seed=3005, samples=4_000_000, epochs=20, batch=1024,
+        learned=dict(train_requests=192, train_seed=3005, samples=4_000_000, epochs=20, batch=1024,
                      lr=2e-3, tolerance=0.05, accuracy_requests=32)),
     "seqsim-learned-256x256x128": dict(
         dims=(256, 256, 128), family=1, sill=1.0, ranges=(20.0, 20.0, 20.0), nugget=0.01, n_data=0,
