@@ -707,6 +707,275 @@ __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, con
   }
 }
 
+// ---- one system per lane (n <= NB) ------------------------------------------
+// Small systems need no exchange between lanes: each lane assembles, factors
+// and solves its own system in registers (lower triangles packed row by row),
+// with the operation order of k_krige16 restricted to the system's own rows
+// (the padded rows of the packed kernel leave the real rows' arithmetic
+// unchanged), so the results are bitwise those of k_krige16 and the oracle.
+// k_krige16 spends most of its instructions moving operands between lanes and
+// half its lanes idle on systems of at most 8 neighbours; here every
+// instruction works on 64 systems.
+template <int NB>
+__device__ __forceinline__ constexpr int tri(int j, int m) {
+  return j * (j + 1) / 2 + m;
+}
+
+// FP64 Cholesky and both substitutions on a[] (in place; packed_fp64's order).
+template <int NB>
+__device__ inline bool lane_fp64(double (&a)[NB * (NB + 1) / 2], const double (&c)[NB], int n,
+                                 double C0, double (&lam)[NB], double& sig) {
+  bool fail = false;
+  double dinv[NB];
+  static_for<0, NB>([&](auto colc) {
+    constexpr int col = decltype(colc)::value;
+    if (col < n) {
+      const double piv = a[tri<NB>(col, col)];
+      fail = fail || !(piv > 0.0);
+      const double lcc = __builtin_sqrt(piv);
+      const double rinv = 1.0 / lcc;
+      dinv[col] = rinv;
+      a[tri<NB>(col, col)] = lcc;
+      static_for<col + 1, NB>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if (r < n) a[tri<NB>(r, col)] = a[tri<NB>(r, col)] * rinv;
+      });
+      static_for<col + 1, NB>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if (m < n) {
+          const double lmc = a[tri<NB>(m, col)];
+          static_for<m, NB>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if (r < n) a[tri<NB>(r, m)] = a[tri<NB>(r, m)] - a[tri<NB>(r, col)] * lmc;
+          });
+        }
+      });
+    }
+  });
+  double s[NB], y[NB];
+  static_for<0, NB>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    s[r] = c[r];
+    y[r] = 0.0;
+  });
+  static_for<0, NB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if (m < n) {
+      y[m] = s[m] * dinv[m];
+      static_for<m + 1, NB>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if (r < n) s[r] = s[r] - a[tri<NB>(r, m)] * y[m];
+      });
+    }
+  });
+  static_for<0, NB>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    lam[r] = 0.0;
+  });
+  static_for<0, NB>([&](auto kc) {
+    constexpr int m = NB - 1 - decltype(kc)::value;
+    if (m < n) {
+      lam[m] = y[m] * dinv[m];
+      static_for<0, m>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        y[r] = y[r] - a[tri<NB>(m, r)] * lam[m];
+      });
+    }
+  });
+  double acc = 0.0;
+  static_for<0, NB>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    if (j < n) acc = acc + lam[j] * c[j];
+  });
+  double var = C0 - acc;
+  if (var < 0.0) var = 0.0;
+  sig = __builtin_sqrt(var);
+  return fail;
+}
+
+// Largest system solved one per lane. 8 takes 177-184 VGPRs; 10 fits in
+// 227-252 without spilling but, holding more of the register file, slowed the
+// level kernels beside it (batch of 512: step 78.6 -> 80.4 ms); 12 spills
+// into AGPRs.
+constexpr int kLaneNB = 8;
+// Fewest such systems for which the lane kernel runs (GSP_LANE_MIN
+// overrides): each lane works through a whole system serially, so a launch of
+// few waves waits on one long chain (a 10,000-node plan measured 51 us
+// against 23 us in k_krige16).
+constexpr int64_t kLaneMin = 32768;
+
+template <int NB, int PREC>
+__global__ __launch_bounds__(256) void k_krige_lane(const PlanArgs A, const int32_t* list,
+                                                    int32_t count) {
+  constexpr int T = NB * (NB + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;; slot += (int64_t)gridDim.x * blockDim.x) {
+    // wave-uniform exit: every lane of a wave leaves together
+    if (!__ballot(slot < count)) break;
+    const bool act = slot < count;
+    const int gi = act ? list[slot] : 0;
+    const int p = act ? A.node_req[gi] : 0;
+    const Req& R = A.reqs[p];
+    const int n = act ? A.nfound[gi] : 0;
+    const bool solve = act && n >= R.kmin;
+    const double C0 = R.M.sill;
+    double lam[NB];
+    static_for<0, NB>([&](auto jc) { lam[decltype(jc)::value] = 0.0; });
+    double sig = R.prior_sigma;
+    int flag = act && !solve ? GSP_FLAG_FEW : 0;
+    bool fail = false, fell_back = false;
+    if (solve && n > 0) {
+      double x[NB][3], c[NB], a[T];
+      static_for<0, NB>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        x[j][0] = x[j][1] = x[j][2] = 0.0;
+        if (j < n)
+          neighbour_coords(A, R, A.nbr[(int64_t)gi * A.K + j], x[j][0], x[j][1], x[j][2]);
+      });
+      double t0, t1, t2;
+      node_coords(R, gi - R.node_off, t0, t1, t2);
+      static_for<0, NB>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        c[j] = j < n ? cov_points(R.M, x[j][0], x[j][1], x[j][2], t0, t1, t2) : 0.0;
+        static_for<0, j + 1>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          a[tri<NB>(j, m)] = j < n ? cov_points(R.M, x[j][0], x[j][1], x[j][2], x[m][0], x[m][1],
+                                                 x[m][2])
+                                   : 0.0;
+        });
+      });
+      if constexpr (PREC == GSP_FP32_REFINE) {
+        const float tau = (float)(C0 * 0x1p-12);
+        float f[T], dinv[NB];
+        static_for<0, T>([&](auto ec) { f[decltype(ec)::value] = (float)a[decltype(ec)::value]; });
+        bool f32fail = false;
+        static_for<0, NB>([&](auto colc) {
+          constexpr int col = decltype(colc)::value;
+          if (col < n) {
+            const float piv = f[tri<NB>(col, col)];
+            f32fail = f32fail || !(piv > tau);
+            const float lcc = __builtin_sqrtf(piv);
+            const float rinv = 1.0f / lcc;
+            dinv[col] = rinv;
+            f[tri<NB>(col, col)] = lcc;
+            static_for<col + 1, NB>([&](auto rc) {
+              constexpr int r = decltype(rc)::value;
+              if (r < n) f[tri<NB>(r, col)] = f[tri<NB>(r, col)] * rinv;
+            });
+            static_for<col + 1, NB>([&](auto mc) {
+              constexpr int m = decltype(mc)::value;
+              if (m < n) {
+                const float lmc = f[tri<NB>(m, col)];
+                static_for<m, NB>([&](auto rc) {
+                  constexpr int r = decltype(rc)::value;
+                  if (r < n) f[tri<NB>(r, m)] = f[tri<NB>(r, m)] - f[tri<NB>(r, col)] * lmc;
+                });
+              }
+            });
+          }
+        });
+        float rhs[NB], sol[NB], l[NB];
+        static_for<0, NB>([&](auto rc) {
+          constexpr int r = decltype(rc)::value;
+          rhs[r] = (float)c[r];
+          sol[r] = 0.0f;
+        });
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+          float s[NB], y[NB];
+          static_for<0, NB>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            s[r] = rhs[r];
+            y[r] = 0.0f;
+            l[r] = 0.0f;
+          });
+          static_for<0, NB>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (m < n) {
+              y[m] = s[m] * dinv[m];
+              static_for<m + 1, NB>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                if (r < n) s[r] = s[r] - f[tri<NB>(r, m)] * y[m];
+              });
+            }
+          });
+          static_for<0, NB>([&](auto kc) {
+            constexpr int m = NB - 1 - decltype(kc)::value;
+            if (m < n) {
+              l[m] = y[m] * dinv[m];
+              static_for<0, m>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                y[r] = y[r] - f[tri<NB>(m, r)] * l[m];
+              });
+            }
+          });
+          if (pass == 0) {
+            // residual of the FP32 solution against the FP64 matrix (full rows)
+            static_for<0, NB>([&](auto rc) {
+              constexpr int r = decltype(rc)::value;
+              sol[r] = l[r];
+              double res = c[r];
+              static_for<0, NB>([&](auto mc) {
+                constexpr int m = decltype(mc)::value;
+                if (m < n)
+                  res = res - a[m <= r ? tri<NB>(r, m) : tri<NB>(m, r)] * (double)l[m];
+              });
+              rhs[r] = (float)res;
+            });
+          }
+        }
+        static_for<0, NB>([&](auto rc) {
+          constexpr int r = decltype(rc)::value;
+          lam[r] = r < n ? (double)sol[r] + (double)l[r] : 0.0;
+        });
+        double acc = 0.0;
+        static_for<0, NB>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if (j < n) acc = acc + lam[j] * c[j];
+        });
+        double var = C0 - acc;
+        if (var < 0.0) var = 0.0;
+        sig = __builtin_sqrt(var);
+        // systems whose FP32 factor failed are solved again in FP64
+        if (f32fail) {
+          fail = lane_fp64<NB>(a, c, n, C0, lam, sig);
+          fell_back = true;
+        }
+      } else {
+        fail = lane_fp64<NB>(a, c, n, C0, lam, sig);
+      }
+    }
+    if (solve) {
+      if (fell_back) flag |= GSP_FLAG_FP64_FALLBACK;
+      if (fail) {
+        flag |= GSP_FLAG_FAIL;
+        sig = R.prior_sigma;
+      }
+    } else {
+      sig = R.prior_sigma;
+    }
+    if (act) {
+      double* lr = A.lam + (int64_t)gi * A.K;
+      static_for<0, NB>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j < A.K) lr[j] = (j < n && solve && !fail) ? lam[j] : 0.0;
+      });
+      for (int j = NB; j < A.K; ++j) lr[j] = 0.0;
+      A.sigma[gi] = sig;
+      A.flags[gi] = (uint8_t)flag;
+    }
+    // counters: one atomic per wavefront and counter
+    const int prior = __popcll(__ballot(act && (flag & (GSP_FLAG_FEW | GSP_FLAG_FAIL))));
+    const int failed = __popcll(__ballot(act && (flag & GSP_FLAG_FAIL)));
+    const int fb = __popcll(__ballot(act && (flag & GSP_FLAG_FP64_FALLBACK)));
+    if (lane == 0) {
+      if (prior) atomicAdd(&A.counters[CTR_PRIOR], prior);
+      if (failed) atomicAdd(&A.counters[CTR_FAIL], failed);
+      if (fb) atomicAdd(&A.counters[CTR_FALLBACK], fb);
+    }
+  }
+}
+
 // ---- two systems per wavefront (16 < n <= KB, KB = 24 or 32) ---------------
 // Lane 32 g + r holds row r of system g (rows r < KB take part). The Cholesky step of column col
 // scales the column in every row's lane, stores it to LDS and every lane of
@@ -2627,15 +2896,29 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
                          0, s, plan->nfound.p, ids, (int32_t)n_exact, nhist.p, list.p);
     tr.mark("bucket by size");
     TimedLaunch tl(ctx, GSP_K_PLAN);
-    const int64_t small = upto[16];
+    // n <= kLaneNB: one system per lane (when there are enough of them);
+    // the rest up to n = 16: four per wavefront
+    int64_t lane_min = kLaneMin;
+    if (const char* env = std::getenv("GSP_LANE_MIN")) lane_min = std::atoll(env);
+    const int64_t tiny = upto[kLaneNB] >= lane_min ? upto[kLaneNB] : 0;
+    if (tiny > 0) {
+      const int blocks = grid_for(tiny, 256, 65536);
+      if (precision == GSP_FP32_REFINE)
+        hipLaunchKernelGGL((k_krige_lane<kLaneNB, GSP_FP32_REFINE>), dim3(blocks), dim3(256), 0, s,
+                           A, list.p, (int32_t)tiny);
+      else
+        hipLaunchKernelGGL((k_krige_lane<kLaneNB, GSP_FP64>), dim3(blocks), dim3(256), 0, s, A,
+                           list.p, (int32_t)tiny);
+    }
+    const int64_t small = upto[16] - tiny;
     if (small > 0) {
       const int blocks = grid_for((small + 3) / 4, k16Waves, 65536);
       if (precision == GSP_FP32_REFINE)
         hipLaunchKernelGGL(k_krige16<GSP_FP32_REFINE>, dim3(blocks), dim3(64 * k16Waves), 0, s, A,
-                           list.p, (int32_t)small);
+                           list.p + tiny, (int32_t)small);
       else
         hipLaunchKernelGGL(k_krige16<GSP_FP64>, dim3(blocks), dim3(64 * k16Waves), 0, s, A,
-                           list.p, (int32_t)small);
+                           list.p + tiny, (int32_t)small);
     }
     launch_mid<24>(ctx, A, precision, list.p + upto[16], upto[24] - upto[16], s);
     launch_mid<32>(ctx, A, precision, list.p + upto[24], upto[32] - upto[24], s);

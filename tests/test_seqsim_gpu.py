@@ -103,6 +103,17 @@ def test_plan_parity_fp32_refine(ctx, i):
     assert rel_err(gp.export()["lam"], ref.lam) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", [O.FP64, O.FP32_REFINE])
+@pytest.mark.parametrize("i", [0, 1, 2, 3, 4, 7])
+def test_plan_parity_lane_kernel(ctx, i, precision, monkeypatch):
+    """Systems of at most 8 neighbours solved one per lane (forced on these
+    small plans, which would otherwise stay in the four-per-wavefront kernel):
+    bitwise equal to the oracle in both precisions, including prior draws,
+    failed systems (duplicated data) and FP64 fallbacks."""
+    monkeypatch.setenv("GSP_LANE_MIN", "0")
+    check_plan(ctx, make_case(i, O.KNN_THEN_MASK, precision=precision))
+
+
 @pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
 @pytest.mark.parametrize("i", [0, 1, 3, 4, 7])
 def test_sweep_parity(ctx, i, mode):
