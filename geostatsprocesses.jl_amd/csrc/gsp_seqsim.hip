@@ -1355,7 +1355,7 @@ __global__ void k_batch_path(int64_t* path, const int64_t* node_off, int32_t* no
 }
 
 // pos[i] = q for the node id i in the low 32 bits of path[q], flagging
-// out-of-range ids and repeats.
+// out-of-range ids; plain stores (a repeated id leaves one of its positions).
 __global__ void k_positions(const int64_t* path, int64_t N, int32_t* pos, int32_t* counters) {
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N;
        q += (int64_t)gridDim.x * blockDim.x) {
@@ -1365,8 +1365,19 @@ __global__ void k_positions(const int64_t* path, int64_t N, int32_t* pos, int32_
       atomicOr(&counters[CTR_POS_ERR], 1);
       continue;
     }
-    const int old = atomicExch(&pos[i], (int)q);
-    if (old != -1) atomicOr(&counters[CTR_POS_ERR], 2);
+    pos[i] = (int)q;
+  }
+}
+
+// Repeats: a path position whose id's recorded position is another one holds
+// an id that occurs twice (N positions, so a repeat also means a missing id).
+__global__ void k_positions_check(const int64_t* path, int64_t N, const int32_t* pos,
+                                  int32_t* counters) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = path[q];
+    const int64_t i = v & 0xFFFFFFFF;
+    if (v >= 0 && i < N && pos[i] != (int)q) atomicOr(&counters[CTR_POS_ERR], 2);
   }
 }
 
@@ -2711,6 +2722,8 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, pos.p, -1, NT);
   hipLaunchKernelGGL(k_fill_i32, dim3(gn), dim3(256), 0, s, plan->level.p, -1, NT);
   hipLaunchKernelGGL(k_positions, dim3(gn), dim3(256), 0, s, path.p, NT, pos.p, ctx->counters.p);
+  hipLaunchKernelGGL(k_positions_check, dim3(gn), dim3(256), 0, s, path.p, NT, pos.p,
+                     ctx->counters.p);
   GSP_HIP(hipGetLastError());
   int32_t pos_err = 0;
   GSP_HIP(hipMemcpyAsync(&pos_err, ctx->counters.p + CTR_POS_ERR, sizeof(int32_t),
