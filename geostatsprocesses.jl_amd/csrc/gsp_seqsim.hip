@@ -793,11 +793,12 @@ __device__ inline bool lane_fp64(double (&a)[NB * (NB + 1) / 2], const double (&
   return fail;
 }
 
-// Largest system solved one per lane. 8 takes 177-184 VGPRs; 10 fits in
-// 227-252 without spilling but, holding more of the register file, slowed the
-// level kernels beside it (batch of 512: step 78.6 -> 80.4 ms); 12 spills
-// into AGPRs.
-constexpr int kLaneNB = 8;
+// Largest system solved one per lane, by precision. FP32 + refinement: 8
+// (184 VGPRs); 10 fits in 252 without spilling but, holding more of the
+// register file, slowed the level kernels beside it (batch of 512: step 78.6
+// -> 80.4 ms). FP64: 10 (227 VGPRs). 12 spills into AGPRs in both.
+constexpr int kLaneNB32 = 8;
+constexpr int kLaneNB64 = 10;
 // Fewest such systems for which the lane kernel runs (GSP_LANE_MIN
 // overrides): each lane works through a whole system serially, so a launch of
 // few waves waits on one long chain (a 10,000-node plan measured 51 us
@@ -2909,18 +2910,19 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
                          0, s, plan->nfound.p, ids, (int32_t)n_exact, nhist.p, list.p);
     tr.mark("bucket by size");
     TimedLaunch tl(ctx, GSP_K_PLAN);
-    // n <= kLaneNB: one system per lane (when there are enough of them);
-    // the rest up to n = 16: four per wavefront
+    // n <= kLaneNB32 / kLaneNB64: one system per lane (when there are enough
+    // of them); the rest up to n = 16: four per wavefront
     int64_t lane_min = kLaneMin;
     if (const char* env = std::getenv("GSP_LANE_MIN")) lane_min = std::atoll(env);
-    const int64_t tiny = upto[kLaneNB] >= lane_min ? upto[kLaneNB] : 0;
+    const int lane_nb = precision == GSP_FP32_REFINE ? kLaneNB32 : kLaneNB64;
+    const int64_t tiny = upto[lane_nb] >= lane_min ? upto[lane_nb] : 0;
     if (tiny > 0) {
       const int blocks = grid_for(tiny, 256, 65536);
       if (precision == GSP_FP32_REFINE)
-        hipLaunchKernelGGL((k_krige_lane<kLaneNB, GSP_FP32_REFINE>), dim3(blocks), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_krige_lane<kLaneNB32, GSP_FP32_REFINE>), dim3(blocks), dim3(256), 0, s,
                            A, list.p, (int32_t)tiny);
       else
-        hipLaunchKernelGGL((k_krige_lane<kLaneNB, GSP_FP64>), dim3(blocks), dim3(256), 0, s, A,
+        hipLaunchKernelGGL((k_krige_lane<kLaneNB64, GSP_FP64>), dim3(blocks), dim3(256), 0, s, A,
                            list.p, (int32_t)tiny);
     }
     const int64_t small = upto[16] - tiny;

@@ -106,10 +106,11 @@ def test_plan_parity_fp32_refine(ctx, i):
 @pytest.mark.parametrize("precision", [O.FP64, O.FP32_REFINE])
 @pytest.mark.parametrize("i", [0, 1, 2, 3, 4, 7])
 def test_plan_parity_lane_kernel(ctx, i, precision, monkeypatch):
-    """Systems of at most 8 neighbours solved one per lane (forced on these
-    small plans, which would otherwise stay in the four-per-wavefront kernel):
-    bitwise equal to the oracle in both precisions, including prior draws,
-    failed systems (duplicated data) and FP64 fallbacks."""
+    """Systems of at most 8 (FP32 + refinement) or 10 (FP64) neighbours solved
+    one per lane (forced on these small plans, which would otherwise stay in
+    the four-per-wavefront kernel): bitwise equal to the oracle in both
+    precisions, including prior draws, failed systems (duplicated data) and
+    FP64 fallbacks."""
     monkeypatch.setenv("GSP_LANE_MIN", "0")
     check_plan(ctx, make_case(i, O.KNN_THEN_MASK, precision=precision))
 
