@@ -141,7 +141,7 @@ def test_config2_fp64_parity(ctx):
 
 def test_config2_mixed_precision_error(ctx):
     """The mixed form (FP32 MFMA trailing updates, FP64 diagonal tiles, one FP64
-    refinement step of the conditioning solve) against the FP64 oracle at config 2.
+    refinement step of the conditioning solve) against the FP64 oracle on the 64 x 64 grid.
     Bounds: d2 comes from the refined solve, x1 = x0 + C11~^-1 (z1 - C11 x0), whose
     error is (kappa(C11) u32)^2 relative (kappa(C11) ~ 1e2 here): <= 1e-11. The
     factor is backward stable in FP32 accumulation: |L~L~' - S| / |S| <= 64 u32

@@ -30,9 +30,9 @@ def sweep_both(ctx, plan, z, vals):
 
 
 def test_noalloc_sweep_matches_sweep_twice(ctx):
-    # first call: config 1 conditional, R = 10
+    # first call: the 100 x 100 grid with 50 data, R = 10
     case, z = config1(conditional=True, R=10)
-    say("noalloc: config1 conditional R=10")
+    say("noalloc: 100 x 100 grid, 50 data, R=10")
     plan = gpu_plan(ctx, case)
     ref, got = sweep_both(ctx, plan, z, case.data_vals)
     np.testing.assert_array_equal(got, ref)
