@@ -201,31 +201,35 @@ def problem(spec, path_ptr, data_ptr, precision, N):
 
 # kernels of each bench name in the committed traffic profiles
 PROFILE_KERNELS = {"search": ["k_search<false>", "k_search<true>"],
-                   "krige": ["k_krige16<1>", "k_krige<24, 1>", "k_krige16<0>", "k_krige<24, 0>"],
+                   "krige": ["k_krige_lane<8, 1>", "k_krige_lane<10, 0>", "k_krige16<1>",
+                             "k_krige16<0>", "k_krige<24, 1>", "k_krige<24, 0>", "k_krige<32, 1>",
+                             "k_krige<32, 0>", "k_krige32<24, 1>", "k_krige32<24, 0>",
+                             "k_krige32<32, 1>", "k_krige32<32, 0>"],
                    "sweep": ["k_sweep_level"], "learned": ["k_mlp_infer"],
                    "stats": ["k_ens_stats<false>"]}
 
 
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of the kernel family `kernel` (read + written,
-    counter-calibrated, averaged over its dispatches) from
-    profiles/<workload>_traffic.md, which rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-    runs of this script produced; None when no profile covers it."""
+    counter-calibrated) from profiles/<workload>_traffic.md, which rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE runs of this script produced: the sum over the
+    family's kernels of each one's average per dispatch (a family launch, such
+    as one plan's kriging, runs each of its kernels once); None when no profile
+    covers it."""
     path = os.path.join(ROOT, "profiles", workload.replace("-", "_") + "_traffic.md")
     if not os.path.exists(path):
         return None
-    total, n = 0.0, 0
+    total, found = 0.0, False
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) >= 4 and cells[0] in PROFILE_KERNELS.get(kernel, []):
             try:
-                dispatches = int(cells[1])
                 per = (float(cells[2]) + float(cells[3])) * 1e6
             except ValueError:
                 continue
-            total += per * dispatches
-            n += dispatches
-    return int(total / n) if n else None
+            total += per
+            found = True
+    return int(total) if found else None
 
 
 def cpu_share():
