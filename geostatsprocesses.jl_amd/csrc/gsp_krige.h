@@ -236,6 +236,7 @@ template <int KB, int PREC>
 __global__ __launch_bounds__(64 * plan_waves(KB)) __attribute__((amdgpu_waves_per_eu(krige_waves_per_simd(KB))))
 void k_krige(const PlanArgs A, const int32_t* list,
                                                                int32_t count) {
+  __builtin_amdgcn_s_setprio(3);  // issue ahead of the level kernels beside the solves
   constexpr int kW = plan_waves(KB);
   __shared__ KrigeLds<KB, PREC> s_krige[kW];
   const int lane = threadIdx.x & 63;

@@ -531,6 +531,7 @@ constexpr int k16Waves = 1;
 template <int PREC>
 __global__ __launch_bounds__(64 * k16Waves) void k_krige16(const PlanArgs A, const int32_t* list,
                                                          int32_t count) {
+  __builtin_amdgcn_s_setprio(3);  // issue ahead of the level kernels beside the solves
   __shared__ Krige16Lds<PREC> s_lds[k16Waves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -808,6 +809,7 @@ constexpr int64_t kLaneMin = 32768;
 template <int NB, int PREC>
 __global__ __launch_bounds__(256) void k_krige_lane(const PlanArgs A, const int32_t* list,
                                                     int32_t count) {
+  __builtin_amdgcn_s_setprio(3);  // issue ahead of the level kernels beside the solves
   constexpr int T = NB * (NB + 1) / 2;
   const int lane = threadIdx.x & 63;
   for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;; slot += (int64_t)gridDim.x * blockDim.x) {
@@ -1082,6 +1084,7 @@ constexpr int k32Waves = 1;
 template <int KB, int PREC>
 __global__ __launch_bounds__(64 * k32Waves) void k_krige32(const PlanArgs A, const int32_t* list,
                                                          int32_t count) {
+  __builtin_amdgcn_s_setprio(3);  // issue ahead of the level kernels beside the solves
   __shared__ Krige32Lds<KB, PREC> s_lds[k32Waves];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;

@@ -740,6 +740,7 @@ __global__ __launch_bounds__(64) void k_mfma_probe_bf16(double* out) {
 // their features, LDS or outputs.
 template <int SLOTS, int CT, bool DATA>
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const InferArgs A) {
+  __builtin_amdgcn_s_setprio(3);  // issue ahead of the level kernels running beside it
   constexpr int kD = DATA ? kDataSlots : 0;
   constexpr int kOutI = out_rows(SLOTS), kUsedOut = SLOTS + 1 + kD, kNodes = 16 * CT;
   __shared__ FragBuf s_w0, s_w1, s_w2;
