@@ -14,9 +14,10 @@ struct Req {
   int32_t hB, n_data;    // hash cell width in grid cells; conditioning data
   int32_t node_off, data_off;
   int32_t learned;       // s + 1: the plan's surrogate s may give this request's weights
-  int32_t pad_;
+  int32_t tile_off;      // data-only tile search: first tile of this request
   int64_t table_off, cell_off;  // first entry of the offset table / of hstart
   double r2, alpha, prior_sigma, mean;
+  double trho;           // data-only tile search: first staging radius (physical)
   double org[3], sp[3];  // physical grid
   double ao[3], as[3];   // scaled grid (alpha*origin, alpha*spacing)
   double hcs[3];         // data-hash cell size (physical)

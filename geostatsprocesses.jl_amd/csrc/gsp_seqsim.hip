@@ -70,14 +70,13 @@ __device__ inline double data_d2(const RQ& R, const double* x, const int* t) {
 // (d2, index) and the mask. Writes nbr[i][0..K) and nfound[i].
 // RING: searches without a distance bound (data only, or masked with no
 // ball) visit rings of hash cells; the other searches never lack a bound
-// (the offset table always yields kmax grid candidates), and keep the simpler
-// scan of every cell for that case.
+// (the offset table always yields kmax grid candidates).
 // MI355X dispatches consecutive workgroups round-robin to its 8 XCDs, each
 // with its own L2.
 constexpr int kXcds = 8;
 
 template <bool RING>
-__global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
+__global__ __launch_bounds__(256, 8) void k_search(const PlanArgs A) {
   __shared__ int s_gidx[kWaves][64];
   __shared__ double s_gd2[kWaves][64];
   __shared__ int s_didx[kWaves][128];
@@ -235,7 +234,7 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
       };
       const int32_t* hs = A.hstart + R.cell_off;
       const auto hsv = const_view(hs);  // wave-uniform list bounds: scalar loads
-      // Searches without a bound gather item ranges (runs of adjacent cells
+      // Searches over many cells gather item ranges (runs of adjacent cells
       // along x) in lanes, up to 64 at a time, and scan them as one flat list:
       // few, full 64-lane passes instead of one pass per cell.
       int rs = 0, rn = 0, nr = 0;  // this lane's range (first item, count); ranges held
@@ -289,16 +288,21 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
           lo[d] = (int)fl;
           hi[d] = (int)fh;
         }
-        for (int cz = lo[2]; cz <= hi[2]; ++cz)
-          for (int cy = lo[1]; cy <= hi[1]; ++cy) {
-            const int row = R.hd[0] * (cy + R.hd[1] * cz);
-            if constexpr (RING) {
-              add_range(hsv[row + lo[0]], hsv[row + hi[0] + 1]);
-            } else {
-              for (int cx = lo[0]; cx <= hi[0]; ++cx) scan_list(hs[row + cx], hs[row + cx + 1]);
-            }
+        // the cells lo[0]..hi[0] of one (cy, cz) row hold one run of items:
+        // one lane per row reads its run's bounds (a single round trip for up
+        // to 64 rows) and flush() scans the runs as one list
+        const int nyr = hi[1] - lo[1] + 1;
+        const int nrows = nyr * (hi[2] - lo[2] + 1);
+        for (int r0 = 0; r0 < nrows; r0 += 64) {
+          const int rr = r0 + lane;
+          if (rr < nrows) {
+            const int row = R.hd[0] * (lo[1] + rr % nyr + R.hd[1] * (lo[2] + rr / nyr));
+            rs = hs[row + lo[0]];
+            rn = hs[row + hi[0] + 1] - rs;
           }
-        if constexpr (RING) flush();
+          nr = nrows - r0 < 64 ? nrows - r0 : 64;
+          flush();
+        }
       } else {
         // no bound yet: rings of cells at Chebyshev distance r = 0, 1, ... from
         // the node's cell, until the kmax-th candidate is closer than every
@@ -436,6 +440,198 @@ __global__ __launch_bounds__(256) void k_search(const PlanArgs A) {
         A.indeg[gi] = c;
         A.mark[gi] = c == 0 ? 1 : 0;
       }
+    }
+  }
+}
+
+// ---- data-only search, one node per lane ----------------------------------
+// The data-only search (the k nearest data of every node, no grid
+// neighbours, no dependencies) of plans whose requests all search data only,
+// with box hash cells and kmax <= KT. A workgroup takes a tile of 256 nodes of
+// one request (8 x 8 x 4 in 3D, 16 x 16 in 2D, 256 in 1D), stages the data of
+// every hash cell within rho of the tile in LDS, and each lane keeps the KT
+// best (d2, index) keys of its node in registers, sorted. A node's result is
+// final when its kmax-th distance (the ball radius when it has fewer) lies
+// strictly inside the cells visited: no datum outside them can be closer (data
+// outside the grid sit in the edge cells, which every region reaching the edge
+// holds). Otherwise the workgroup visits again with rho doubled, until every
+// node is final or every cell has been visited, so the result is the exact
+// top kmax under the order of k_search.
+constexpr int kTileNodes = 256;
+constexpr int kTileItems = 512;  // data staged in LDS per pass
+
+__host__ __device__ inline void tile_shape(int ndim, int* t) {
+  t[0] = ndim >= 3 ? 8 : (ndim == 2 ? 16 : 256);
+  t[1] = ndim >= 3 ? 8 : (ndim == 2 ? 16 : 1);
+  t[2] = ndim >= 3 ? 4 : 1;
+}
+
+template <int KT>
+__global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, int32_t n_req) {
+  using Scan = hipcub::BlockScan<int, kTileNodes>;
+  __shared__ typename Scan::TempStorage s_scan;
+  __shared__ double s_x[kTileItems][3];
+  __shared__ int s_id[kTileItems];
+  __shared__ int s_rs[kTileNodes];       // first item of each row run
+  __shared__ int s_pre[kTileNodes + 1];  // items before each row run
+  const int tid = threadIdx.x;
+  const int tile = blockIdx.x;
+  // the request of this tile: the last one whose first tile is <= tile
+  int p = 0;
+  {
+    int lo = 0, hi = n_req - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (const_view(A.reqs)[mid].tile_off <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    p = lo;
+  }
+  const auto& R = const_view(A.reqs)[p];
+  int ts[3];
+  tile_shape(R.ndim, ts);
+  const int ntx = (R.dims[0] + ts[0] - 1) / ts[0], nty = (R.dims[1] + ts[1] - 1) / ts[1];
+  const int tl = tile - R.tile_off;
+  const int t0[3] = {(tl % ntx) * ts[0], ((tl / ntx) % nty) * ts[1], (tl / (ntx * nty)) * ts[2]};
+  const int ti[3] = {t0[0] + tid % ts[0], t0[1] + (tid / ts[0]) % ts[1], t0[2] + tid / (ts[0] * ts[1])};
+  const bool live = ti[0] < R.dims[0] && ti[1] < R.dims[1] && ti[2] < R.dims[2];
+  // the node's centre, as data_d2 forms it
+  double c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c[d] = R.org[d] + ((double)ti[d] + 0.5) * R.sp[d];
+  const int kmax = R.kmax;
+  const int32_t* hs = A.hstart + R.cell_off;
+
+  double kd[KT];
+  int ki[KT];
+  double rho = R.trho;  // > 0
+  for (int pass = 0;; ++pass, rho = rho + rho) {
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+      kd[j] = __builtin_inf();
+      ki[j] = 0x7fffffff;
+    }
+    // the cells overlapping the tile's node centres widened by rho (every
+    // cell after 40 doublings, which ends the loop)
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= R.ndim) break;
+      if (pass >= 40) {
+        hi[d] = R.hd[d] - 1;
+        continue;
+      }
+      const int last = (t0[d] + ts[d] < R.dims[d] ? t0[d] + ts[d] : R.dims[d]) - 1;
+      const double cl = R.org[d] + ((double)t0[d] + 0.5) * R.sp[d];
+      const double ch = R.org[d] + ((double)last + 0.5) * R.sp[d];
+      double fl = __builtin_floor((cl - rho - R.org[d]) / R.hcs[d]);
+      double fh = __builtin_floor((ch + rho - R.org[d]) / R.hcs[d]);
+      fl = fl < 0.0 ? 0.0 : (fl > R.hd[d] - 1 ? R.hd[d] - 1 : fl);
+      fh = fh < 0.0 ? 0.0 : (fh > R.hd[d] - 1 ? R.hd[d] - 1 : fh);
+      lo[d] = (int)fl;
+      hi[d] = (int)fh;
+    }
+    // the cells lo[0]..hi[0] of one (cy, cz) row hold one run of items
+    const int nyr = hi[1] - lo[1] + 1;
+    const int nrows = nyr * (hi[2] - lo[2] + 1);
+    for (int r0 = 0; r0 < nrows; r0 += kTileNodes) {
+      const int rr = r0 + tid;
+      int len = 0;
+      if (rr < nrows) {
+        const int row = R.hd[0] * (lo[1] + rr % nyr + R.hd[1] * (lo[2] + rr / nyr));
+        const int a = hs[row + lo[0]];
+        s_rs[tid] = a;
+        len = hs[row + hi[0] + 1] - a;
+      }
+      int pre = 0, total = 0;
+      Scan(s_scan).ExclusiveSum(len, pre, total);
+      s_pre[tid] = pre;
+      const int nr = nrows - r0 < kTileNodes ? nrows - r0 : kTileNodes;
+      for (int i0 = 0; i0 < total; i0 += kTileItems) {
+        __syncthreads();  // s_pre / s_rs written, the previous items consumed
+        const int n = total - i0 < kTileItems ? total - i0 : kTileItems;
+        for (int u = tid; u < n; u += kTileNodes) {
+          const int t = i0 + u;
+          int a = 0, b = nr - 1;  // the run holding item t: last with s_pre <= t
+          while (a < b) {
+            const int m = (a + b + 1) >> 1;
+            if (s_pre[m] <= t) a = m;
+            else b = m - 1;
+          }
+          const int di = A.hitems[s_rs[a] + (t - s_pre[a])];
+          const double* x = A.data + (size_t)di * 3;
+          s_x[u][0] = x[0];
+          s_x[u][1] = x[1];
+          s_x[u][2] = x[2];
+          s_id[u] = A.NT + di;
+        }
+        __syncthreads();
+        if (live) {
+          for (int u = 0; u < n; ++u) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              if (d >= R.ndim) break;
+              const double w = s_x[u][d] - c[d];
+              d2 = d2 + w * w;
+            }
+            const int id = s_id[u];
+            if (d2 <= R.r2 && key_less(d2, id, kd[KT - 1], ki[KT - 1])) {
+              // insert into the sorted list: slot j takes slot j-1's key
+              // when the new key precedes that one
+              static_for<0, KT - 1>([&](auto m) {
+                constexpr int j = KT - 1 - decltype(m)::value;
+                const bool up = key_less(d2, id, kd[j - 1], ki[j - 1]);
+                const bool here = key_less(d2, id, kd[j], ki[j]);
+                kd[j] = up ? kd[j - 1] : (here ? d2 : kd[j]);
+                ki[j] = up ? ki[j - 1] : (here ? id : ki[j]);
+              });
+              if (key_less(d2, id, kd[0], ki[0])) {
+                kd[0] = d2;
+                ki[0] = id;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();  // before the next rows overwrite s_rs / s_pre
+    }
+    // final: the kmax-th key (or the ball) closer than every unvisited cell
+    int nf = 0;
+    double D = R.r2;
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+      nf += (j < kmax && ki[j] != 0x7fffffff) ? 1 : 0;
+      if (j == kmax - 1 && ki[j] != 0x7fffffff) D = kd[j];
+    }
+    double gap = __builtin_inf();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= R.ndim) break;
+      if (lo[d] > 0) {
+        const double g = c[d] - (R.org[d] + (double)lo[d] * R.hcs[d]) - 0x1p-30 * R.hcs[d];
+        gap = g < gap ? g : gap;
+      }
+      if (hi[d] < R.hd[d] - 1) {
+        const double g = R.org[d] + (double)(hi[d] + 1) * R.hcs[d] - c[d] - 0x1p-30 * R.hcs[d];
+        gap = g < gap ? g : gap;
+      }
+    }
+    const bool done = !live || !(gap < __builtin_inf()) || (gap > 0.0 && D < gap * gap);
+    if (__syncthreads_and(done)) {
+      if (live) {
+        const int i = ti[0] + R.dims[0] * (ti[1] + R.dims[1] * ti[2]);
+        const int64_t gi = (int64_t)R.node_off + i;
+        int* nb = A.nbr + gi * A.K;
+#pragma unroll
+        for (int j = 0; j < KT; ++j)
+          if (j < A.K) nb[j] = j < nf ? ki[j] : -1;
+        for (int j = KT; j < A.K; ++j) nb[j] = -1;
+        if (A.nbr_slot)
+          for (int j = 0; j < A.K; ++j) A.nbr_slot[gi * A.K + j] = (uint8_t)255;
+        A.nfound[gi] = nf;
+      }
+      return;
     }
   }
 }
@@ -2591,14 +2787,25 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
     build_hash(p->grid, p->search_mode, h);
   });
   std::vector<int64_t> table_off((size_t)n_req + 1, 0), cell_off((size_t)n_req + 1, 0),
-      item_off((size_t)n_req + 1, 0);
+      item_off((size_t)n_req + 1, 0), tile_off((size_t)n_req + 1, 0);
   for (int32_t q = 0; q < n_req; ++q) {
     table_off[(size_t)q + 1] = table_off[(size_t)q] + (int64_t)H[(size_t)q].table.size();
     cell_off[(size_t)q + 1] = cell_off[(size_t)q] + (int64_t)H[(size_t)q].hstart.size();
     item_off[(size_t)q + 1] = item_off[(size_t)q] + (int64_t)H[(size_t)q].hitems.size();
+    int ts[3];
+    tile_shape(H[(size_t)q].nd, ts);
+    int64_t tiles = 1;
+    for (int d = 0; d < H[(size_t)q].nd; ++d) tiles *= (problems[q].grid.dims[d] + ts[d] - 1) / ts[d];
+    tile_off[(size_t)q + 1] = tile_off[(size_t)q] + tiles;
   }
   const int64_t n_table = table_off.back(), n_cells = cell_off.back(), n_items = item_off.back();
   GSP_REQUIRE(n_items < ((int64_t)1 << 31), GSP_ERR_UNSUPPORTED, "data hash too large");
+  // the data-only tile search: every request searches data only (no hash
+  // cell lists expanded), kmax <= 16; GSP_TILE_SEARCH=0 keeps the wavefront
+  // search
+  bool tile_search = no_deps && K <= 16 && tile_off.back() < ((int64_t)1 << 31);
+  for (int32_t q = 0; q < n_req; ++q) tile_search = tile_search && H[(size_t)q].hexp == 0;
+  if (const char* env = std::getenv("GSP_TILE_SEARCH")) tile_search = tile_search && std::atoi(env) != 0;
 
   // concatenated device tables
   std::vector<Req> reqs((size_t)n_req);
@@ -2629,7 +2836,27 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       R.data_off = (int32_t)data_off[(size_t)q];
       R.table_off = t_off;
       R.cell_off = c_off;
+      R.tile_off = (int32_t)tile_off[(size_t)q];
       R.r2 = h.r2;
+      {
+        // first staging radius of the tile search: 1.5 times the radius that
+        // holds kmax data at the request's mean data density (a tile whose
+        // nodes are not all settled by it visits again with twice the radius)
+        double vol = 1.0, smin = g.spacing[0];
+        for (int d = 0; d < h.nd; ++d) {
+          vol *= (double)g.dims[d] * g.spacing[d];
+          smin = std::min(smin, g.spacing[d]);
+        }
+        const double pi = 3.141592653589793;
+        const double unit = h.nd == 1 ? 2.0 : (h.nd == 2 ? pi : 4.0 * pi / 3.0);
+        const double per = vol / (double)std::max<int64_t>(h.nD, 1);
+        double rho = 1.5 * std::pow((double)h.kmax * per / unit, 1.0 / h.nd);
+        if (h.r2 < std::numeric_limits<double>::infinity())
+          rho = std::min(rho, std::sqrt(h.r2) * 1.001);
+        if (!std::isfinite(rho)) rho = vol;
+        if (!(rho >= 0.5 * smin)) rho = 0.5 * smin;
+        R.trho = rho;
+      }
       R.alpha = h.alpha;
       R.prior_sigma = std::sqrt(p->model.sill);
       R.mean = p->mean;
@@ -2781,7 +3008,13 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
 
   {
     TimedLaunch tl(ctx, GSP_K_SEARCH);
-    if (ring)
+    if (tile_search) {
+      const dim3 tg((unsigned)tile_off.back());
+      if (K <= 8)
+        hipLaunchKernelGGL(k_search_tile<8>, tg, dim3(kTileNodes), 0, s, A, n_req);
+      else
+        hipLaunchKernelGGL(k_search_tile<16>, tg, dim3(kTileNodes), 0, s, A, n_req);
+    } else if (ring)
       hipLaunchKernelGGL(k_search<true>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
     else
       hipLaunchKernelGGL(k_search<false>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);

@@ -40,6 +40,11 @@ def data_case(seed, dims=(30, 24, 10), n_data=300, kmax=10, radius=0.0, kmin=1,
     dict(seed=9, n_data=200, kmax=20, spread=(-2.0, 3.0)),
     dict(seed=10, dims=(300,), n_data=17, kmax=6),
     dict(seed=11, dims=(20, 16), n_data=2000, kmax=12),
+    # the tile search (kmax <= 16): data far outside the grid, fewer data than
+    # kmax (tiles widen until every cell is visited), partial tiles on every axis
+    dict(seed=12, n_data=200, kmax=12, spread=(-2.0, 3.0)),
+    dict(seed=13, dims=(40, 36, 12), n_data=5, kmax=8),
+    dict(seed=14, dims=(17, 9, 5), n_data=400, kmax=16, family=O.EXPONENTIAL),
 ])
 def test_data_only_plan_matches_oracle(ctx, spec):
     case = data_case(**spec)
@@ -52,6 +57,36 @@ def test_data_only_plan_matches_oracle(ctx, spec):
     assert np.all(g["level"] == 0)
     z = np.zeros((1, case.n_nodes))
     np.testing.assert_array_equal(plan.sweep(z, case.data_vals), O.seqsim_sweep(case, o, z))
+
+
+TILE_SPECS = [dict(seed=1), dict(seed=4, kmax=12, radius=3.0, kmin=3), dict(seed=8, n_data=200, spread=(0.0, 0.2)),
+              dict(seed=10, dims=(300,), n_data=17, kmax=6), dict(seed=13, dims=(40, 36, 12), n_data=5, kmax=8)]
+
+
+@pytest.mark.parametrize("spec", TILE_SPECS)
+def test_tile_and_wavefront_search_agree(ctx, spec, monkeypatch):
+    """The data-only plans of kmax <= 16 search by tiles of nodes; the
+    wavefront search (GSP_TILE_SEARCH=0) must give the same plan, and both the
+    oracle's."""
+    case = data_case(**spec)
+    say("tile vs wavefront search", spec)
+    tile = _lib.SeqsimPlan(ctx, gpu_problem(case), keep=(case,)).export()
+    monkeypatch.setenv("GSP_TILE_SEARCH", "0")
+    wave = _lib.SeqsimPlan(ctx, gpu_problem(case), keep=(case,)).export()
+    o = O.seqsim_plan(case)
+    for k in ("nfound", "nbr", "lam", "sigma", "flags"):
+        np.testing.assert_array_equal(wave[k], getattr(o, k), err_msg=k)
+        np.testing.assert_array_equal(tile[k], wave[k], err_msg=k)
+
+
+def test_data_only_batch(ctx):
+    """Several data-only requests in one plan (1D, 2D and 3D grids, each its
+    own run of tiles): every request gives what the oracle gives for it alone."""
+    from test_batch_gpu import check_batch
+    cases = [data_case(21), data_case(22, dims=(64, 48), n_data=500, kmax=16),
+             data_case(23, dims=(300,), n_data=17, kmax=6),
+             data_case(24, dims=(17, 9, 5), n_data=400, kmax=12, radius=2.5)]
+    check_batch(ctx, cases, R=3, seed=2)
 
 
 def test_posterior_mean_matches_numpy_simple_kriging(ctx):
