@@ -458,7 +458,8 @@ __global__ __launch_bounds__(256, 8) void k_search(const PlanArgs A) {
 // node is final or every cell has been visited, so the result is the exact
 // top kmax under the order of k_search.
 constexpr int kTileNodes = 256;
-constexpr int kTileItems = 512;  // data staged in LDS per pass
+constexpr int kTileItems = 512;  // data staged in LDS per pass (two per thread)
+constexpr int kShells = 4;       // distance shells of the staging order
 
 __host__ __device__ inline void tile_shape(int ndim, int* t) {
   t[0] = ndim >= 3 ? 8 : (ndim == 2 ? 16 : 256);
@@ -474,6 +475,9 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
   __shared__ int s_id[kTileItems];
   __shared__ int s_rs[kTileNodes];       // first item of each row run
   __shared__ int s_pre[kTileNodes + 1];  // items before each row run
+  __shared__ int s_cnt[kShells];
+  __shared__ double s_xu[kTileItems][3];  // the items in run order, before the shells
+  __shared__ int s_idu[kTileItems];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x;
   // the request of this tile: the last one whose first tile is <= tile
@@ -493,8 +497,20 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
   const int ntx = (R.dims[0] + ts[0] - 1) / ts[0], nty = (R.dims[1] + ts[1] - 1) / ts[1];
   const int tl = tile - R.tile_off;
   const int t0[3] = {(tl % ntx) * ts[0], ((tl / ntx) % nty) * ts[1], (tl / (ntx * nty)) * ts[2]};
-  const int ti[3] = {t0[0] + tid % ts[0], t0[1] + (tid / ts[0]) % ts[1], t0[2] + tid / (ts[0] * ts[1])};
+  // each wavefront a compact block of nodes (4 x 4 x 4 in 3D, 8 x 8 in 2D)
+  const int w = tid >> 6, l = tid & 63;
+  const int ti[3] = {
+      t0[0] + (R.ndim >= 3 ? (w & 1) * 4 + (l & 3) : (R.ndim == 2 ? (w & 1) * 8 + (l & 7) : tid)),
+      t0[1] + (R.ndim >= 3 ? (w >> 1) * 4 + ((l >> 2) & 3) : (R.ndim == 2 ? (w >> 1) * 8 + (l >> 3) : 0)),
+      t0[2] + (R.ndim >= 3 ? l >> 4 : 0)};
   const bool live = ti[0] < R.dims[0] && ti[1] < R.dims[1] && ti[2] < R.dims[2];
+  // the tile's centre
+  double tc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int last = (t0[d] + ts[d] < R.dims[d] ? t0[d] + ts[d] : R.dims[d]) - 1;
+    tc[d] = R.org[d] + (0.5 * (double)(t0[d] + last) + 0.5) * R.sp[d];
+  }
   // the node's centre, as data_d2 forms it
   double c[3];
 #pragma unroll
@@ -531,6 +547,8 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
       lo[d] = (int)fl;
       hi[d] = (int)fh;
     }
+    // shells at 0.75, 1.25 and 1.75 rho from the tile's centre
+    const double sh2[3] = {0.5625 * rho * rho, 1.5625 * rho * rho, 3.0625 * rho * rho};
     // the cells lo[0]..hi[0] of one (cy, cz) row hold one run of items
     const int nyr = hi[1] - lo[1] + 1;
     const int nrows = nyr * (hi[2] - lo[2] + 1);
@@ -550,21 +568,51 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
       for (int i0 = 0; i0 < total; i0 += kTileItems) {
         __syncthreads();  // s_pre / s_rs written, the previous items consumed
         const int n = total - i0 < kTileItems ? total - i0 : kTileItems;
-        for (int u = tid; u < n; u += kTileNodes) {
-          const int t = i0 + u;
-          int a = 0, b = nr - 1;  // the run holding item t: last with s_pre <= t
-          while (a < b) {
-            const int m = (a + b + 1) >> 1;
-            if (s_pre[m] <= t) a = m;
-            else b = m - 1;
+        // the items go to LDS nearest shell first (shells of the distance
+        // to the tile's centre): the lanes' lists then fill with near data
+        // and most of the far data fail the first comparison
+        if (tid < kShells) s_cnt[tid] = 0;
+        __syncthreads();
+        int sh[2], slot[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int u = tid + h * kTileNodes;
+          sh[h] = -1;
+          if (u < n) {
+            const int t = i0 + u;
+            int a = 0, b = nr - 1;  // the run holding item t: last with s_pre <= t
+            while (a < b) {
+              const int m = (a + b + 1) >> 1;
+              if (s_pre[m] <= t) a = m;
+              else b = m - 1;
+            }
+            const int di = A.hitems[s_rs[a] + (t - s_pre[a])];
+            const double* x = A.data + (size_t)di * 3;
+            double q = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              const double xd = x[d];
+              s_xu[u][d] = xd;
+              const double v = d < R.ndim ? xd - tc[d] : 0.0;
+              q = q + v * v;
+            }
+            s_idu[u] = A.NT + di;
+            sh[h] = (q > sh2[0]) + (q > sh2[1]) + (q > sh2[2]);
+            slot[h] = atomicAdd(&s_cnt[sh[h]], 1);
           }
-          const int di = A.hitems[s_rs[a] + (t - s_pre[a])];
-          const double* x = A.data + (size_t)di * 3;
-          s_x[u][0] = x[0];
-          s_x[u][1] = x[1];
-          s_x[u][2] = x[2];
-          s_id[u] = A.NT + di;
         }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (sh[h] >= 0) {
+            const int u = tid + h * kTileNodes;
+            int o = slot[h];
+            for (int b = 0; b < sh[h]; ++b) o += s_cnt[b];
+            s_x[o][0] = s_xu[u][0];
+            s_x[o][1] = s_xu[u][1];
+            s_x[o][2] = s_xu[u][2];
+            s_id[o] = s_idu[u];
+          }
         __syncthreads();
         if (live) {
           for (int u = 0; u < n; ++u) {
@@ -3012,6 +3060,8 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
       const dim3 tg((unsigned)tile_off.back());
       if (K <= 8)
         hipLaunchKernelGGL(k_search_tile<8>, tg, dim3(kTileNodes), 0, s, A, n_req);
+      else if (K <= 12)
+        hipLaunchKernelGGL(k_search_tile<12>, tg, dim3(kTileNodes), 0, s, A, n_req);
       else
         hipLaunchKernelGGL(k_search_tile<16>, tg, dim3(kTileNodes), 0, s, A, n_req);
     } else if (ring)
