@@ -1804,10 +1804,15 @@ __global__ __launch_bounds__(1024) void k_levels_small(int32_t N, const int32_t*
     __syncthreads();  // the new level's order[] entries visible; s_wave free again
     for (int32_t t = tid; t < size; t += 1024) {
       const int a = order[start + t];
-      const int e1 = rowptr[a + 1];
-      for (int e = rowptr[a]; e < e1; ++e) {
-        const int j = deps[e];
-        atomicSub(&s_deg[j >> 2], 1u << (8 * (j & 3)));
+      const int e0 = rowptr[a], e1 = rowptr[a + 1];
+      // eight dependents' ids loaded together, then their decrements
+      for (int e = e0; e < e1; e += 8) {
+        int dj[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) dj[u] = e + u < e1 ? deps[e + u] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (dj[u] >= 0) atomicSub(&s_deg[dj[u] >> 2], 1u << (8 * (dj[u] & 3)));
       }
     }
     __syncthreads();
@@ -2623,15 +2628,20 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
                      B.deps.p);
   const int cb = grid_for(NT, 2048, 1024);
   int32_t depth = -1;
+  std::vector<int32_t> hcnt;  // the level sizes, when read together with the depth
   if (NT <= kSmallLevels) {
-    // one workgroup, one launch, one read of the depth
+    // one workgroup, one launch, one read of the depth and the first level sizes
     hipLaunchKernelGGL(k_levels_small, dim3(1), dim3(1024), 0, st, (int32_t)NT, B.rowptr.p,
                        B.deps.p, indeg, plan->level.p, plan->order.p, B.off.p, B.cnt.p,
                        B.depth.p);
+    const int64_t npre = NT + 16 < 1024 ? NT + 16 : 1024;
+    std::vector<int32_t> hpre((size_t)npre);
     int32_t d = 0;
     GSP_HIP(hipMemcpyAsync(&d, B.depth.p, 4, hipMemcpyDeviceToHost, st));
+    GSP_HIP(hipMemcpyAsync(hpre.data(), B.cnt.p, (size_t)npre * 4, hipMemcpyDeviceToHost, st));
     GSP_HIP(hipStreamSynchronize(st));
     depth = d;
+    if (depth >= 1 && depth <= npre) hcnt.assign(hpre.begin(), hpre.begin() + depth);
   } else {
   const int64_t n16 = (NT + 15) / 16;
   uint4* mark16 = reinterpret_cast<uint4*>(B.mark.p);
@@ -2661,9 +2671,11 @@ void build_levels(gsp_seqsim_plan* plan, const DepArgs& D, const int32_t* outdeg
   }
   }
   GSP_REQUIRE(depth >= 1, GSP_ERR_INTERNAL, "empty dependency levels");
-  std::vector<int32_t> hcnt((size_t)depth);
-  GSP_HIP(hipMemcpyAsync(hcnt.data(), B.cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, st));
-  GSP_HIP(hipStreamSynchronize(st));
+  if (hcnt.empty()) {
+    hcnt.resize((size_t)depth);
+    GSP_HIP(hipMemcpyAsync(hcnt.data(), B.cnt.p, (size_t)depth * 4, hipMemcpyDeviceToHost, st));
+    GSP_HIP(hipStreamSynchronize(st));
+  }
   plan->depth = depth;
   plan->level_off.assign((size_t)depth + 1, 0);
   for (int32_t l = 0; l < depth; ++l)
