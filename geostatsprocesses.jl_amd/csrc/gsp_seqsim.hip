@@ -444,6 +444,82 @@ __global__ __launch_bounds__(256, 8) void k_search(const PlanArgs A) {
   }
 }
 
+// ---- grid-only search, one node per lane ----------------------------------
+// The search of a plan of one request without data, bounded by its offset
+// table: node i's candidates are the first kmax in-grid offsets of the table
+// in (d2, index delta) order, of which the not yet simulated are then dropped
+// (GSP_KNN_THEN_MASK), or among which only simulated nodes count
+// (GSP_MASKED_KNN): k_search's lists, in its order. The 64 lanes of a
+// wavefront take 64 consecutive nodes and walk the table in step (one scalar
+// read per offset), so the position reads and the dependency counts of one
+// offset fall on consecutive addresses; the dependency counts take a second
+// walk once the node's count (and so whether it has dependencies) is known.
+__global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = g < A.NT;
+  const int i = live ? (int)g : 0;
+  const auto& R = const_view(A.reqs)[0];  // node_off 0
+  const int nx = R.dims[0], ny = R.dims[1], nz = R.dims[2];
+  const int ti[3] = {i % nx, (i / nx) % ny, i / (nx * ny)};
+  const int pi = live ? A.pos[i] : 0;
+  const int kmax = R.kmax;
+  const bool masked = R.mode == GSP_MASKED_KNN;
+  const Offset* tab = A.table + R.table_off;
+  int* nb = A.nbr + (int64_t)i * A.K;
+  uint8_t* ns = A.nbr_slot ? A.nbr_slot + (int64_t)i * A.K : nullptr;
+  int cnt = live ? 0 : kmax;  // candidates taken
+  int n = 0;                  // candidates kept
+  for (int t = 0; t < R.T; ++t) {
+    if (!__any(cnt < kmax)) break;
+    const auto& e = const_view(tab)[t];
+    const int ex = e.dx, ey = e.dy, ez = e.dz;
+    if (cnt < kmax) {
+      const int x = ti[0] + ex, y = ti[1] + ey, z = ti[2] + ez;
+      if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+        const int j = x + nx * (y + ny * z);
+        const bool before = A.pos[j] < pi;
+        if (!masked || before) ++cnt;
+        if (before) {
+          nb[n] = j;
+          if (ns) ns[n] = (uint8_t)(t < 255 ? t : 255);
+          ++n;
+        }
+      }
+    }
+  }
+  if (live) {
+    for (int m = n; m < A.K; ++m) {
+      nb[m] = -1;
+      if (ns) ns[m] = (uint8_t)255;
+    }
+    A.nfound[i] = n;
+  }
+  if (A.outdeg) {
+    // a node with fewer than kmin neighbours depends on nothing
+    const bool dep = live && n >= R.kmin;
+    int c = dep ? 0 : kmax;
+    for (int t = 0; t < R.T; ++t) {
+      if (!__any(c < kmax)) break;
+      const auto& e = const_view(tab)[t];
+      const int ex = e.dx, ey = e.dy, ez = e.dz;
+      if (c < kmax) {
+        const int x = ti[0] + ex, y = ti[1] + ey, z = ti[2] + ez;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+          const int j = x + nx * (y + ny * z);
+          const bool before = A.pos[j] < pi;
+          if (!masked || before) ++c;
+          if (before) atomicAdd(&A.outdeg[j], 1);
+        }
+      }
+    }
+    if (live) {
+      const int d = dep ? n : 0;
+      A.indeg[i] = d;
+      A.mark[i] = d == 0 ? 1 : 0;
+    }
+  }
+}
+
 // ---- data-only search, one node per lane ----------------------------------
 // The data-only search (the k nearest data of every node, no grid
 // neighbours, no dependencies) of plans whose requests all search data only,
@@ -2866,6 +2942,10 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
   bool tile_search = no_deps && K <= 16 && tile_off.back() < ((int64_t)1 << 31);
   for (int32_t q = 0; q < n_req; ++q) tile_search = tile_search && H[(size_t)q].hexp == 0;
   if (const char* env = std::getenv("GSP_TILE_SEARCH")) tile_search = tile_search && std::atoi(env) != 0;
+  // one request, no data, a bounded search: the grid-only search by lanes
+  bool grid_lane = !no_deps && !ring && n_req == 1 && nDT == 0 &&
+                   problems[0].search_mode != GSP_DATA_ONLY;
+  if (const char* env = std::getenv("GSP_GRID_LANE")) grid_lane = grid_lane && std::atoi(env) != 0;
 
   // concatenated device tables
   std::vector<Req> reqs((size_t)n_req);
@@ -3076,7 +3156,10 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
         hipLaunchKernelGGL(k_search_tile<12>, tg, dim3(kTileNodes), 0, s, A, n_req);
       else
         hipLaunchKernelGGL(k_search_tile<16>, tg, dim3(kTileNodes), 0, s, A, n_req);
-    } else if (ring)
+    } else if (grid_lane)
+      hipLaunchKernelGGL(k_search_grid_lane, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, s,
+                         A);
+    else if (ring)
       hipLaunchKernelGGL(k_search<true>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
     else
       hipLaunchKernelGGL(k_search<false>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);

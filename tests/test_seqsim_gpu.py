@@ -253,3 +253,25 @@ def test_errors(ctx):
     gp = gpu_plan(ctx, case)
     with pytest.raises(_lib.ArgumentError):
         gp.sweep(np.zeros((1, 5)), case.data_vals)
+
+
+@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
+def test_grid_lane_search(ctx, mode, monkeypatch):
+    """A 3D grid without data (48 x 40 x 24 = 46,080 nodes, kmax 24, anisotropic
+    ranges and spacing): the one-node-per-lane grid search gives the oracle's
+    neighbour lists, counts, flags, levels and weights bit for bit, and the
+    wavefront search's (GSP_GRID_LANE=0) the same lists and levels."""
+    rng = np.random.default_rng(91)
+    dims = (48, 40, 24)
+    N = int(np.prod(dims))
+    case = O.SeqsimCase(dims=dims, spacing=(1.0, 1.5, 0.5), family=O.EXPONENTIAL,
+                        ranges=(10.0, 7.0, 3.0), nugget=0.02, kmax=24,
+                        radius=9.0 if mode == O.MASKED_KNN else 0.0,
+                        data_coords=np.zeros((0, 3)), data_vals=np.zeros(0),
+                        path=rng.permutation(N), mode=mode)
+    gp, o = check_plan(ctx, case)
+    monkeypatch.setenv("GSP_GRID_LANE", "0")
+    w = gpu_plan(ctx, case).export()
+    g = gp.export()
+    for key in ("nfound", "nbr", "flags", "level", "lam", "sigma"):
+        np.testing.assert_array_equal(g[key], w[key], err_msg=key)
