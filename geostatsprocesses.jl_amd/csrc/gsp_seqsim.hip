@@ -452,8 +452,9 @@ __global__ __launch_bounds__(256, 8) void k_search(const PlanArgs A) {
 // (GSP_MASKED_KNN): k_search's lists, in its order. The 64 lanes of a
 // wavefront take 64 consecutive nodes and walk the table in step (one scalar
 // read per offset), so the position reads and the dependency counts of one
-// offset fall on consecutive addresses; the dependency counts take a second
-// walk once the node's count (and so whether it has dependencies) is known.
+// offset fall on consecutive addresses; eight offsets' position reads are in
+// flight together. The dependency counts take a second walk once the node's
+// count (and so whether it has dependencies) is known.
 __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = g < A.NT;
@@ -467,25 +468,38 @@ __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
   const Offset* tab = A.table + R.table_off;
   int* nb = A.nbr + (int64_t)i * A.K;
   uint8_t* ns = A.nbr_slot ? A.nbr_slot + (int64_t)i * A.K : nullptr;
+  // eight offsets at a time: their in-grid targets and the targets' positions
+  // (loads in flight together), then the decisions in table order
+  auto block = [&](int t0, bool want, int* jj, int* pj) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int t = t0 + u < R.T ? t0 + u : R.T - 1;
+      const auto& e = const_view(tab)[t];
+      const int x = ti[0] + e.dx, y = ti[1] + e.dy, z = ti[2] + e.dz;
+      const bool in = want && t0 + u < R.T && x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 &&
+                      z < nz;
+      jj[u] = in ? x + nx * (y + ny * z) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pj[u] = jj[u] >= 0 ? A.pos[jj[u]] : 0;
+  };
   int cnt = live ? 0 : kmax;  // candidates taken
   int n = 0;                  // candidates kept
-  for (int t = 0; t < R.T; ++t) {
+  for (int t0 = 0; t0 < R.T; t0 += 8) {
     if (!__any(cnt < kmax)) break;
-    const auto& e = const_view(tab)[t];
-    const int ex = e.dx, ey = e.dy, ez = e.dz;
-    if (cnt < kmax) {
-      const int x = ti[0] + ex, y = ti[1] + ey, z = ti[2] + ez;
-      if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
-        const int j = x + nx * (y + ny * z);
-        const bool before = A.pos[j] < pi;
+    int jj[8], pj[8];
+    block(t0, cnt < kmax, jj, pj);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (jj[u] >= 0 && cnt < kmax) {
+        const bool before = pj[u] < pi;
         if (!masked || before) ++cnt;
         if (before) {
-          nb[n] = j;
-          if (ns) ns[n] = (uint8_t)(t < 255 ? t : 255);
+          nb[n] = jj[u];
+          if (ns) ns[n] = (uint8_t)(t0 + u < 255 ? t0 + u : 255);
           ++n;
         }
       }
-    }
   }
   if (live) {
     for (int m = n; m < A.K; ++m) {
@@ -498,19 +512,17 @@ __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
     // a node with fewer than kmin neighbours depends on nothing
     const bool dep = live && n >= R.kmin;
     int c = dep ? 0 : kmax;
-    for (int t = 0; t < R.T; ++t) {
+    for (int t0 = 0; t0 < R.T; t0 += 8) {
       if (!__any(c < kmax)) break;
-      const auto& e = const_view(tab)[t];
-      const int ex = e.dx, ey = e.dy, ez = e.dz;
-      if (c < kmax) {
-        const int x = ti[0] + ex, y = ti[1] + ey, z = ti[2] + ez;
-        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
-          const int j = x + nx * (y + ny * z);
-          const bool before = A.pos[j] < pi;
+      int jj[8], pj[8];
+      block(t0, c < kmax, jj, pj);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (jj[u] >= 0 && c < kmax) {
+          const bool before = pj[u] < pi;
           if (!masked || before) ++c;
-          if (before) atomicAdd(&A.outdeg[j], 1);
+          if (before) atomicAdd(&A.outdeg[jj[u]], 1);
         }
-      }
     }
     if (live) {
       const int d = dep ? n : 0;
