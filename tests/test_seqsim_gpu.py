@@ -255,21 +255,27 @@ def test_errors(ctx):
         gp.sweep(np.zeros((1, 5)), case.data_vals)
 
 
-@pytest.mark.parametrize("mode", [O.KNN_THEN_MASK, O.MASKED_KNN])
-def test_grid_lane_search(ctx, mode, monkeypatch):
+@pytest.mark.parametrize("mode,kmin", [(O.KNN_THEN_MASK, 1), (O.MASKED_KNN, 1),
+                                       (O.KNN_THEN_MASK, 6)])
+def test_grid_lane_search(ctx, mode, kmin, monkeypatch):
     """A 3D grid without data (48 x 40 x 24 = 46,080 nodes, kmax 24, anisotropic
     ranges and spacing): the one-node-per-lane grid search gives the oracle's
     neighbour lists, counts, flags, levels and weights bit for bit, and the
-    wavefront search's (GSP_GRID_LANE=0) the same lists and levels."""
+    wavefront search's (GSP_GRID_LANE=0) the same lists and levels. With
+    kmin 6, nodes with 1..5 simulated neighbours draw from the prior and have
+    no dependencies: the dependency counts their walk added are taken back."""
     rng = np.random.default_rng(91)
     dims = (48, 40, 24)
     N = int(np.prod(dims))
     case = O.SeqsimCase(dims=dims, spacing=(1.0, 1.5, 0.5), family=O.EXPONENTIAL,
-                        ranges=(10.0, 7.0, 3.0), nugget=0.02, kmax=24,
+                        ranges=(10.0, 7.0, 3.0), nugget=0.02, kmax=24, kmin=kmin,
                         radius=9.0 if mode == O.MASKED_KNN else 0.0,
                         data_coords=np.zeros((0, 3)), data_vals=np.zeros(0),
                         path=rng.permutation(N), mode=mode)
     gp, o = check_plan(ctx, case)
+    if kmin > 1:
+        few = (o.nfound > 0) & (o.nfound < kmin)
+        assert np.count_nonzero(few) > 0
     monkeypatch.setenv("GSP_GRID_LANE", "0")
     w = gpu_plan(ctx, case).export()
     g = gp.export()
