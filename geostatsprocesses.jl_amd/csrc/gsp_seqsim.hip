@@ -455,7 +455,12 @@ __global__ __launch_bounds__(256, 8) void k_search(const PlanArgs A) {
 // offset fall on consecutive addresses; eight offsets' position reads are in
 // flight together. The dependency counts take a second walk once the node's
 // count (and so whether it has dependencies) is known.
+// STAGE: each lane writes its row (and slots) to LDS, and the workgroup then
+// stores its 256 rows as one contiguous run (dynamic LDS of 256 K ints and
+// 256 K bytes).
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
+  extern __shared__ int s_dyn[];
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = g < A.NT;
   const int i = live ? (int)g : 0;
@@ -466,8 +471,10 @@ __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
   const int kmax = R.kmax;
   const bool masked = R.mode == GSP_MASKED_KNN;
   const Offset* tab = A.table + R.table_off;
-  int* nb = A.nbr + (int64_t)i * A.K;
-  uint8_t* ns = A.nbr_slot ? A.nbr_slot + (int64_t)i * A.K : nullptr;
+  uint8_t* s_slot = reinterpret_cast<uint8_t*>(s_dyn + 256 * A.K);
+  int* nb = STAGE ? s_dyn + threadIdx.x * A.K : A.nbr + (int64_t)i * A.K;
+  uint8_t* ns = !A.nbr_slot ? nullptr
+                            : (STAGE ? s_slot + threadIdx.x * A.K : A.nbr_slot + (int64_t)i * A.K);
   // eight offsets at a time: their in-grid targets and the targets' positions
   // (loads in flight together), then the decisions in table order
   auto block = [&](int t0, bool want, int* jj, int* pj) {
@@ -507,6 +514,18 @@ __global__ __launch_bounds__(256) void k_search_grid_lane(const PlanArgs A) {
       if (ns) ns[m] = (uint8_t)255;
     }
     A.nfound[i] = n;
+  }
+  if constexpr (STAGE) {
+    __syncthreads();
+    const int64_t first = (int64_t)blockIdx.x * 256;
+    const int64_t rem = (int64_t)A.NT - first;
+    const int words = (int)(rem < 256 ? rem : 256) * A.K;
+    int* dst = A.nbr + first * A.K;
+    for (int q = threadIdx.x; q < words; q += 256) dst[q] = s_dyn[q];
+    if (A.nbr_slot) {
+      uint8_t* dsl = A.nbr_slot + first * A.K;
+      for (int q = threadIdx.x; q < words; q += 256) dsl[q] = s_slot[q];
+    }
   }
   if (A.outdeg) {
     // a node with fewer than kmin neighbours depends on nothing
@@ -3168,10 +3187,13 @@ void seqsim_plan_build(gsp_ctx* ctx, int32_t n_req, const gsp_seqsim_problem* pr
         hipLaunchKernelGGL(k_search_tile<12>, tg, dim3(kTileNodes), 0, s, A, n_req);
       else
         hipLaunchKernelGGL(k_search_tile<16>, tg, dim3(kTileNodes), 0, s, A, n_req);
-    } else if (grid_lane)
-      hipLaunchKernelGGL(k_search_grid_lane, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, s,
-                         A);
-    else if (ring)
+    } else if (grid_lane) {
+      const dim3 lg((unsigned)((NT + 255) / 256));
+      if (K <= 32)
+        hipLaunchKernelGGL(k_search_grid_lane<true>, lg, dim3(256), (size_t)256 * K * 5, s, A);
+      else
+        hipLaunchKernelGGL(k_search_grid_lane<false>, lg, dim3(256), 0, s, A);
+    } else if (ring)
       hipLaunchKernelGGL(k_search<true>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
     else
       hipLaunchKernelGGL(k_search<false>, dim3(grid_for(NT, kWaves, 16384)), dim3(256), 0, s, A);
