@@ -2017,22 +2017,27 @@ __global__ __launch_bounds__(256) void k_level_place(int64_t n16, int32_t L, uin
 
 // ------------------------------------------------------------- the sweep --
 // Zt[q * RC + r] = z[(r0 + r) * n_path + q]: normals of one realization chunk
-// made node-major (path order) through a 64-position LDS tile.
+// made node-major (path order) through a 64-position LDS tile of RC rows
+// (dynamic LDS, RC x 65 doubles: a small chunk leaves room for more
+// workgroups per CU).
 __global__ __launch_bounds__(256) void k_z_transpose(const double* z, int32_t n_path, int32_t RC,
                                                      int32_t r0, double* Zt) {
-  __shared__ double tile[64][65];
+  extern __shared__ double s_ttile[];
   const int64_t q0 = (int64_t)blockIdx.x * 64;
   const int cnt = (int)((n_path - q0) < 64 ? (n_path - q0) : 64);
   for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
     const int rr = t / cnt, qq = t - rr * cnt;
-    tile[rr][qq] = z[(int64_t)(r0 + rr) * n_path + q0 + qq];
+    s_ttile[rr * 65 + qq] = z[(int64_t)(r0 + rr) * n_path + q0 + qq];
   }
   __syncthreads();
   for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
     const int qq = t / RC, rr = t - qq * RC;
-    Zt[(q0 + qq) * RC + rr] = tile[rr][qq];
+    Zt[(q0 + qq) * RC + rr] = s_ttile[rr * 65 + qq];
   }
 }
+
+// LDS bytes of the transposes' tiles for a chunk of RC realizations.
+inline size_t transpose_lds(int32_t RC) { return (size_t)RC * 65 * sizeof(double); }
 
 struct LevelArgs {
   const int32_t* order;  // the nodes of this level
@@ -2223,22 +2228,23 @@ __global__ void k_data_rows(double* X, int32_t N, int32_t n_data, int32_t RC,
     X[(int64_t)N * RC + t] = vals[t / RC];
 }
 
-// out[(r0 + r) * N + i] = X[i * RC + r] through a 64-node LDS tile (int32
-// outputs: the indicator sweep's categories, held as exact doubles in X).
+// out[(r0 + r) * N + i] = X[i * RC + r] through a 64-node LDS tile of RC rows
+// (dynamic LDS, RC x 65 doubles; int32 outputs: the indicator sweep's
+// categories, held as exact doubles in X).
 template <class OUT>
 __global__ __launch_bounds__(256) void k_transpose(const double* X, int32_t N, int32_t RC,
                                                    int32_t r0, OUT* out) {
-  __shared__ double tile[64][65];
+  extern __shared__ double s_ttile[];
   const int64_t i0 = (int64_t)blockIdx.x * 64;
   const int cnt = (int)((N - i0) < 64 ? (N - i0) : 64);
   for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
     const int ii = t / RC, rr = t - ii * RC;
-    tile[rr][ii] = X[(i0 + ii) * RC + rr];
+    s_ttile[rr * 65 + ii] = X[(i0 + ii) * RC + rr];
   }
   __syncthreads();
   for (int t = threadIdx.x; t < cnt * RC; t += blockDim.x) {
     const int rr = t / cnt, ii = t - rr * cnt;
-    out[(int64_t)(r0 + rr) * N + i0 + ii] = (OUT)tile[rr][ii];
+    out[(int64_t)(r0 + rr) * N + i0 + ii] = (OUT)s_ttile[rr * 65 + ii];
   }
 }
 
@@ -3486,7 +3492,8 @@ void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, con
                          Xs, (int32_t)N, (int32_t)nD, RC, data_vals);
     {
       TimedLaunch tl(tctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, z,
+      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256),
+                         transpose_lds(RC), s, z,
                          (int32_t)N, RC, r0, Zts);
     }
     LevelArgs A;
@@ -3518,7 +3525,8 @@ void seqsim_sweep_into(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32_t R, con
     }
     {
       TimedLaunch tl(tctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, Xs,
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((N + 63) / 64)), dim3(256),
+                         transpose_lds(RC), s, Xs,
                          (int32_t)N, RC, r0, out);
     }
     GSP_HIP(hipGetLastError());
@@ -3617,7 +3625,8 @@ void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32
                          ctx->X.p, (int32_t)N, (int32_t)nD, RC, dcat.p);
     {
       TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s, u,
+      hipLaunchKernelGGL(k_z_transpose, dim3((unsigned)((N + 63) / 64)), dim3(256),
+                         transpose_lds(RC), s, u,
                          (int32_t)N, RC, r0, ctx->Zt.p);
     }
     LevelArgs& A = I.L;
@@ -3649,7 +3658,8 @@ void seqsim_sweep_indicator_run(gsp_ctx* ctx, const gsp_seqsim_plan* plan, int32
     }
     {
       TimedLaunch tl(ctx, GSP_K_TRANSPOSE);
-      hipLaunchKernelGGL(k_transpose<int32_t>, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_transpose<int32_t>, dim3((unsigned)((N + 63) / 64)), dim3(256),
+                         transpose_lds(RC), s,
                          ctx->X.p, (int32_t)N, RC, r0, out);
     }
     GSP_HIP(hipGetLastError());
