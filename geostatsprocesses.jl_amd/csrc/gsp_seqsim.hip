@@ -583,8 +583,6 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
   __shared__ int s_rs[kTileNodes];       // first item of each row run
   __shared__ int s_pre[kTileNodes + 1];  // items before each row run
   __shared__ int s_cnt[kShells];
-  __shared__ double s_xu[kTileItems][3];  // the items in run order, before the shells
-  __shared__ int s_idu[kTileItems];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x;
   // the request of this tile: the last one whose first tile is <= tile
@@ -680,7 +678,8 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
         // and most of the far data fail the first comparison
         if (tid < kShells) s_cnt[tid] = 0;
         __syncthreads();
-        int sh[2], slot[2];
+        int sh[2], slot[2], idr[2];
+        double xr[2][3];  // this thread's two items, until their slots are known
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int u = tid + h * kTileNodes;
@@ -699,11 +698,11 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
               const double xd = x[d];
-              s_xu[u][d] = xd;
+              xr[h][d] = xd;
               const double v = d < R.ndim ? xd - tc[d] : 0.0;
               q = q + v * v;
             }
-            s_idu[u] = A.NT + di;
+            idr[h] = A.NT + di;
             sh[h] = (q > sh2[0]) + (q > sh2[1]) + (q > sh2[2]);
             slot[h] = atomicAdd(&s_cnt[sh[h]], 1);
           }
@@ -712,13 +711,12 @@ __global__ __launch_bounds__(kTileNodes) void k_search_tile(const PlanArgs A, in
 #pragma unroll
         for (int h = 0; h < 2; ++h)
           if (sh[h] >= 0) {
-            const int u = tid + h * kTileNodes;
             int o = slot[h];
             for (int b = 0; b < sh[h]; ++b) o += s_cnt[b];
-            s_x[o][0] = s_xu[u][0];
-            s_x[o][1] = s_xu[u][1];
-            s_x[o][2] = s_xu[u][2];
-            s_id[o] = s_idu[u];
+            s_x[o][0] = xr[h][0];
+            s_x[o][1] = xr[h][1];
+            s_x[o][2] = xr[h][2];
+            s_id[o] = idr[h];
           }
         __syncthreads();
         if (live) {
